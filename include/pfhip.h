@@ -1,0 +1,178 @@
+/* pfhip.h -- C ABI of the pyfourier_amd NUFFT engine (MI355X / gfx950).
+ *
+ * The reference (INFN-MRI/pyfourier) is pure Python; its plug-in points for
+ * this path are Python functions and kernel dispatch tables.  Each entry
+ * point below names the reference interface it replaces (paths relative to
+ * the reference repository root).
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (hipMalloc / torch CUDA tensors)
+ *     unless marked "host".  complex data are interleaved complex64
+ *     (float re, float im), passed as void*.
+ *   - Every call only enqueues work on `stream` (a hipStream_t passed as
+ *     void*, NULL = default stream) and returns without synchronising, except
+ *     pf_plan_create which synchronises once when the plan is built.
+ *   - Every call returns 0 on success or a PF_ERR_* code; pf_last_error()
+ *     returns a thread-local message for the last failure.  Arguments are
+ *     validated on the host before any launch.
+ *   - A plan is immutable after creation; calls on one plan may be issued
+ *     from several threads, but calls that overlap in time on different
+ *     streams share the plan's rocFFT work buffer and must be serialised by
+ *     the caller.
+ *
+ * Shapes (reference conventions, src/pyfourier/_nufft/_fwd.py:42-126):
+ *   coordinates  (F, npts, ndim) float32 in grid units [-N/2, N/2), last axis
+ *                ordered (x, y, z); shape/width/oversamp ordered (z, y, x)
+ *   image        (C, nimg, [Nz], Ny, Nx)   nimg = F (plain), K (subspace)
+ *   k-space      (C, F, npts)
+ */
+#ifndef PFHIP_H
+#define PFHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_OK 0
+#define PF_ERR_ARG 1
+#define PF_ERR_HIP 2
+#define PF_ERR_FFT 3
+#define PF_ERR_ALLOC 4
+#define PF_ERR_UNSUPPORTED 5
+
+#define PF_NORM_NONE 0  /* forward unscaled, adjoint 1/Ghat (reference norm=None) */
+#define PF_NORM_ORTHO 1 /* both 1/sqrt(Ghat) (reference norm="ortho") */
+/* The exact adjoint of the *other* direction under PF_NORM_NONE: the forward
+ * scaled by 1/Ghat (adjoint of nufft_adj) or the adjoint unscaled (adjoint of
+ * nufft).  Used by autograd backward passes (reference defect A.22 fixed). */
+#define PF_NORM_ADJOINT 2
+
+typedef struct pf_plan pf_plan;
+
+/* Batch description of one operator call. */
+typedef struct pf_batch_desc {
+    int batch;                /* C: coils x slices x ... (>= 1) */
+    int nseg;                 /* L: B0 time segments, 1 = no field map */
+    int ncoeff;               /* K: subspace coefficients, 0 = no basis */
+    const void* basis;        /* (F, K) complex64, required when ncoeff > 0 */
+    const void* b0_temporal;  /* (L, nt) complex64, required when nseg > 1 */
+    const void* b0_spatial;   /* (L, Nz*Ny*Nx) complex64, required when nseg > 1 */
+    int nt;                   /* readout length of b0_temporal; the readout
+                                 index of sample j of a frame is j % nt */
+} pf_batch_desc;
+
+/* Plan summary (host struct). */
+typedef struct pf_plan_info {
+    int ndim;
+    int shape[3];    /* (z, y, x), leading 1 for ndim < 3 */
+    int os_shape[3]; /* ceil(oversamp * shape) */
+    int width[3];
+    float beta[3];
+    double scale;    /* kernel-sum normalisation */
+    int nframes;
+    long long npts;
+    int tile[3];
+    long long spread_entries;
+    int spread_items;
+    int spread_slots;
+} pf_plan_info;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pf_last_error(void);
+
+/* ABI version (major*100 + minor). */
+int pf_version(void);
+
+/* Build a plan on the current HIP device.
+ * Replaces plan_nufft / Interpolator
+ *   src/pyfourier/_nufft/_plan.py:20-199 and
+ *   src/pyfourier/_subroutines/_grid/_plan.py:14-114,192-210.
+ * coord:   (F*npts, ndim) float32, grid units (already normalised; the
+ *          reference normalisation _subroutines/_traj.py:10-54 is done by the
+ *          caller)
+ * shape, width, oversamp: host arrays of ndim entries, (z, y, x) order; a
+ *          Cartesian axis is passed with width 1 and oversamp 1.0
+ *          (src/pyfourier/_nufft/_plan.py:139-146). */
+int pf_plan_create(const float* coord, int ndim, const int* shape, const int* width,
+                   const float* oversamp, int nframes, long long npts, void* stream,
+                   pf_plan** out);
+
+/* Release a plan and its device buffers (src/pyfourier/_nufft/_plan.py:203-225). */
+int pf_plan_destroy(pf_plan* plan);
+
+/* Fill a host pf_plan_info. */
+int pf_plan_get_info(const pf_plan* plan, pf_plan_info* info);
+
+/* Tile-sorted order of the plan's samples: copies (F*npts) int32 sorted
+ * position -> original sample index into the device buffer `perm`. */
+int pf_plan_get_permutation(const pf_plan* plan, int* perm, void* stream);
+
+/* Workspace bytes that pf_nufft (op=0) or pf_nufft_adj (op=1) need for the
+ * batch `desc` (device memory the caller allocates; 256-byte aligned). */
+int pf_workspace_size(const pf_plan* plan, const pf_batch_desc* desc, int op, size_t* bytes);
+
+/* Forward NUFFT  y = weight * (1/scale) * Gather(FFT(pad(apod * x))).
+ * Replaces _nufft_fwd/_do_nufft src/pyfourier/_nufft/_nufft.py:11-90,218-251
+ * (with the B0 segment loop, defect A.13 fixed) behind nufft()
+ * src/pyfourier/_nufft/_fwd.py:18-230.
+ * image (C, nimg, N...) -> kspace (C, F, npts); weight: (F*npts) float32 or
+ * NULL; norm: PF_NORM_*. */
+int pf_nufft(pf_plan* plan, const pf_batch_desc* desc, const void* image, void* kspace,
+             const float* weight, int norm, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Adjoint NUFFT  x = apod * crop(IFFT((1/scale) * Spread(weight * y))).
+ * Replaces _nufft_adj/_do_nufft_adj src/pyfourier/_nufft/_nufft.py:93-169,254-287
+ * behind nufft_adj() src/pyfourier/_nufft/_adj.py:18-216.  Deterministic:
+ * run-to-run bit-identical (no global float atomics). */
+int pf_nufft_adj(pf_plan* plan, const pf_batch_desc* desc, const void* kspace, void* image,
+                 const float* weight, int norm, void* workspace, size_t workspace_bytes,
+                 void* stream);
+
+/* ---- stage entry points (the reference's kernel plug-in points) ------- */
+
+/* Grid bytes for one call: nfg*C*R*prod(os_shape)*8. */
+int pf_grid_size(const pf_plan* plan, const pf_batch_desc* desc, size_t* bytes);
+
+/* Deapodise, zero-pad and ifftshift into the oversampled grid.
+ * Replaces _apodize + _resize + ifftshift, src/pyfourier/_nufft/_nufft.py:234-238. */
+int pf_deapod_pad(const pf_plan* plan, const pf_batch_desc* desc, const void* image, void* grid,
+                  void* stream);
+
+/* Unnormalised in-place FFT of the grid (inverse != 0: e^{+}).
+ * Replaces fft/ifft src/pyfourier/_subroutines/_fft.py:11-194 (torch branch). */
+int pf_fft(pf_plan* plan, const pf_batch_desc* desc, void* grid, int inverse, void* stream);
+
+/* Gather (forward interpolation) into tile-sorted samples (F*npts, C).
+ * Replaces the _degrid dispatch table
+ * src/pyfourier/_subroutines/_grid/_cart2noncart/__init__.py:17-18,62-91. */
+int pf_interp(const pf_plan* plan, const pf_batch_desc* desc, const void* grid, void* ksp_sorted,
+              void* stream);
+
+/* Deterministic spread (adjoint interpolation) of tile-sorted samples into the
+ * grid.  Replaces the _grid dispatch table
+ * src/pyfourier/_subroutines/_grid/_noncart2cart/__init__.py:17-18,71-100.
+ * workspace: pf_spread_workspace_size() bytes (may be NULL when 0). */
+int pf_spread(const pf_plan* plan, const pf_batch_desc* desc, const void* ksp_sorted, void* grid,
+              void* workspace, void* stream);
+int pf_spread_workspace_size(const pf_plan* plan, const pf_batch_desc* desc, size_t* bytes);
+
+/* Crop, fftshift and deapodise (times `factor`).
+ * Replaces _resize + _apodize, src/pyfourier/_nufft/_nufft.py:278-282. */
+int pf_crop_deapod(const pf_plan* plan, const pf_batch_desc* desc, const void* grid, void* image,
+                   float factor, void* stream);
+
+/* User order (C, F, npts) <-> tile-sorted (F*npts, C), weight fused.
+ * Replace the transpose-copies of _cart2noncart/__init__.py:53-54,101-105
+ * and _noncart2cart/__init__.py:59-60. */
+int pf_sort_samples(const pf_plan* plan, const pf_batch_desc* desc, const void* kspace,
+                    void* ksp_sorted, const float* weight, void* stream);
+int pf_unsort_samples(const pf_plan* plan, const pf_batch_desc* desc, const void* ksp_sorted,
+                      void* kspace, const float* weight, float factor, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFHIP_H */

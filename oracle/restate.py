@@ -1,0 +1,461 @@
+"""CPU restatement of pyfourier's NUFFT path (numpy).
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: it may be imported by
+``tests/``, by ``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of
+``bench.py``, and by nothing else.  The product (``pyfourier_amd``) never
+imports, calls or links it.
+
+It restates the *intended* algorithm of the reference (paths relative to the
+reference repository root):
+
+* plan parameters      src/pyfourier/_nufft/_plan.py:117-155
+* KB kernel            src/pyfourier/_subroutines/_grid/_plan.py:157-189
+* tap tables           src/pyfourier/_subroutines/_grid/_plan.py:14-91,192-210
+* kernel scale         src/pyfourier/_subroutines/_grid/_plan.py:118-141
+* coordinate scaling   src/pyfourier/_subroutines/_traj.py:57-66
+* normalisation        src/pyfourier/_subroutines/_traj.py:10-54 (intent: one
+                       global radius, not each point's own radius)
+* apodisation          src/pyfourier/_subroutines/_apod.py:16-46
+* zero-pad / crop      src/pyfourier/_subroutines/_resize.py:16-42
+* centered FFT         src/pyfourier/_subroutines/_fft.py:72-79,163-170
+* gather               src/pyfourier/_subroutines/_grid/_cart2noncart/_{1,2,3}D.py
+* spread               src/pyfourier/_subroutines/_grid/_noncart2cart/_{1,2,3}D.py
+* subspace gather      src/pyfourier/_subroutines/_grid/_cart2noncart/_2D_subspace.py:13-54
+* subspace spread      src/pyfourier/_subroutines/_grid/_noncart2cart/_2D_subspace.py:13-53
+* operator pipeline    src/pyfourier/_nufft/_nufft.py:11-169,218-287
+* B0 segmentation      src/pyfourier/_subroutines/_b0.py:10-135
+* Toeplitz Gram        src/pyfourier/_sparsegram/_plan.py:108-249,
+                       src/pyfourier/_sparsegram/_sparsegram.py:11-81
+
+Arrays follow the reference conventions: coordinates ``(F, M, d)`` with the
+last axis ordered (x, y, z); grid/width/oversampling ordered (z, y, x).
+
+How each function is pinned (independently of this file) is listed in
+``tests/test_oracle.py``; functions with no independent pin are marked
+"parity unpinned" below.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+# --------------------------------------------------------------------------
+# Kaiser-Bessel kernel: I0(beta*sqrt(1-x^2)) with the Abramowitz & Stegun
+# 9.8.1 (|u| < 3.75) and 9.8.2 (u >= 3.75) polynomial approximations, zero for
+# |x| > 1.  Follows src/pyfourier/_subroutines/_grid/_plan.py:157-189.
+# --------------------------------------------------------------------------
+_AS_SMALL = (1.0, 3.5156229, 3.0899424, 1.2067492, 0.2659732, 0.0360768, 0.0045813)
+_AS_LARGE = (
+    0.39894228,
+    0.01328592,
+    0.00225319,
+    -0.00157565,
+    0.00916281,
+    -0.02057706,
+    0.02635537,
+    -0.01647633,
+    0.00392377,
+)
+
+
+def kb(x, beta):
+    """KB kernel value at normalised offset ``x`` (float64)."""
+    x = np.asarray(x, dtype=np.float64)
+    beta = float(beta)
+    inside = np.abs(x) <= 1.0
+    u = beta * np.sqrt(np.where(inside, 1.0 - x * x, 0.0))
+    t = u / 3.75
+    t2 = t * t
+    small = 0.0
+    for c in reversed(_AS_SMALL):
+        small = small * t2 + c
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        it = np.where(u >= 3.75, 1.0 / np.where(t > 0, t, 1.0), 0.0)
+        large = 0.0
+        for c in reversed(_AS_LARGE):
+            large = large * it + c
+        large = np.where(u >= 3.75, u ** -0.5 * np.exp(u) * large, 0.0)
+    val = np.where(u < 3.75, small, large)
+    return np.where(inside, val, 0.0)
+
+
+# --------------------------------------------------------------------------
+# Plan parameters: src/pyfourier/_nufft/_plan.py:117-155
+# --------------------------------------------------------------------------
+def kb_beta(width, oversamp):
+    """beta = pi*sqrt((W/s*(s-1/2))^2 - 0.8), evaluated in float32 like the
+    reference (width int16, oversamp float32): _nufft/_plan.py:128."""
+    w = np.asarray(width, dtype=np.int16)
+    s = np.asarray(oversamp, dtype=np.float32)
+    inner = ((w / s) * (s - np.float32(0.5))) ** 2 - np.float32(0.8)
+    return (np.float32(math.pi) * inner.astype(np.float32) ** np.float32(0.5)).astype(
+        np.float32
+    )
+
+
+def os_shape(shape, oversamp):
+    """ceil(sigma*N) per axis: _subroutines/_utils/_common.py:16-18 (intent)."""
+    s = np.asarray(oversamp, dtype=np.float32)
+    n = np.asarray(shape, dtype=np.int64)
+    return [int(v) for v in np.ceil(s * n.astype(np.float32))]
+
+
+def kernel_scale(beta, width):
+    """Product over axes of sum_e KB((ceil(-W/2)+e)/(W/2)); 1 on W=1 axes.
+    _subroutines/_grid/_plan.py:118-141."""
+    total = 1.0
+    for b, w in zip(beta, width):
+        w = int(w)
+        if w == 1:
+            continue
+        start = math.ceil(-w / 2)
+        total *= float(np.sum([kb((start + e) / (w / 2), b) for e in range(w)]))
+    return total
+
+
+def normalize_coordinates(coord, shape_xyz):
+    """Map coordinates of any unit to grid units [-N/2, N/2).
+
+    Intent of _subroutines/_traj.py:10-54 with defect A.11 fixed: a single
+    global radius (max |k| over all points) is mapped to N/2, instead of each
+    point being divided by its own radius.  For [-pi, pi) input this is
+    k*N/(2*pi).
+    """
+    coord = np.asarray(coord, dtype=np.float32)
+    shape_xyz = np.asarray(shape_xyz, dtype=np.float32)
+    cabs = float(np.sqrt((coord.astype(np.float64) ** 2).sum(axis=-1)).max())
+    if cabs == 0.0:
+        return coord.copy()
+    if (coord < 0).any():
+        return (np.float32(0.5) * coord / np.float32(cabs)) * shape_xyz
+    return (coord / np.float32(cabs)) * shape_xyz - np.floor(shape_xyz / 2)
+
+
+def scale_coord(coord, shape_xyz, oversamp_xyz):
+    """c_os = c*ceil(sN)/N + ceil(sN)//2 in float32: _subroutines/_traj.py:57-66."""
+    out = np.array(coord, dtype=np.float32, copy=True)
+    for a in range(out.shape[-1]):
+        n = np.float32(shape_xyz[a])
+        g = np.ceil(np.float32(oversamp_xyz[a]) * n)
+        out[..., a] = out[..., a] * np.float32(g / n) + np.float32(g // 2)
+    return out
+
+
+def interp_tables(coord_os, width_xyz, beta_xyz, grid_xyz):
+    """Per-axis tap index/value tables, ordered (x, y, z).
+
+    index[a] (P, W_a) int32 and value[a] (P, W_a) float32 for P points, from
+    _subroutines/_grid/_plan.py:192-210: x0 = ceil(c - W/2), taps x0+i,
+    value KB((x0+i-c)/(W/2)) computed in float64 and stored as float32,
+    index (x0+i) mod G.  W=1 axes use the integer coordinate with value 1
+    (_grid/_plan.py:68-72, intent of defect A.9).
+    """
+    c = np.asarray(coord_os, dtype=np.float32).reshape(-1, coord_os.shape[-1])
+    index, value = [], []
+    for a in range(c.shape[-1]):
+        w = int(width_xyz[a])
+        g = int(grid_xyz[a])
+        ca = c[:, a].astype(np.float64)
+        if w == 1:
+            index.append(np.mod(np.rint(ca).astype(np.int64), g).astype(np.int32)[:, None])
+            value.append(np.ones((c.shape[0], 1), dtype=np.float32))
+            continue
+        x0 = np.ceil(ca - w / 2)
+        taps = x0[:, None] + np.arange(w)[None, :]
+        value.append(kb((taps - ca[:, None]) / (w / 2), beta_xyz[a]).astype(np.float32))
+        index.append(np.mod(taps, g).astype(np.int32))
+    return index, value
+
+
+def apod_vector(n, oversamp, width, beta):
+    """KB deapodisation along one axis of length n: _subroutines/_apod.py:27-41."""
+    if int(width) == 1:
+        return np.ones(n, dtype=np.float64)
+    os_n = math.ceil(float(np.float32(oversamp)) * n)
+    idx = np.arange(n, dtype=np.float64)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        a = np.sqrt(float(beta) ** 2 - (math.pi * int(width) * (idx - n // 2) / os_n) ** 2)
+        a = a / np.sinh(a)
+        a = a / a[n // 2]
+    return np.nan_to_num(a, nan=1.0)
+
+
+def resize(x, oshape):
+    """Centered zero-pad or crop of the trailing axes: _subroutines/_resize.py:16-42."""
+    x = np.asarray(x)
+    d = len(oshape)
+    ishape = x.shape[-d:]
+    lead = x.shape[:-d]
+    out = np.zeros(lead + tuple(oshape), dtype=x.dtype)
+    isl, osl = [], []
+    for i, o in zip(ishape, oshape):
+        si = max(i // 2 - o // 2, 0)
+        so = max(o // 2 - i // 2, 0)
+        cpy = min(i - si, o - so)
+        isl.append(slice(si, si + cpy))
+        osl.append(slice(so, so + cpy))
+    out[(Ellipsis,) + tuple(osl)] = x[(Ellipsis,) + tuple(isl)]
+    return out
+
+
+def fftc(x, ndim, norm=None, inverse=False, centered=True):
+    """fftshift(fftn(ifftshift(x))) over the last ndim axes (torch branch of
+    _subroutines/_fft.py:72-79 / 163-170); norm None or 'ortho'."""
+    axes = tuple(range(-ndim, 0))
+    f = np.fft.ifftn if inverse else np.fft.fftn
+    if centered:
+        return np.fft.fftshift(f(np.fft.ifftshift(x, axes=axes), axes=axes, norm=norm), axes=axes)
+    return f(x, axes=axes, norm=norm)
+
+
+# --------------------------------------------------------------------------
+# Plan
+# --------------------------------------------------------------------------
+class Plan:
+    """Restatement of plan_nufft (src/pyfourier/_nufft/_plan.py:20-199).
+
+    ``coord`` (F, ..., d) in any unit; ``shape`` the image matrix (z, y, x).
+    """
+
+    def __init__(self, coord, shape, width=4, oversamp=1.25, normalize=True):
+        coord = np.asarray(coord, dtype=np.float32)
+        d = coord.shape[-1]
+        while coord.ndim < 4:
+            coord = coord[None]
+        self.ndim = d
+        self.nframes = coord.shape[0]
+        self.dshape = coord.shape[1:-1]  # (nviews, nsamples)
+        self.npts = int(np.prod(self.dshape))
+        shape = [shape] * d if np.isscalar(shape) else list(shape)[-d:]
+        width = [width] * d if np.isscalar(width) else list(width)
+        oversamp = [oversamp] * d if np.isscalar(oversamp) else list(oversamp)
+        self.shape = [int(s) for s in shape]  # (z, y, x)
+        shape_xyz = self.shape[::-1]
+        if normalize:
+            coord = normalize_coordinates(coord, shape_xyz)
+        self.coord = coord  # grid units, (F, nv, ns, d)
+        # Cartesian axes (all integer) get sigma=1, W=1 (_nufft/_plan.py:139-146)
+        is_cart = [np.allclose(coord[..., a], np.round(coord[..., a])) for a in range(d)][::-1]
+        width = np.asarray(width, dtype=np.int16)
+        oversamp = np.asarray(oversamp, dtype=np.float32)
+        width[np.asarray(is_cart)] = 1
+        oversamp[np.asarray(is_cart)] = 1.0
+        self.width = [int(w) for w in width]  # (z, y, x)
+        self.oversamp = [float(s) for s in oversamp]
+        self.beta = [float(b) for b in kb_beta(width, oversamp)]
+        self.os_shape = os_shape(self.shape, oversamp)  # (z, y, x)
+        self.coord_os = scale_coord(coord, shape_xyz, list(oversamp[::-1]))
+        self.index, self.value = interp_tables(
+            self.coord_os.reshape(-1, d),
+            self.width[::-1],
+            self.beta[::-1],
+            self.os_shape[::-1],
+        )
+        self.scale = kernel_scale(self.beta[::-1], self.width[::-1])
+
+    # separable tap weights for all points: list over axes (x, y, z)
+    def taps(self):
+        return self.index, self.value
+
+
+# --------------------------------------------------------------------------
+# Gather / spread kernels (per frame), all batch items at once.
+# _cart2noncart/_3D.py:13-51 and _noncart2cart/_3D.py:13-51 (and 1D/2D twins)
+# --------------------------------------------------------------------------
+def _tap_iter(plan, frame):
+    """Yield (flat grid index array, weight array) for every tap combination
+    of the points of one frame, in (z, y, x) loop order."""
+    d = plan.ndim
+    sl = slice(frame * plan.npts, (frame + 1) * plan.npts)
+    idx = [i[sl] for i in plan.index][::-1]  # (z, y, x)
+    val = [v[sl].astype(np.float64) for v in plan.value][::-1]
+    gshape = plan.os_shape
+    widths = [i.shape[1] for i in idx]
+    for taps in np.ndindex(*widths):
+        flat = np.zeros(plan.npts, dtype=np.int64)
+        w = np.ones(plan.npts, dtype=np.float64)
+        for a in range(d):
+            flat = flat * gshape[a] + idx[a][:, taps[a]]
+            w = w * val[a][:, taps[a]]
+        yield flat, w
+
+
+def gather(grid, plan):
+    """grid (B, F, *G) -> samples (B, F, M); /scale applied."""
+    grid = np.asarray(grid)
+    b = grid.shape[0]
+    out = np.zeros((b, plan.nframes, plan.npts), dtype=np.complex128)
+    for f in range(plan.nframes):
+        g = grid[:, f].reshape(b, -1)
+        for flat, w in _tap_iter(plan, f):
+            out[:, f] += w[None] * g[:, flat]
+    return out / plan.scale
+
+
+def spread(ksp, plan):
+    """samples (B, F, M) -> grid (B, F, *G); /scale applied."""
+    ksp = np.asarray(ksp)
+    b = ksp.shape[0]
+    gsize = int(np.prod(plan.os_shape))
+    out = np.zeros((b, plan.nframes, gsize), dtype=np.complex128)
+    for f in range(plan.nframes):
+        for flat, w in _tap_iter(plan, f):
+            for bi in range(b):
+                out[bi, f] += np.bincount(flat, weights=(w * ksp[bi, f]).real, minlength=gsize)
+                out[bi, f] += 1j * np.bincount(flat, weights=(w * ksp[bi, f]).imag, minlength=gsize)
+    return out.reshape((b, plan.nframes) + tuple(plan.os_shape)) / plan.scale
+
+
+def apodize(x, plan):
+    """Multiply the trailing ndim axes by the separable apodisation (pure)."""
+    out = np.array(x, dtype=np.complex128, copy=True)
+    d = plan.ndim
+    for n in range(1, d + 1):
+        ax = -n
+        a = apod_vector(out.shape[ax], plan.oversamp[ax], plan.width[ax], plan.beta[ax])
+        out = out * a.reshape([-1] + [1] * (n - 1))
+    return out
+
+
+# --------------------------------------------------------------------------
+# Operators: src/pyfourier/_nufft/_nufft.py:218-287 (no B0) with the B0 loop
+# of :11-169 (defect A.13 fixed: each segment added once).
+# Batch layout here: x (B, F|K, *N), y (B, F, M).
+# --------------------------------------------------------------------------
+def _basis_c(basis):
+    return None if basis is None else np.asarray(basis, dtype=np.complex128)
+
+
+def _to_grid(x, plan, norm):
+    """x (B, P, *N) -> oversampled centred k-space grid (B, P, *G)."""
+    y = apodize(x, plan)
+    y = resize(y, plan.os_shape)
+    return fftc(y, plan.ndim, norm=norm)
+
+
+def _from_grid(g, plan, norm):
+    """grid (B, P, *G) -> image (B, P, *N)."""
+    x = fftc(g, plan.ndim, norm=norm, inverse=True)
+    x = resize(x, plan.shape)
+    return apodize(x, plan)
+
+
+def nufft(x, plan, basis=None, weight=None, norm=None, b0=None):
+    """Forward NUFFT.  x (B, F|K, *N) -> y (B, F, M).
+
+    Without basis, frame f of x is sampled with frame f's taps.  With
+    basis (F, K): y[f] = sum_k B[f,k] A_f X_k (B.8).  With
+    b0 = (Bt (L, nsamples), Ct (L, *N)): y = sum_l Bt[l,t] A(Ct_l x) (B.9).
+    """
+    x = np.asarray(x, dtype=np.complex128)
+    B = x.shape[0]
+    F = plan.nframes
+    basis = _basis_c(basis)
+    segs = [(None, None)] if b0 is None else list(zip(b0[0], b0[1]))
+    y = np.zeros((B, F, plan.npts), dtype=np.complex128)
+    for bt, ct in segs:
+        xs = x if ct is None else x * ct
+        if basis is None:
+            ys = gather(_to_grid(xs, plan, norm), plan)
+        else:
+            ys = np.zeros((B, F, plan.npts), dtype=np.complex128)
+            for k in range(basis.shape[1]):
+                gk = _to_grid(xs[:, k : k + 1], plan, norm)
+                gk = np.broadcast_to(gk, (B, F) + gk.shape[2:])
+                ys += basis[None, :, k, None] * gather(gk, plan)
+        if bt is not None:
+            t = np.asarray(bt).reshape(-1)
+            ys = ys * np.tile(t, plan.npts // t.size)[None, None]
+        y += ys
+    if weight is not None:
+        y = y * np.asarray(weight).reshape(1, F, -1)
+    return y
+
+
+def nufft_adj(y, plan, basis=None, weight=None, norm=None, b0=None):
+    """Adjoint NUFFT.  y (B, F, M) -> x (B, F|K, *N)."""
+    y = np.asarray(y, dtype=np.complex128)
+    F = plan.nframes
+    if weight is not None:
+        y = y * np.asarray(weight).reshape(1, F, -1)
+    basis = _basis_c(basis)
+    segs = [(None, None)] if b0 is None else list(zip(b0[0], b0[1]))
+    x = None
+    for bt, ct in segs:
+        ys = y
+        if bt is not None:
+            t = np.asarray(bt).reshape(-1)
+            ys = ys * np.conj(np.tile(t, plan.npts // t.size))[None, None]
+        if basis is None:
+            xs = _from_grid(spread(ys, plan), plan, norm)
+        else:
+            xs = np.concatenate(
+                [
+                    _from_grid(
+                        spread(np.conj(basis[None, :, k, None]) * ys, plan).sum(axis=1, keepdims=True),
+                        plan,
+                        norm,
+                    )
+                    for k in range(basis.shape[1])
+                ],
+                axis=1,
+            )
+        if ct is not None:
+            xs = xs * np.conj(ct)
+        x = xs if x is None else x + xs
+    return x
+
+
+# --------------------------------------------------------------------------
+# B0 time segmentation: intent of src/pyfourier/_subroutines/_b0.py:10-135
+# (MIRT mri_exp_approx 'hist,time,unif'), defect A.15 fixed: bin centres are
+# (e[i]+e[i+1])/2 and the temporal basis fits exp(-z t) (same sign as Ct).
+# parity unpinned by the reference (it cannot run); pinned instead by the
+# approximation error to the exact exp(-z t) (tests/test_oracle.py).
+# --------------------------------------------------------------------------
+def mri_exp_approx(zmap, t, L=6, nbins=40):
+    zmap = np.asarray(zmap)
+    t = np.asarray(t, dtype=np.float64).reshape(-1)
+    if np.isrealobj(zmap):
+        z = 2j * math.pi * zmap.astype(np.float64)
+        hk, e = np.histogram(z.imag.ravel(), bins=nbins if np.isscalar(nbins) else nbins[0])
+        zk = 1j * (e[:-1] + e[1:]) / 2
+    else:
+        z = 2 * math.pi * zmap.astype(np.complex128)
+        nb = (nbins, nbins) if np.isscalar(nbins) else tuple(nbins)
+        hk, ei, er = np.histogram2d(z.imag.ravel(), z.real.ravel(), bins=nb)
+        zk = (1j * ((ei[:-1] + ei[1:]) / 2)[:, None] + ((er[:-1] + er[1:]) / 2)[None, :])
+    hk = hk.ravel().astype(np.float64)
+    zk = zk.ravel()
+    tl = np.linspace(0, L, L) / L * t[-1]
+    ch = np.exp(-tl[:, None] * zk[None, :])  # (L, nk)
+    w = np.sqrt(hk)
+    p = np.linalg.pinv(w[:, None] * ch.T) * w[None, :]  # (L, nk)
+    bt = p @ np.exp(-zk[:, None] * t[None, :])  # (L, nt)
+    ct = np.exp(-tl.reshape((-1,) + (1,) * z.ndim) * z[None])
+    return bt, ct
+
+
+# --------------------------------------------------------------------------
+# Toeplitz Gram: B.10.  PSF(dr) = sum_j w_j exp(+2 pi i k_j . dr / N) on the
+# 2N grid, T = FFT_u(wrap(PSF)) / Ghat, Gx = crop(IFFT_u(T * FFT_u(pad(x)))).
+# ---------------------------------------------------------------------------
+def toeplitz_kernel_exact(coord_grid, shape, weight=None, ghat=1.0):
+    """FP64 exact Toeplitz kernel from the NUDFT PSF (small problems only)."""
+    from .nudft import psf_exact
+
+    psf = psf_exact(coord_grid, shape, weight)  # (2N...) centred at index N
+    d = len(shape)
+    axes = tuple(range(-d, 0))
+    return np.fft.fftn(np.fft.ifftshift(psf, axes=axes), axes=axes) / ghat
+
+
+def toeplitz_apply(x, T, shape):
+    """Gx = crop_N(IFFT_u(T * FFT_u(pad_2N(x)))): _sparsegram/_sparsegram.py:59-81."""
+    d = len(shape)
+    big = [2 * s for s in shape]
+    xp = resize(np.asarray(x, dtype=np.complex128), big)
+    yp = fftc(T * fftc(xp, d, centered=False), d, inverse=True, centered=False)
+    return resize(yp, list(shape))

@@ -1,0 +1,28 @@
+"""pyfourier_amd: an MI355X-native NUFFT engine with pyfourier's NUFFT API.
+
+Drop-in entry points (same names, signatures and shape rules as
+INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
+
+    plan_nufft, nufft, nufft_adj      forward / adjoint NUFFT (+ subspace, B0)
+
+The compute path is libpfhip.so: hand-written HIP kernels for gfx950 behind a
+C ABI (include/pfhip.h), called through ctypes on PyTorch-ROCm streams.
+"""
+
+from ._lib import PFError
+from ._nufft import nufft, nufft_adj
+from ._plan import NUFFTPlan, plan_nufft
+from ._traj import normalize_coordinates
+from ._b0 import mri_exp_approx
+
+__all__ = [
+    "PFError",
+    "NUFFTPlan",
+    "plan_nufft",
+    "nufft",
+    "nufft_adj",
+    "normalize_coordinates",
+    "mri_exp_approx",
+]
+
+__version__ = "0.1.0"

@@ -1,0 +1,331 @@
+"""Forward and adjoint NUFFT: drop-in nufft / nufft_adj.
+
+Mirrors src/pyfourier/_nufft/_fwd.py:18-230, src/pyfourier/_nufft/_adj.py:18-216
+and the operator pipeline of src/pyfourier/_nufft/_nufft.py:11-287.  Every
+call goes through the native engine (libpfhip, include/pfhip.h); there is no
+CPU or PyTorch fallback.
+
+Deviations from the reference (its defects, see the module docstrings of
+_plan/_traj/_b0): B0 segments are summed once (A.13); autograd backward
+passes use the exact adjoint of the forward in use, also under norm=None
+(A.22); inputs are never modified in place (A.19).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._plan import NUFFTPlan, plan_nufft, resolve_device
+
+_NORMS = {None: _lib.PF_NORM_NONE, "ortho": _lib.PF_NORM_ORTHO}
+
+
+def _norm_code(norm) -> int:
+    if norm not in _NORMS:
+        raise ValueError(f"norm must be None or 'ortho', got {norm!r}")
+    return _NORMS[norm]
+
+
+def _to_torch(x):
+    if x is None or isinstance(x, torch.Tensor):
+        return x
+    return torch.as_tensor(np.asarray(x))
+
+
+class _Call:
+    """Everything an operator call needs besides the data tensors."""
+
+    def __init__(self, plan: NUFFTPlan, device, basis, weight, norm):
+        self.plan = plan
+        self.device = device
+        self.interp = plan.interpolator
+        self.native = self.interp.native(device)
+        self.norm = norm
+        self.basis = None
+        if basis is not None:
+            b = _to_torch(basis)
+            if b.dim() != 2 or b.shape[0] != self.interp.nframes:
+                raise ValueError(
+                    f"basis must have shape (ncontrasts={self.interp.nframes}, ncoeff), got {tuple(b.shape)}"
+                )
+            self.basis = b.to(device=device, dtype=torch.complex64).contiguous()
+        self.weight_flat = None  # (F*npts,) float32 for the native path
+        self.weight_full = None  # anything else, applied with torch
+        if weight is not None:
+            w = _to_torch(weight).to(device)
+            if not torch.is_complex(w):
+                ks = (self.interp.nframes,) + tuple(self.interp.dshape)
+                try:
+                    we = torch.broadcast_to(w.to(torch.float32), ks)
+                    if we.dim() >= w.dim():
+                        self.weight_flat = we.reshape(-1).contiguous()
+                except RuntimeError:
+                    self.weight_flat = None
+            if self.weight_flat is None:
+                self.weight_full = w.to(torch.complex64 if torch.is_complex(w) else torch.float32)
+        self.segments = self._segments()
+
+    def _segments(self):
+        p = self.plan
+        if p.zmap_s_kernel is None:
+            return [(None, None)]
+        bt = p.zmap_t_kernel.to(device=self.device, dtype=torch.complex64)
+        bs = p.zmap_s_kernel.to(device=self.device, dtype=torch.complex64)
+        L = bt.shape[0]
+        step = int(p.zmap_batch_size or L)
+        out = []
+        for l0 in range(0, L, step):
+            l1 = min(L, l0 + step)
+            out.append((bt[l0:l1].contiguous(), bs[l0:l1].reshape(l1 - l0, -1).contiguous()))
+        return out
+
+    @property
+    def nimg(self):
+        return self.basis.shape[1] if self.basis is not None else self.interp.nframes
+
+    def desc(self, C, seg):
+        d = _lib.BatchDesc()
+        d.batch = int(C)
+        d.ncoeff = int(self.basis.shape[1]) if self.basis is not None else 0
+        d.basis = _lib.ptr(self.basis)
+        bt, bs = seg
+        d.nseg = int(bt.shape[0]) if bt is not None else 1
+        d.b0_temporal = _lib.ptr(bt)
+        d.b0_spatial = _lib.ptr(bs)
+        d.nt = int(bt.shape[1]) if bt is not None else 1
+        return d
+
+    # ---- native operator calls on (C, nimg, *N) / (C, F, npts) ------------
+    def forward(self, x: torch.Tensor, norm_code: int) -> torch.Tensor:
+        C = x.shape[0]
+        it = self.interp
+        out = torch.empty((C, it.nframes, it.npts), dtype=torch.complex64, device=self.device)
+        if C == 0:
+            return out
+        L = _lib.lib()
+        acc = None
+        for seg in self.segments:
+            desc = self.desc(C, seg)
+            nbytes = ctypes.c_size_t()
+            _lib.check(L.pf_workspace_size(self.native.handle, ctypes.byref(desc), 0, ctypes.byref(nbytes)))
+            ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
+            y = out if acc is None else torch.empty_like(out)
+            with torch.cuda.device(self.device):
+                _lib.check(
+                    L.pf_nufft(
+                        self.native.handle, ctypes.byref(desc), _lib.ptr(x), _lib.ptr(y), _lib.ptr(self.weight_flat),
+                        norm_code, _lib.ptr(ws), nbytes.value, _lib.stream_of(self.device),
+                    )
+                )
+            acc = y if acc is None else acc.add_(y)
+        return acc
+
+    def adjoint(self, y: torch.Tensor, norm_code: int) -> torch.Tensor:
+        C = y.shape[0]
+        it = self.interp
+        out = torch.empty((C, self.nimg) + tuple(it.shape), dtype=torch.complex64, device=self.device)
+        if C == 0:
+            return out
+        L = _lib.lib()
+        acc = None
+        for seg in self.segments:
+            desc = self.desc(C, seg)
+            nbytes = ctypes.c_size_t()
+            _lib.check(L.pf_workspace_size(self.native.handle, ctypes.byref(desc), 1, ctypes.byref(nbytes)))
+            ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
+            x = out if acc is None else torch.empty_like(out)
+            with torch.cuda.device(self.device):
+                _lib.check(
+                    L.pf_nufft_adj(
+                        self.native.handle, ctypes.byref(desc), _lib.ptr(y), _lib.ptr(x), _lib.ptr(self.weight_flat),
+                        norm_code, _lib.ptr(ws), nbytes.value, _lib.stream_of(self.device),
+                    )
+                )
+            acc = x if acc is None else acc.add_(x)
+        return acc
+
+
+# ---- shape handling (src/pyfourier/_subroutines/_utils/_common.py:53-71, intent) ----
+def _image_layout(image: torch.Tensor, call: _Call):
+    it = call.interp
+    d = it.ndim
+    if tuple(image.shape[-d:]) != tuple(it.shape):
+        raise ValueError(f"image spatial shape {tuple(image.shape[-d:])} does not match plan shape {it.shape}")
+    if call.basis is not None or it.nframes > 1:
+        nimg = call.nimg
+        if image.dim() < d + 1 or image.shape[-(d + 1)] != nimg:
+            what = "ncoeff" if call.basis is not None else "ncontrasts"
+            raise ValueError(f"image must have a {what}={nimg} axis before the spatial axes")
+        batch_shape = tuple(image.shape[: -(d + 1)])
+    else:
+        nimg = 1
+        batch_shape = tuple(image.shape[:-d])
+    C = int(np.prod(batch_shape)) if batch_shape else 1
+    return batch_shape, C, nimg
+
+
+def _kspace_layout(kspace: torch.Tensor, call: _Call):
+    it = call.interp
+    nd = len(it.dshape)
+    if tuple(kspace.shape[-nd:]) != tuple(it.dshape):
+        raise ValueError(f"k-space shape {tuple(kspace.shape)} does not match the trajectory {it.dshape}")
+    if it.nframes > 1:
+        if kspace.shape[-(nd + 1)] != it.nframes:
+            raise ValueError(f"k-space must have an ncontrasts={it.nframes} axis")
+        batch_shape = tuple(kspace.shape[: -(nd + 1)])
+    else:
+        batch_shape = tuple(kspace.shape[:-nd])
+    C = int(np.prod(batch_shape)) if batch_shape else 1
+    return batch_shape, C
+
+
+# ---- autograd --------------------------------------------------------------
+class _NUFFTFn(torch.autograd.Function):
+    """y = A x with A the forward NUFFT; backward is the exact adjoint."""
+
+    @staticmethod
+    def forward(ctx, image, call, norm_code):
+        ctx.call = call
+        ctx.norm_code = norm_code
+        return call.forward(image, norm_code)
+
+    @staticmethod
+    def backward(ctx, g):
+        call = ctx.call
+        code = _lib.PF_NORM_ORTHO if ctx.norm_code == _lib.PF_NORM_ORTHO else (
+            _lib.PF_NORM_ADJOINT if ctx.norm_code == _lib.PF_NORM_NONE else _lib.PF_NORM_NONE
+        )
+        return call.adjoint(g.contiguous().to(torch.complex64), code), None, None
+
+
+class _NUFFTAdjFn(torch.autograd.Function):
+    """x = A^H y with the adjoint NUFFT; backward is the exact adjoint (A)."""
+
+    @staticmethod
+    def forward(ctx, kspace, call, norm_code):
+        ctx.call = call
+        ctx.norm_code = norm_code
+        return call.adjoint(kspace, norm_code)
+
+    @staticmethod
+    def backward(ctx, g):
+        call = ctx.call
+        code = _lib.PF_NORM_ORTHO if ctx.norm_code == _lib.PF_NORM_ORTHO else (
+            _lib.PF_NORM_ADJOINT if ctx.norm_code == _lib.PF_NORM_NONE else _lib.PF_NORM_NONE
+        )
+        return call.forward(g.contiguous().to(torch.complex64), code), None, None
+
+
+def _restore(out: torch.Tensor, dtype, backend_numpy: bool, idevice):
+    if not dtype.is_complex:
+        out = out.real
+    out = out.to(dtype)
+    if backend_numpy:
+        return out.detach().cpu().numpy()
+    return out.to(idevice)
+
+
+def _prepare(x, coord, plan, shape_arg, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, for_adj):
+    backend_numpy = not isinstance(x, torch.Tensor)
+    x = _to_torch(x)
+    idevice = x.device
+    dev = resolve_device(idevice if device is None else device)
+    if plan is None:
+        if coord is None:
+            raise ValueError("provide coord or nufft_plan")
+        c = _to_torch(coord)
+        if shape_arg is None:
+            if for_adj:
+                raise ValueError("nufft_adj needs the image shape when no plan is given")
+            shape_arg = tuple(x.shape[-c.shape[-1]:])
+        plan = plan_nufft(c.to(torch.float32), shape_arg, width, oversamp, zmap, L, nbins, dt, T, L_batch_size)
+    plan.to(dev)
+    return x, backend_numpy, idevice, dev, plan
+
+
+def nufft(
+    image,
+    coord=None,
+    shape=None,
+    nufft_plan=None,
+    basis=None,
+    zmap=None,
+    L=6,
+    nbins=(40, 40),
+    dt=None,
+    T=None,
+    L_batch_size=None,
+    weight=None,
+    norm=None,
+    device="cpu",
+    threadsperblock=128,
+    width=4,
+    oversamp=1.25,
+):
+    """N-dimensional forward NUFFT (drop-in for pyfourier.nufft).
+
+    image (..., [ncontrasts | ncoeff], [nz], ny, nx) -> kspace (..., [ncontrasts], nviews, nsamples).
+    `device` must name a GPU ("cuda:n" or n); `threadsperblock` is accepted and ignored.
+    """
+    code = _norm_code(norm)
+    x, backend_numpy, idevice, dev, plan = _prepare(
+        image, coord, nufft_plan, shape, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, False
+    )
+    dtype = x.dtype
+    call = _Call(plan, dev, basis, weight, norm)
+    batch_shape, C, nimg = _image_layout(x, call)
+    xin = x.to(device=dev, dtype=torch.complex64).reshape((C, nimg) + tuple(call.interp.shape)).contiguous()
+    y = _NUFFTFn.apply(xin, call, code)
+    it = call.interp
+    oshape = batch_shape + ((it.nframes,) if it.nframes > 1 else ()) + tuple(it.dshape)
+    y = y.reshape(oshape)
+    if call.weight_full is not None:
+        y = y * call.weight_full
+    return _restore(y, dtype if dtype.is_floating_point or dtype.is_complex else torch.complex64, backend_numpy, idevice)
+
+
+def nufft_adj(
+    kspace,
+    coord=None,
+    shape=None,
+    nufft_plan=None,
+    basis=None,
+    zmap=None,
+    L=6,
+    nbins=(40, 40),
+    dt=None,
+    T=None,
+    L_batch_size=None,
+    weight=None,
+    norm=None,
+    device="cpu",
+    threadsperblock=128,
+    width=4,
+    oversamp=1.25,
+):
+    """N-dimensional adjoint NUFFT (drop-in for pyfourier.nufft_adj).
+
+    kspace (..., [ncontrasts], nviews, nsamples) -> image (..., [ncontrasts | ncoeff], [nz], ny, nx).
+    """
+    code = _norm_code(norm)
+    y, backend_numpy, idevice, dev, plan = _prepare(
+        kspace, coord, nufft_plan, shape, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, True
+    )
+    dtype = y.dtype
+    call = _Call(plan, dev, basis, weight, norm)
+    batch_shape, C = _kspace_layout(y, call)
+    it = call.interp
+    yin = y.to(device=dev, dtype=torch.complex64)
+    if call.weight_full is not None:
+        yin = yin * call.weight_full
+    yin = yin.reshape(C, it.nframes, it.npts).contiguous()
+    x = _NUFFTAdjFn.apply(yin, call, code)
+    keep = call.basis is not None or it.nframes > 1
+    oshape = batch_shape + ((call.nimg,) if keep else ()) + tuple(it.shape)
+    x = x.reshape(oshape)
+    return _restore(x, dtype if dtype.is_floating_point or dtype.is_complex else torch.complex64, backend_numpy, idevice)

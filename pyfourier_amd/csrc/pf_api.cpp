@@ -1,0 +1,294 @@
+// extern "C" entry points of libpfhip (declared in include/pfhip.h).
+#include <cmath>
+#include <new>
+#include <string>
+
+#include "../../include/pfhip.h"
+#include "pf_internal.h"
+
+int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_zyx,
+                   const float* oversamp_zyx, int nframes, int64_t npts);
+int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream);
+void pf_plan_free(pf_plan* p);
+
+static thread_local std::string g_err;
+
+void pf_set_error(const std::string& msg) { g_err = msg; }
+int pf_fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+namespace {
+
+const size_t ALIGN = 256;
+size_t align_up(size_t n) { return (n + ALIGN - 1) / ALIGN * ALIGN; }
+
+int make_batch(const pf_plan* p, const pf_batch_desc* d, pf_batch* b) {
+    if (!p) return pf_fail(PF_ERR_ARG, "null plan");
+    if (!d) return pf_fail(PF_ERR_ARG, "null batch description");
+    if (d->batch < 1) return pf_fail(PF_ERR_ARG, "batch must be >= 1");
+    if (d->nseg < 1) return pf_fail(PF_ERR_ARG, "nseg must be >= 1");
+    if (d->ncoeff < 0) return pf_fail(PF_ERR_ARG, "ncoeff must be >= 0");
+    if (d->ncoeff > 0 && !d->basis) return pf_fail(PF_ERR_ARG, "ncoeff > 0 needs a basis");
+    const bool has_b0 = d->b0_temporal || d->b0_spatial;
+    if ((d->nseg > 1 || has_b0) && (!d->b0_temporal || !d->b0_spatial))
+        return pf_fail(PF_ERR_ARG, "a field map needs both b0_temporal and b0_spatial");
+    if (has_b0 && d->nt < 1) return pf_fail(PF_ERR_ARG, "nt must be >= 1");
+    b->C = d->batch;
+    b->Cv = (d->batch % 8 == 0) ? 8 : (d->batch % 4 == 0) ? 4 : (d->batch % 2 == 0) ? 2 : 1;
+    b->Cb = b->C / b->Cv;
+    b->L = d->nseg;
+    b->K = d->ncoeff > 0 ? d->ncoeff : 1;
+    b->R = b->L * b->K;
+    if (b->R > 4096) return pf_fail(PF_ERR_UNSUPPORTED, "nseg*ncoeff too large");
+    b->basis = d->ncoeff > 0 ? static_cast<const float2*>(d->basis) : nullptr;
+    b->b0t = has_b0 ? static_cast<const float2*>(d->b0_temporal) : nullptr;
+    b->b0s = has_b0 ? static_cast<const float2*>(d->b0_spatial) : nullptr;
+    b->nt = has_b0 ? d->nt : 1;
+    b->nfg = b->basis ? 1 : p->nframes;
+    b->nimg = b->basis ? b->K : p->nframes;
+    return PF_OK;
+}
+
+size_t grid_bytes(const pf_plan* p, const pf_batch& b) {
+    return (size_t)b.nfg * b.C * b.R * (size_t)pf_gtot(p) * sizeof(float2);
+}
+size_t samples_bytes(const pf_plan* p, const pf_batch& b) { return (size_t)p->mtot * b.C * sizeof(float2); }
+
+double ghat(const pf_plan* p) { return (double)pf_gtot(p); }
+
+}  // namespace
+
+extern "C" {
+
+const char* pf_last_error(void) { return g_err.c_str(); }
+
+int pf_version(void) { return 100; }
+
+int pf_plan_create(const float* coord, int ndim, const int* shape, const int* width, const float* oversamp,
+                   int nframes, long long npts, void* stream, pf_plan** out) {
+    if (!out) return pf_fail(PF_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!coord || !shape || !width || !oversamp) return pf_fail(PF_ERR_ARG, "null argument");
+    pf_plan* p = new (std::nothrow) pf_plan();
+    if (!p) return pf_fail(PF_ERR_ALLOC, "host allocation failed");
+    int rc = pf_plan_params(p, ndim, shape, width, oversamp, nframes, (int64_t)npts);
+    if (rc != PF_OK) {
+        pf_plan_free(p);
+        return rc;
+    }
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        pf_plan_free(p);
+        return pf_fail(PF_ERR_HIP, "hipGetDevice failed");
+    }
+    // deapodisation vectors (src/pyfourier/_subroutines/_apod.py:27-41)
+    int ntot = p->shape[0] + p->shape[1] + p->shape[2];
+    float* h = new float[ntot];
+    int o = 0;
+    for (int a = 0; a < 3; ++a) {
+        int n = p->shape[a];
+        for (int i = 0; i < n; ++i) h[o + i] = 1.0f;
+        if (p->width[a] > 1) {
+            auto raw = [&](int i) {
+                double t = M_PI * p->width[a] * (double)(i - n / 2) / p->os[a];
+                double v = std::sqrt((double)p->beta[a] * p->beta[a] - t * t);
+                return v / std::sinh(v);
+            };
+            double c = raw(n / 2);
+            for (int i = 0; i < n; ++i) {
+                double v = raw(i) / c;
+                h[o + i] = std::isnan(v) ? 1.0f : (float)v;
+            }
+        }
+        o += n;
+    }
+    hipError_t he = hipMalloc(&p->apod, ntot * sizeof(float));
+    if (he == hipSuccess) he = hipMemcpy(p->apod, h, ntot * sizeof(float), hipMemcpyHostToDevice);
+    delete[] h;
+    if (he != hipSuccess) {
+        pf_plan_free(p);
+        return pf_fail(PF_ERR_HIP, std::string("apod upload: ") + hipGetErrorString(he));
+    }
+    rc = pf_plan_build(p, coord, static_cast<hipStream_t>(stream));
+    if (rc != PF_OK) {
+        pf_plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return PF_OK;
+}
+
+int pf_plan_destroy(pf_plan* plan) {
+    pf_plan_free(plan);
+    return PF_OK;
+}
+
+int pf_plan_get_info(const pf_plan* p, pf_plan_info* info) {
+    if (!p || !info) return pf_fail(PF_ERR_ARG, "null argument");
+    info->ndim = p->ndim;
+    for (int a = 0; a < 3; ++a) {
+        info->shape[a] = p->shape[a];
+        info->os_shape[a] = p->os[a];
+        info->width[a] = p->width[a];
+        info->beta[a] = p->beta[a];
+        info->tile[a] = p->tile[a];
+    }
+    info->scale = p->scale;
+    info->nframes = p->nframes;
+    info->npts = p->npts;
+    info->spread_entries = p->sl[0].nslist;
+    info->spread_items = p->sl[0].nsitems;
+    info->spread_slots = p->sl[0].nslots;
+    return PF_OK;
+}
+
+int pf_plan_get_permutation(const pf_plan* p, int* perm, void* stream) {
+    if (!p || !perm) return pf_fail(PF_ERR_ARG, "null argument");
+    PF_HIP(hipMemcpyAsync(perm, p->perm, p->mtot * sizeof(int), hipMemcpyDeviceToDevice,
+                          static_cast<hipStream_t>(stream)));
+    return PF_OK;
+}
+
+int pf_grid_size(const pf_plan* p, const pf_batch_desc* d, size_t* bytes) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!bytes) return pf_fail(PF_ERR_ARG, "null output");
+    *bytes = grid_bytes(p, b);
+    return PF_OK;
+}
+
+int pf_spread_workspace_size(const pf_plan* p, const pf_batch_desc* d, size_t* bytes) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!bytes) return pf_fail(PF_ERR_ARG, "null output");
+    *bytes = pf_spread_partial_bytes(p, b);
+    return PF_OK;
+}
+
+int pf_workspace_size(const pf_plan* p, const pf_batch_desc* d, int op, size_t* bytes) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!bytes) return pf_fail(PF_ERR_ARG, "null output");
+    if (op != 0 && op != 1) return pf_fail(PF_ERR_ARG, "op must be 0 (forward) or 1 (adjoint)");
+    size_t n = align_up(grid_bytes(p, b)) + align_up(samples_bytes(p, b));
+    if (op == 1) n += align_up(pf_spread_partial_bytes(p, b));
+    *bytes = n;
+    return PF_OK;
+}
+
+int pf_deapod_pad(const pf_plan* p, const pf_batch_desc* d, const void* image, void* grid, void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!image || !grid) return pf_fail(PF_ERR_ARG, "null array");
+    return pf_launch_deapod_pad(p, b, static_cast<const float2*>(image), static_cast<float2*>(grid),
+                                static_cast<hipStream_t>(stream));
+}
+
+int pf_fft(pf_plan* p, const pf_batch_desc* d, void* grid, int inverse, void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!grid) return pf_fail(PF_ERR_ARG, "null array");
+    return pf_fft_exec(p, b, static_cast<float2*>(grid), inverse ? 1 : 0, static_cast<hipStream_t>(stream));
+}
+
+int pf_interp(const pf_plan* p, const pf_batch_desc* d, const void* grid, void* ksp_sorted, void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!grid || !ksp_sorted) return pf_fail(PF_ERR_ARG, "null array");
+    return pf_launch_gather(p, b, static_cast<const float2*>(grid), static_cast<float2*>(ksp_sorted),
+                            static_cast<hipStream_t>(stream));
+}
+
+int pf_spread(const pf_plan* p, const pf_batch_desc* d, const void* ksp_sorted, void* grid, void* workspace,
+              void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!grid || !ksp_sorted) return pf_fail(PF_ERR_ARG, "null array");
+    return pf_launch_spread(p, b, static_cast<const float2*>(ksp_sorted), static_cast<float2*>(grid),
+                            static_cast<float2*>(workspace), static_cast<hipStream_t>(stream));
+}
+
+int pf_crop_deapod(const pf_plan* p, const pf_batch_desc* d, const void* grid, void* image, float factor,
+                   void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!image || !grid) return pf_fail(PF_ERR_ARG, "null array");
+    return pf_launch_crop_deapod(p, b, static_cast<const float2*>(grid), static_cast<float2*>(image), factor,
+                                 static_cast<hipStream_t>(stream));
+}
+
+int pf_sort_samples(const pf_plan* p, const pf_batch_desc* d, const void* kspace, void* ksp_sorted,
+                    const float* weight, void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!kspace || !ksp_sorted) return pf_fail(PF_ERR_ARG, "null array");
+    return pf_launch_sort(p, b, static_cast<const float2*>(kspace), static_cast<float2*>(ksp_sorted), weight,
+                          static_cast<hipStream_t>(stream));
+}
+
+int pf_unsort_samples(const pf_plan* p, const pf_batch_desc* d, const void* ksp_sorted, void* kspace,
+                      const float* weight, float factor, void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!kspace || !ksp_sorted) return pf_fail(PF_ERR_ARG, "null array");
+    return pf_launch_unsort(p, b, static_cast<const float2*>(ksp_sorted), static_cast<float2*>(kspace), weight,
+                            factor, static_cast<hipStream_t>(stream));
+}
+
+int pf_nufft(pf_plan* p, const pf_batch_desc* d, const void* image, void* kspace, const float* weight, int norm,
+             void* workspace, size_t workspace_bytes, void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!image || !kspace) return pf_fail(PF_ERR_ARG, "null array");
+    if (norm < PF_NORM_NONE || norm > PF_NORM_ADJOINT) return pf_fail(PF_ERR_ARG, "norm must be 0, 1 or 2");
+    size_t need = align_up(grid_bytes(p, b)) + align_up(samples_bytes(p, b));
+    if (!workspace || workspace_bytes < need) return pf_fail(PF_ERR_ARG, "workspace too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    float2* grid = static_cast<float2*>(workspace);
+    float2* ys = reinterpret_cast<float2*>(static_cast<char*>(workspace) + align_up(grid_bytes(p, b)));
+    double factor = 1.0 / p->scale;
+    if (norm == PF_NORM_ORTHO) factor /= std::sqrt(ghat(p));
+    if (norm == PF_NORM_ADJOINT) factor /= ghat(p);
+    if ((rc = pf_launch_deapod_pad(p, b, static_cast<const float2*>(image), grid, s)) != PF_OK) return rc;
+    if ((rc = pf_fft_exec(p, b, grid, 0, s)) != PF_OK) return rc;
+    if ((rc = pf_launch_gather(p, b, grid, ys, s)) != PF_OK) return rc;
+    return pf_launch_unsort(p, b, ys, static_cast<float2*>(kspace), weight, (float)factor, s);
+}
+
+int pf_nufft_adj(pf_plan* p, const pf_batch_desc* d, const void* kspace, void* image, const float* weight,
+                 int norm, void* workspace, size_t workspace_bytes, void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!image || !kspace) return pf_fail(PF_ERR_ARG, "null array");
+    if (norm < PF_NORM_NONE || norm > PF_NORM_ADJOINT) return pf_fail(PF_ERR_ARG, "norm must be 0, 1 or 2");
+    size_t gb = align_up(grid_bytes(p, b)), sb = align_up(samples_bytes(p, b));
+    size_t need = gb + sb + align_up(pf_spread_partial_bytes(p, b));
+    if (!workspace || workspace_bytes < need) return pf_fail(PF_ERR_ARG, "workspace too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(workspace);
+    float2* grid = reinterpret_cast<float2*>(ws);
+    float2* ys = reinterpret_cast<float2*>(ws + gb);
+    float2* part = reinterpret_cast<float2*>(ws + gb + sb);
+    double factor = 1.0 / p->scale;
+    if (norm == PF_NORM_ORTHO) factor /= std::sqrt(ghat(p));
+    if (norm == PF_NORM_NONE) factor /= ghat(p);
+    if ((rc = pf_launch_sort(p, b, static_cast<const float2*>(kspace), ys, weight, s)) != PF_OK) return rc;
+    if ((rc = pf_launch_spread(p, b, ys, grid, part, s)) != PF_OK) return rc;
+    if ((rc = pf_fft_exec(p, b, grid, 1, s)) != PF_OK) return rc;
+    return pf_launch_crop_deapod(p, b, grid, static_cast<float2*>(image), (float)factor, s);
+}
+
+}  // extern "C"

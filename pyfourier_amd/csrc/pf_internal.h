@@ -1,0 +1,148 @@
+// Internal declarations of the pyfourier_amd NUFFT engine (gfx950 / CDNA4).
+//
+// Data layout in HBM (all complex data are interleaved float2 = complex64):
+//
+//   oversampled grid   (nfg, Cb, Gz, Gy, Gx, R, Cv)
+//       nfg  frame grids: F for the plain multi-frame operator, 1 when a
+//            subspace basis maps every frame onto the same K coefficient grids
+//       Cb   coil blocks, C = Cb*Cv batch items (coils, slices, ...)
+//       G*   grid points in *stored* (un-shifted) FFT order: the centred
+//            index j of the reference is stored at (j - G//2) mod G, so the
+//            fftshift/ifftshift pairs of the reference fold into index maps
+//       R    reduction vector: L B0 segments x K subspace coefficients
+//       Cv   coils innermost, so one tap of one sample reads Cv*R contiguous
+//            complex values
+//   samples (sorted)   (Mtot, C) in tile-sorted order (coil innermost)
+//   samples (user)     (C, F, npts) complex64, the reference's layout
+//
+// Every 1-D/2-D problem is stored as 3-D with leading unit axes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pfhip.h"
+
+#define PF_WMAX 8
+
+struct pf_fft_cache;
+
+// Spread work list: for every (frame grid, tile) the sorted samples whose
+// footprint touches the tile, split into chunks of at most max_chunk entries.
+// Tiles with several chunks accumulate into partial tiles that a second
+// kernel sums in chunk order (deterministic).
+struct pf_slist {
+    int32_t* slist = nullptr;    // sorted sample ids grouped by key
+    int64_t nslist = 0;
+    int4* sitems = nullptr;      // (key, e0, e1, slot or -1)
+    int nsitems = 0;
+    int4* ritems = nullptr;      // (key, slot0, nslot, 0)
+    int nritems = 0;
+    int nslots = 0;
+};
+
+struct pf_plan {
+    int ndim = 0;          // spatial dims 1..3
+    int shape[3];          // image matrix (z, y, x), leading 1s for d<3
+    int os[3];             // oversampled grid (z, y, x)
+    int width[3];          // kernel width per axis (1 on unit / Cartesian axes)
+    float oversamp[3];
+    float beta[3];
+    double scale = 1.0;    // kernel-sum normalisation (divided out)
+    int nframes = 1;
+    int64_t npts = 0;      // samples per frame
+    int64_t mtot = 0;      // nframes * npts
+    int tile[3];           // spread/sort tile (z, y, x)
+    int ntiles[3];
+    int64_t ntile_tot = 0; // tiles per frame
+    int rec = 16;          // dwords per sample tap record
+    int woff[3];           // offset of each axis' weights inside a record
+    int device = 0;
+
+    // device arrays (sorted order unless stated)
+    int32_t* perm = nullptr;     // sorted -> original sample
+    int32_t* iperm = nullptr;    // original -> sorted
+    float* taprec = nullptr;     // (mtot, rec): int start[3], int frame, weights
+    float* cos_sorted = nullptr; // (mtot, 3) oversampled coordinate (z, y, x)
+    float* apod = nullptr;       // deapodisation vectors, Nz + Ny + Nx floats
+    // spread lists: [0] keyed by (frame, tile) for per-frame grids,
+    // [1] keyed by tile only for one grid shared by all frames (subspace);
+    // identical (aliased) when nframes == 1
+    pf_slist sl[2];
+    int max_chunk = 0;
+
+    pf_fft_cache* fft = nullptr;
+};
+
+// ---- error handling --------------------------------------------------------
+void pf_set_error(const std::string& msg);
+int pf_fail(int code, const std::string& msg);
+
+#define PF_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return pf_fail(PF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// status codes PF_OK / PF_ERR_* come from the public header
+
+// ---- batch description shared by the operator kernels ---------------------
+struct pf_batch {
+    int C;        // batch items (coils ...)
+    int Cv;       // items innermost per grid point
+    int Cb;       // C / Cv
+    int R;        // reduction vector length (L*K)
+    int L;        // B0 segments (1 = none)
+    int K;        // subspace coefficients (1 = none)
+    int nfg;      // frame grids (F or 1)
+    int nimg;     // image planes per batch item in user layout (F, K or 1)
+    const float2* basis;  // (F, K) complex or null
+    const float2* b0t;    // (L, nt) temporal interpolator or null
+    const float2* b0s;    // (L, Nz*Ny*Nx) spatial maps or null
+    int nt;               // readout length for b0t
+};
+
+// ---- internal launchers (return PF_OK or an error code) -------------------
+int pf_launch_deapod_pad(const pf_plan* p, const pf_batch& b, const float2* img, float2* grid,
+                         hipStream_t s);
+int pf_launch_crop_deapod(const pf_plan* p, const pf_batch& b, const float2* grid, float2* img,
+                          float factor, hipStream_t s);
+int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, float2* ys,
+                     hipStream_t s);
+int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, float2* grid,
+                     float2* partial, hipStream_t s);
+int pf_launch_unsort(const pf_plan* p, const pf_batch& b, const float2* ys, float2* y,
+                     const float* weight, float factor, hipStream_t s);
+int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2* ys,
+                   const float* weight, hipStream_t s);
+int pf_fft_exec(pf_plan* p, const pf_batch& b, float2* grid, int inverse, hipStream_t s);
+void pf_fft_destroy(pf_fft_cache* c);
+size_t pf_spread_partial_bytes(const pf_plan* p, const pf_batch& b);
+int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b);
+
+// ---- small helpers --------------------------------------------------------
+static inline int64_t pf_gtot(const pf_plan* p) { return (int64_t)p->os[0] * p->os[1] * p->os[2]; }
+static inline int64_t pf_ntot(const pf_plan* p) {
+    return (int64_t)p->shape[0] * p->shape[1] * p->shape[2];
+}
+// spread list for a batch: frame-merged when all frames share one grid
+static inline const pf_slist& pf_slist_for(const pf_plan* p, const pf_batch& b) {
+    return (b.nfg == 1 && p->nframes > 1) ? p->sl[1] : p->sl[0];
+}
+
+// XCD-aware block remap: the hardware deals workgroups round-robin over the
+// 8 XCDs; remap so each XCD walks a contiguous range of the logical order and
+// neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ unsigned pf_xcd_remap(unsigned b, unsigned nb) {
+    const unsigned nx = 8;
+    unsigned per = nb / nx, rem = nb % nx;
+    unsigned x = b % nx, i = b / nx;
+    // XCDs [0, rem) own per+1 logical blocks, the rest own per
+    unsigned base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    return base + i;
+}

@@ -1,0 +1,424 @@
+// Plan construction: per-sample Kaiser-Bessel tap records, tile binning and a
+// stable sort of the trajectory, and the spread work lists.
+//
+// Follows the reference interpolator plan
+//   src/pyfourier/_nufft/_plan.py:117-155            (beta, oversampled shape)
+//   src/pyfourier/_subroutines/_traj.py:57-66         (c_os = c*G/N + G//2, float32)
+//   src/pyfourier/_subroutines/_grid/_plan.py:157-210 (taps, KB values, mod G)
+//   src/pyfourier/_subroutines/_grid/_plan.py:118-141 (kernel scale)
+// but stores one compact record per sample (first stored tap + W weights per
+// axis) instead of (index, value) tables, in tile-sorted order.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pf_internal.h"
+
+namespace {
+
+// Kaiser-Bessel I0(beta*sqrt(1-x^2)) with the Abramowitz & Stegun 9.8.1 and
+// 9.8.2 polynomial approximations (the ones the reference evaluates), 0 for
+// |x| > 1.  Double precision; the tap value is rounded to float afterwards,
+// as the reference stores float32 tables.
+__host__ __device__ double kb_value(double x, double beta) {
+    if (fabs(x) > 1.0) return 0.0;
+    double u = beta * sqrt(1.0 - x * x);
+    double t = u / 3.75;
+    if (u < 3.75) {
+        double t2 = t * t;
+        return 1.0 + t2 * (3.5156229 + t2 * (3.0899424 + t2 * (1.2067492 +
+               t2 * (0.2659732 + t2 * (0.0360768 + t2 * 0.0045813)))));
+    }
+    double it = 1.0 / t;
+    double p = 0.39894228 + it * (0.01328592 + it * (0.00225319 + it * (-0.00157565 +
+               it * (0.00916281 + it * (-0.02057706 + it * (0.02635537 +
+               it * (-0.01647633 + it * 0.00392377)))))));
+    return p * exp(u) / sqrt(u);
+}
+
+struct TapParams {
+    int ndim;
+    int os[3];
+    int width[3];
+    float beta[3];
+    float sc[3];   // G/N in float32
+    float sh[3];   // G//2 in float32
+    int tile[3];
+    int ntiles[3];
+    int64_t npts;
+    int64_t mtot;
+    int64_t ntile_tot;
+    int rec;
+    int woff[3];
+};
+
+__device__ __forceinline__ int64_t pmod(int64_t a, int64_t m) {
+    int64_t r = a % m;
+    return r < 0 ? r + m : r;
+}
+
+// One thread per original sample: tap record and tile key.
+__global__ void k_taps(const float* __restrict__ coord, TapParams P, uint32_t* __restrict__ keys,
+                       int32_t* __restrict__ ids, float* __restrict__ rec,
+                       float* __restrict__ cos_out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.mtot) return;
+    int64_t f = i / P.npts;
+    int64_t tcoord[3];
+    float* r = rec + i * P.rec;
+    int* ri = reinterpret_cast<int*>(r);
+    for (int a = 0; a < 3; ++a) {
+        int W = P.width[a];
+        int G = P.os[a];
+        int64_t x0;
+        if (a < 3 - P.ndim) {
+            x0 = 0;
+            r[P.woff[a]] = 1.0f;
+            cos_out[i * 3 + a] = 0.0f;
+            ri[a] = 0;
+            tcoord[a] = 0;
+            continue;
+        }
+        // coordinate axis order is (x, y, z): grid slot 2 <- coord[..., 0]
+        float c = coord[i * P.ndim + (2 - a)];
+        float cos_ = __fadd_rn(__fmul_rn(c, P.sc[a]), P.sh[a]);
+        cos_out[i * 3 + a] = cos_;
+        double cd = (double)cos_;
+        if (W == 1) {
+            x0 = (int64_t)rint(cd);
+            r[P.woff[a]] = 1.0f;
+        } else {
+            double half = 0.5 * (double)W;
+            x0 = (int64_t)ceil(cd - half);
+            for (int k = 0; k < W; ++k)
+                r[P.woff[a] + k] = (float)kb_value(((double)(x0 + k) - cd) / half, (double)P.beta[a]);
+        }
+        // centred index j is stored at (j - G//2) mod G
+        int64_t st = pmod(x0 - G / 2, G);
+        ri[a] = (int)st;
+        tcoord[a] = st / P.tile[a];
+    }
+    ri[3] = (int)f;
+    keys[i] = (uint32_t)(f * P.ntile_tot +
+                         (tcoord[0] * P.ntiles[1] + tcoord[1]) * P.ntiles[2] + tcoord[2]);
+    ids[i] = (int32_t)i;
+}
+
+// Gather records into sorted order and build the inverse permutation.
+__global__ void k_permute(const int32_t* __restrict__ perm, const float* __restrict__ rec_in,
+                          float* __restrict__ rec_out, const float* __restrict__ cos_in,
+                          float* __restrict__ cos_out, int32_t* __restrict__ iperm, int64_t mtot,
+                          int rec) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= mtot) return;
+    int64_t i = perm[s];
+    iperm[i] = (int32_t)s;
+    const int* ri = reinterpret_cast<const int*>(rec_in) + i * rec;
+    int* ro = reinterpret_cast<int*>(rec_out) + s * rec;
+    for (int k = 0; k < rec; ++k) ro[k] = ri[k];
+    for (int k = 0; k < 3; ++k) cos_out[s * 3 + k] = cos_in[i * 3 + k];
+}
+
+// Distinct tiles touched along one axis by taps start..start+W-1 (mod G).
+__device__ __forceinline__ int axis_tiles(int start, int W, int G, int T, int* out) {
+    int n = 0;
+    int prev = -1;
+    for (int k = 0; k < W; ++k) {
+        int g = start + k;
+        if (g >= G) g -= G;
+        int t = g / T;
+        if (t != prev) {
+            bool seen = false;
+            for (int q = 0; q < n; ++q) seen |= (out[q] == t);
+            if (!seen) out[n++] = t;
+            prev = t;
+        }
+    }
+    return n;
+}
+
+__global__ void k_slist_count(const float* __restrict__ rec, TapParams P, uint32_t* __restrict__ cnt) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.mtot) return;
+    const int* ri = reinterpret_cast<const int*>(rec + s * P.rec);
+    int t[PF_WMAX];
+    uint32_t n = 1;
+    for (int a = 0; a < 3; ++a) n *= (uint32_t)axis_tiles(ri[a], P.width[a], P.os[a], P.tile[a], t);
+    cnt[s] = n;
+}
+
+__global__ void k_slist_emit(const float* __restrict__ rec, TapParams P, int merge, const uint32_t* __restrict__ off,
+                             uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.mtot) return;
+    const int* ri = reinterpret_cast<const int*>(rec + s * P.rec);
+    int tz[PF_WMAX], ty[PF_WMAX], tx[PF_WMAX];
+    int nz = axis_tiles(ri[0], P.width[0], P.os[0], P.tile[0], tz);
+    int ny = axis_tiles(ri[1], P.width[1], P.os[1], P.tile[1], ty);
+    int nx = axis_tiles(ri[2], P.width[2], P.os[2], P.tile[2], tx);
+    int64_t f = merge ? 0 : ri[3];
+    uint32_t o = off[s];
+    for (int a = 0; a < nz; ++a)
+        for (int b = 0; b < ny; ++b)
+            for (int c = 0; c < nx; ++c) {
+                keys[o] = (uint32_t)(f * P.ntile_tot + ((int64_t)tz[a] * P.ntiles[1] + ty[b]) * P.ntiles[2] + tx[c]);
+                vals[o] = (int32_t)s;
+                ++o;
+            }
+}
+
+__global__ void k_bounds(const uint32_t* __restrict__ keys, int64_t n, int32_t* __restrict__ start,
+                         int32_t* __restrict__ end) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    uint32_t k = keys[e];
+    if (e == 0 || keys[e - 1] != k) start[k] = (int32_t)e;
+    if (e == n - 1 || keys[e + 1] != k) end[k] = (int32_t)(e + 1);
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipMalloc(&p, n ? n : 1); }
+    template <class T>
+    T* as() {
+        return static_cast<T*>(p);
+    }
+};
+
+unsigned bits_for(uint64_t n) {
+    unsigned b = 1;
+    while ((1ull << b) < n) ++b;
+    return b;
+}
+
+inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+double pf_kb_value_host(double x, double beta) { return kb_value(x, beta); }
+
+// Fill the scalar plan parameters; coordinates are in grid units.
+int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_zyx,
+                   const float* oversamp_zyx, int nframes, int64_t npts) {
+    if (ndim < 1 || ndim > 3) return pf_fail(PF_ERR_ARG, "ndim must be 1, 2 or 3");
+    if (nframes < 1 || npts < 1) return pf_fail(PF_ERR_ARG, "empty trajectory");
+    if ((int64_t)nframes * npts >= (1ll << 31))
+        return pf_fail(PF_ERR_UNSUPPORTED, "more than 2^31-1 samples in one plan");
+    p->ndim = ndim;
+    p->nframes = nframes;
+    p->npts = npts;
+    p->mtot = (int64_t)nframes * npts;
+    double scale = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        int src = a - (3 - ndim);
+        if (src < 0) {
+            p->shape[a] = 1;
+            p->os[a] = 1;
+            p->width[a] = 1;
+            p->oversamp[a] = 1.0f;
+            p->beta[a] = 0.0f;
+            continue;
+        }
+        int N = shape_zyx[src];
+        int W = width_zyx[src];
+        float s = oversamp_zyx[src];
+        if (N < 1) return pf_fail(PF_ERR_ARG, "image size must be positive");
+        if (W < 1 || W > PF_WMAX) return pf_fail(PF_ERR_UNSUPPORTED, "kernel width must be in [1, 8]");
+        if (!(s >= 1.0f)) return pf_fail(PF_ERR_ARG, "oversampling must be >= 1");
+        p->shape[a] = N;
+        p->width[a] = W;
+        p->oversamp[a] = s;
+        // beta = pi*sqrt((W/s*(s-0.5))^2 - 0.8) in float32 (_nufft/_plan.py:128)
+        float q = ((float)W / s) * (s - 0.5f);
+        float inner = q * q - 0.8f;
+        p->beta[a] = (float)M_PI * sqrtf(inner);
+        p->os[a] = (int)ceilf(s * (float)N);
+        if (p->os[a] < W) return pf_fail(PF_ERR_ARG, "oversampled grid smaller than the kernel");
+        if (W > 1) {
+            // sum of KB at the half-open centred taps (_grid/_plan.py:118-141)
+            double start = std::ceil(-W / 2.0), sum = 0.0;
+            for (int e = 0; e < W; ++e) sum += kb_value((start + e) / (W / 2.0), (double)p->beta[a]);
+            scale *= sum;
+        }
+    }
+    p->scale = scale;
+    // tiles: 8^3 in 3-D, 32^2 in 2-D, 1024 in 1-D (clamped to the grid)
+    int t3[3] = {8, 8, 8}, t2[3] = {1, 32, 32}, t1[3] = {1, 1, 1024};
+    const int* tt = ndim == 3 ? t3 : (ndim == 2 ? t2 : t1);
+    p->ntile_tot = 1;
+    int off = 4;
+    for (int a = 0; a < 3; ++a) {
+        p->tile[a] = std::min(tt[a], p->os[a]);
+        p->ntiles[a] = (p->os[a] + p->tile[a] - 1) / p->tile[a];
+        p->ntile_tot *= p->ntiles[a];
+        p->woff[a] = off;
+        off += p->width[a];
+    }
+    if ((int64_t)nframes * p->ntile_tot >= (1ll << 31))
+        return pf_fail(PF_ERR_UNSUPPORTED, "too many tiles");
+    p->rec = (off + 3) & ~3;
+    p->max_chunk = 1024;
+    return PF_OK;
+}
+
+// Spread list: every (key, sample) pair for the tiles a sample's footprint
+// touches, stable-sorted by key; then work items with heavy tiles split.
+static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t stream, pf_slist& out) {
+    const int64_t M = p->mtot;
+    const int TB = 256;
+    const uint64_t nkeys = (uint64_t)(merge ? 1 : p->nframes) * p->ntile_tot;
+    const unsigned nbits = bits_for(nkeys + 1);
+    DevBuf cnt, off, skeys, svals, skeys2;
+    PF_HIP(cnt.alloc(M * 4));
+    PF_HIP(off.alloc((M + 1) * 4));
+    hipLaunchKernelGGL(k_slist_count, dim3(nblk(M, TB)), dim3(TB), 0, stream, p->taprec, P,
+                       cnt.as<uint32_t>());
+    PF_HIP(hipGetLastError());
+    size_t sbytes = 0;
+    DevBuf stmp;
+    PF_HIP(rocprim::exclusive_scan(nullptr, sbytes, cnt.as<uint32_t>(), off.as<uint32_t>(), 0u,
+                                   (size_t)M, rocprim::plus<uint32_t>(), stream));
+    PF_HIP(stmp.alloc(sbytes));
+    PF_HIP(rocprim::exclusive_scan(stmp.p, sbytes, cnt.as<uint32_t>(), off.as<uint32_t>(), 0u,
+                                   (size_t)M, rocprim::plus<uint32_t>(), stream));
+    uint32_t last_off = 0, last_cnt = 0;
+    PF_HIP(hipMemcpyAsync(&last_off, off.as<uint32_t>() + (M - 1), 4, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipMemcpyAsync(&last_cnt, cnt.as<uint32_t>() + (M - 1), 4, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    const int64_t NE = (int64_t)last_off + last_cnt;
+    if (NE >= (1ll << 31)) return pf_fail(PF_ERR_UNSUPPORTED, "spread list exceeds 2^31 entries");
+    out.nslist = NE;
+    PF_HIP(skeys.alloc(NE * 4));
+    PF_HIP(skeys2.alloc(NE * 4));
+    PF_HIP(svals.alloc(NE * 4));
+    PF_HIP(hipMalloc(&out.slist, NE * 4));
+    hipLaunchKernelGGL(k_slist_emit, dim3(nblk(M, TB)), dim3(TB), 0, stream, p->taprec, P, merge,
+                       off.as<uint32_t>(), skeys.as<uint32_t>(), svals.as<int32_t>());
+    PF_HIP(hipGetLastError());
+    size_t rbytes = 0;
+    DevBuf rtmp;
+    PF_HIP(rocprim::radix_sort_pairs(nullptr, rbytes, skeys.as<uint32_t>(), skeys2.as<uint32_t>(),
+                                     svals.as<int32_t>(), out.slist, (size_t)NE, 0, nbits, stream));
+    PF_HIP(rtmp.alloc(rbytes));
+    PF_HIP(rocprim::radix_sort_pairs(rtmp.p, rbytes, skeys.as<uint32_t>(), skeys2.as<uint32_t>(),
+                                     svals.as<int32_t>(), out.slist, (size_t)NE, 0, nbits, stream));
+    DevBuf dstart, dend;
+    PF_HIP(dstart.alloc(nkeys * 4));
+    PF_HIP(dend.alloc(nkeys * 4));
+    PF_HIP(hipMemsetAsync(dstart.p, 0, nkeys * 4, stream));
+    PF_HIP(hipMemsetAsync(dend.p, 0, nkeys * 4, stream));
+    hipLaunchKernelGGL(k_bounds, dim3(nblk(NE, TB)), dim3(TB), 0, stream, skeys2.as<uint32_t>(), NE,
+                       dstart.as<int32_t>(), dend.as<int32_t>());
+    PF_HIP(hipGetLastError());
+    std::vector<int32_t> hs(nkeys), he(nkeys);
+    PF_HIP(hipMemcpyAsync(hs.data(), dstart.p, nkeys * 4, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipMemcpyAsync(he.data(), dend.p, nkeys * 4, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+
+    // work items: heavy tiles split into chunks that a reduce pass sums in order
+    const int CH = p->max_chunk;
+    std::vector<int4> items, ritems;
+    int slot = 0;
+    for (uint64_t k = 0; k < nkeys; ++k) {
+        int e0 = hs[k], e1 = he[k];
+        int n = e1 - e0;
+        if (n <= CH) {
+            items.push_back(make_int4((int)k, e0, e1, -1));
+        } else {
+            int nch = (n + CH - 1) / CH;
+            ritems.push_back(make_int4((int)k, slot, nch, 0));
+            for (int c = 0; c < nch; ++c) {
+                int a = e0 + (int)((int64_t)n * c / nch);
+                int b = e0 + (int)((int64_t)n * (c + 1) / nch);
+                items.push_back(make_int4((int)k, a, b, slot++));
+            }
+        }
+    }
+    out.nsitems = (int)items.size();
+    out.nritems = (int)ritems.size();
+    out.nslots = slot;
+    PF_HIP(hipMalloc(&out.sitems, std::max<size_t>(1, items.size()) * sizeof(int4)));
+    PF_HIP(hipMemcpy(out.sitems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    PF_HIP(hipMalloc(&out.ritems, std::max<size_t>(1, ritems.size()) * sizeof(int4)));
+    if (!ritems.empty())
+        PF_HIP(hipMemcpy(out.ritems, ritems.data(), ritems.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return PF_OK;
+}
+
+int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
+    TapParams P;
+    P.ndim = p->ndim;
+    for (int a = 0; a < 3; ++a) {
+        P.os[a] = p->os[a];
+        P.width[a] = p->width[a];
+        P.beta[a] = p->beta[a];
+        float G = (float)p->os[a], N = (float)p->shape[a];
+        P.sc[a] = G / N;
+        P.sh[a] = (float)(p->os[a] / 2);
+        P.tile[a] = p->tile[a];
+        P.ntiles[a] = p->ntiles[a];
+        P.woff[a] = p->woff[a];
+    }
+    P.npts = p->npts;
+    P.mtot = p->mtot;
+    P.ntile_tot = p->ntile_tot;
+    P.rec = p->rec;
+    const int64_t M = p->mtot;
+    const int TB = 256;
+
+    DevBuf keys, keys2, ids, rec0, cos0, tmp;
+    PF_HIP(keys.alloc(M * 4));
+    PF_HIP(keys2.alloc(M * 4));
+    PF_HIP(ids.alloc(M * 4));
+    PF_HIP(rec0.alloc(M * p->rec * 4));
+    PF_HIP(cos0.alloc(M * 3 * 4));
+    PF_HIP(hipMalloc(&p->perm, M * 4));
+    PF_HIP(hipMalloc(&p->iperm, M * 4));
+    PF_HIP(hipMalloc(&p->taprec, M * p->rec * 4));
+    PF_HIP(hipMalloc(&p->cos_sorted, M * 3 * 4));
+    PF_HIP(hipMemsetAsync(rec0.p, 0, M * p->rec * 4, stream));
+
+    hipLaunchKernelGGL(k_taps, dim3(nblk(M, TB)), dim3(TB), 0, stream, coord, P, keys.as<uint32_t>(),
+                       ids.as<int32_t>(), rec0.as<float>(), cos0.as<float>());
+    PF_HIP(hipGetLastError());
+
+    const uint64_t nkeys = (uint64_t)p->nframes * p->ntile_tot;
+    unsigned nbits = bits_for(nkeys + 1);
+    size_t tbytes = 0;
+    PF_HIP(rocprim::radix_sort_pairs(nullptr, tbytes, keys.as<uint32_t>(), keys2.as<uint32_t>(),
+                                     ids.as<int32_t>(), p->perm, (size_t)M, 0, nbits, stream));
+    PF_HIP(tmp.alloc(tbytes));
+    PF_HIP(rocprim::radix_sort_pairs(tmp.p, tbytes, keys.as<uint32_t>(), keys2.as<uint32_t>(),
+                                     ids.as<int32_t>(), p->perm, (size_t)M, 0, nbits, stream));
+    hipLaunchKernelGGL(k_permute, dim3(nblk(M, TB)), dim3(TB), 0, stream, p->perm, rec0.as<float>(),
+                       p->taprec, cos0.as<float>(), p->cos_sorted, p->iperm, M, p->rec);
+    PF_HIP(hipGetLastError());
+
+    PF_HIP(hipStreamSynchronize(stream));
+    int rc = build_slist(p, P, 0, stream, p->sl[0]);
+    if (rc != PF_OK) return rc;
+    if (p->nframes > 1) return build_slist(p, P, 1, stream, p->sl[1]);
+    p->sl[1] = p->sl[0];
+    return PF_OK;
+}
+
+void pf_plan_free(pf_plan* p) {
+    if (!p) return;
+    void* ptrs[] = {p->perm, p->iperm, p->taprec, p->cos_sorted, p->apod};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    const bool alias = p->sl[1].slist == p->sl[0].slist;
+    for (int m = 0; m < (alias ? 1 : 2); ++m) {
+        void* sp[] = {p->sl[m].slist, p->sl[m].sitems, p->sl[m].ritems};
+        for (void* q : sp)
+            if (q) (void)hipFree(q);
+    }
+    if (p->fft) pf_fft_destroy(p->fft);
+    delete p;
+}
