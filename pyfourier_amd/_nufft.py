@@ -25,6 +25,45 @@ from ._plan import NUFFTPlan, plan_nufft, resolve_device
 _NORMS = {None: _lib.PF_NORM_NONE, "ortho": _lib.PF_NORM_ORTHO}
 
 
+class stage_timer:
+    """Context manager that runs operator calls stage by stage (the same
+    native launchers as the fused entry points) with HIP events recorded on
+    the call's stream around every stage.
+
+    >>> with stage_timer() as t:
+    ...     nufft(...)
+    >>> t.summary()  # {stage: [ms per call, ...]} after a synchronize
+    """
+
+    active = None
+
+    def __init__(self):
+        self.records = []
+
+    def __enter__(self):
+        stage_timer.active = self
+        return self
+
+    def __exit__(self, *exc):
+        stage_timer.active = None
+        return False
+
+    def record(self, name, fn, device):
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        st = torch.cuda.current_stream(device)
+        s.record(st)
+        fn()
+        e.record(st)
+        self.records.append((name, s, e))
+
+    def summary(self):
+        out = {}
+        for name, s, e in self.records:
+            out.setdefault(name, []).append(s.elapsed_time(e))
+        return out
+
+
 def _norm_code(norm) -> int:
     if norm not in _NORMS:
         raise ValueError(f"norm must be None or 'ortho', got {norm!r}")
@@ -116,14 +155,65 @@ class _Call:
             ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
             y = out if acc is None else torch.empty_like(out)
             with torch.cuda.device(self.device):
-                _lib.check(
-                    L.pf_nufft(
-                        self.native.handle, ctypes.byref(desc), _lib.ptr(x), _lib.ptr(y), _lib.ptr(self.weight_flat),
-                        norm_code, _lib.ptr(ws), nbytes.value, _lib.stream_of(self.device),
+                if stage_timer.active is not None:
+                    self._staged_forward(desc, x, y, ws, norm_code)
+                else:
+                    _lib.check(
+                        L.pf_nufft(
+                            self.native.handle, ctypes.byref(desc), _lib.ptr(x), _lib.ptr(y),
+                            _lib.ptr(self.weight_flat), norm_code, _lib.ptr(ws), nbytes.value,
+                            _lib.stream_of(self.device),
+                        )
                     )
-                )
             acc = y if acc is None else acc.add_(y)
         return acc
+
+    # ---- stage-by-stage execution (timing) ---------------------------------
+    def _factor(self, norm_code: int, adjoint: bool) -> float:
+        g = float(np.prod(self.interp.os_shape))
+        f = 1.0 / self.native.info.scale
+        if norm_code == _lib.PF_NORM_ORTHO:
+            return f / math.sqrt(g)
+        if (norm_code == _lib.PF_NORM_NONE) == adjoint:
+            return f / g
+        return f
+
+    def _split_ws(self, desc, ws):
+        L = _lib.lib()
+        gb = ctypes.c_size_t()
+        _lib.check(L.pf_grid_size(self.native.handle, ctypes.byref(desc), ctypes.byref(gb)))
+        g = (gb.value + 255) // 256 * 256
+        C = desc.batch
+        sb = (self.interp.nframes * self.interp.npts * C * 8 + 255) // 256 * 256
+        return ws[:g], ws[g : g + sb], ws[g + sb :]
+
+    def _staged_forward(self, desc, x, y, ws, norm_code):
+        L = _lib.lib()
+        h, d, st = self.native.handle, ctypes.byref(desc), _lib.stream_of(self.device)
+        grid, ys, _ = self._split_ws(desc, ws)
+        t = stage_timer.active
+        dev = self.device
+        t.record("deapod_pad", lambda: _lib.check(L.pf_deapod_pad(h, d, _lib.ptr(x), _lib.ptr(grid), st)), dev)
+        t.record("fft_fwd", lambda: _lib.check(L.pf_fft(h, d, _lib.ptr(grid), 0, st)), dev)
+        t.record("interp", lambda: _lib.check(L.pf_interp(h, d, _lib.ptr(grid), _lib.ptr(ys), st)), dev)
+        fac = self._factor(norm_code, False)
+        t.record("unsort", lambda: _lib.check(
+            L.pf_unsort_samples(h, d, _lib.ptr(ys), _lib.ptr(y), _lib.ptr(self.weight_flat), fac, st)), dev)
+
+    def _staged_adjoint(self, desc, y, x, ws, norm_code):
+        L = _lib.lib()
+        h, d, st = self.native.handle, ctypes.byref(desc), _lib.stream_of(self.device)
+        grid, ys, part = self._split_ws(desc, ws)
+        t = stage_timer.active
+        dev = self.device
+        t.record("sort", lambda: _lib.check(
+            L.pf_sort_samples(h, d, _lib.ptr(y), _lib.ptr(ys), _lib.ptr(self.weight_flat), st)), dev)
+        t.record("spread", lambda: _lib.check(
+            L.pf_spread(h, d, _lib.ptr(ys), _lib.ptr(grid), _lib.ptr(part), st)), dev)
+        t.record("fft_adj", lambda: _lib.check(L.pf_fft(h, d, _lib.ptr(grid), 1, st)), dev)
+        fac = self._factor(norm_code, True)
+        t.record("crop_deapod", lambda: _lib.check(
+            L.pf_crop_deapod(h, d, _lib.ptr(grid), _lib.ptr(x), fac, st)), dev)
 
     def adjoint(self, y: torch.Tensor, norm_code: int) -> torch.Tensor:
         C = y.shape[0]
@@ -140,12 +230,16 @@ class _Call:
             ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
             x = out if acc is None else torch.empty_like(out)
             with torch.cuda.device(self.device):
-                _lib.check(
-                    L.pf_nufft_adj(
-                        self.native.handle, ctypes.byref(desc), _lib.ptr(y), _lib.ptr(x), _lib.ptr(self.weight_flat),
-                        norm_code, _lib.ptr(ws), nbytes.value, _lib.stream_of(self.device),
+                if stage_timer.active is not None:
+                    self._staged_adjoint(desc, y, x, ws, norm_code)
+                else:
+                    _lib.check(
+                        L.pf_nufft_adj(
+                            self.native.handle, ctypes.byref(desc), _lib.ptr(y), _lib.ptr(x),
+                            _lib.ptr(self.weight_flat), norm_code, _lib.ptr(ws), nbytes.value,
+                            _lib.stream_of(self.device),
+                        )
                     )
-                )
             acc = x if acc is None else acc.add_(x)
         return acc
 
