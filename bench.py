@@ -63,6 +63,8 @@ def make_inputs(N, C, nspokes, nsamples):
 
 def stage_bytes(name, C, N3, G3, M, d):
     b = {
+        "pad_fft": C * N3 * 8 + C * G3 * 8,
+        "ifft_crop": C * G3 * 8 + C * N3 * 8,
         "deapod_pad": C * N3 * 8 + C * G3 * 8,
         "fft_fwd": 2 * C * G3 * 8,
         "interp": C * G3 * 8 + C * M * 8 + M * 4 * d,

@@ -158,6 +158,24 @@ int pf_spread(const pf_plan* plan, const pf_batch_desc* desc, const void* ksp_so
               void* workspace, void* stream);
 int pf_spread_workspace_size(const pf_plan* plan, const pf_batch_desc* desc, size_t* bytes);
 
+/* Byte offsets of the parts of the pf_nufft / pf_nufft_adj workspace:
+ * [grid | tile-sorted samples | FFT scratch | spread partial tiles]. */
+int pf_workspace_offsets(const pf_plan* plan, const pf_batch_desc* desc, size_t* grid_off,
+                         size_t* samples_off, size_t* fft_off, size_t* spread_off);
+
+/* Fused deapodise + zero-pad + centred FFT: image (C, nimg, N...) -> grid.
+ * For grid sizes 2^a 3^b 5^c (<= 4096 per axis) a pruned FFT transforms only
+ * the lines that hold image data; otherwise pf_deapod_pad + pf_fft.
+ * Replaces src/pyfourier/_nufft/_nufft.py:234-241.  fft_workspace: the FFT
+ * scratch part of the pf_nufft workspace (pf_workspace_offsets). */
+int pf_pad_fft(pf_plan* plan, const pf_batch_desc* desc, const void* image, void* grid,
+               void* fft_workspace, void* stream);
+
+/* Fused inverse centred FFT + crop + deapodise (times `factor`): grid ->
+ * image; the grid is overwritten.  Replaces src/pyfourier/_nufft/_nufft.py:275-282. */
+int pf_ifft_crop(pf_plan* plan, const pf_batch_desc* desc, void* grid, void* image, float factor,
+                 void* fft_workspace, void* stream);
+
 /* Crop, fftshift and deapodise (times `factor`).
  * Replaces _resize + _apodize, src/pyfourier/_nufft/_nufft.py:278-282. */
 int pf_crop_deapod(const pf_plan* plan, const pf_batch_desc* desc, const void* grid, void* image,

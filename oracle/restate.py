@@ -90,9 +90,8 @@ def kb_beta(width, oversamp):
     w = np.asarray(width, dtype=np.int16)
     s = np.asarray(oversamp, dtype=np.float32)
     inner = ((w / s) * (s - np.float32(0.5))) ** 2 - np.float32(0.8)
-    return (np.float32(math.pi) * inner.astype(np.float32) ** np.float32(0.5)).astype(
-        np.float32
-    )
+    with np.errstate(invalid="ignore"):  # NaN on width-1 axes, never used
+        return (np.float32(math.pi) * inner.astype(np.float32) ** np.float32(0.5)).astype(np.float32)
 
 
 def os_shape(shape, oversamp):
@@ -116,21 +115,23 @@ def kernel_scale(beta, width):
 
 
 def normalize_coordinates(coord, shape_xyz):
-    """Map coordinates of any unit to grid units [-N/2, N/2).
+    """Map coordinates to grid units [-N/2, N/2).
 
-    Intent of _subroutines/_traj.py:10-54 with defect A.11 fixed: a single
-    global radius (max |k| over all points) is mapped to N/2, instead of each
-    point being divided by its own radius.  For [-pi, pi) input this is
-    k*N/(2*pi).
+    Intent of _subroutines/_traj.py:10-54 with defect A.11 fixed (the
+    reference divides each point by its own radius).  The unit is detected
+    once from the largest radius r: r <= 0.5 -> cycles/FOV (times N),
+    r <= pi -> radians (times N/(2 pi)), otherwise grid units (unchanged).
     """
     coord = np.asarray(coord, dtype=np.float32)
     shape_xyz = np.asarray(shape_xyz, dtype=np.float32)
-    cabs = float(np.sqrt((coord.astype(np.float64) ** 2).sum(axis=-1)).max())
-    if cabs == 0.0:
+    r = float(np.sqrt((coord.astype(np.float64) ** 2).sum(axis=-1)).max()) if coord.size else 0.0
+    if r <= 0.5 + 1e-5:
+        f = 1.0
+    elif r <= math.pi + 1e-5:
+        f = 1.0 / (2.0 * math.pi)
+    else:
         return coord.copy()
-    if (coord < 0).any():
-        return (np.float32(0.5) * coord / np.float32(cabs)) * shape_xyz
-    return (coord / np.float32(cabs)) * shape_xyz - np.floor(shape_xyz / 2)
+    return coord * (shape_xyz * np.float32(f))
 
 
 def scale_coord(coord, shape_xyz, oversamp_xyz):

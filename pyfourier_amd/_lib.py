@@ -78,6 +78,12 @@ _SIG = {
     "pf_spread": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
     "pf_spread_workspace_size": (_I, [_VP, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_size_t)]),
     "pf_crop_deapod": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, ctypes.c_float, _VP]),
+    "pf_workspace_offsets": (
+        _I,
+        [_VP, ctypes.POINTER(BatchDesc)] + [ctypes.POINTER(ctypes.c_size_t)] * 4,
+    ),
+    "pf_pad_fft": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
+    "pf_ifft_crop": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, ctypes.c_float, _VP, _VP]),
     "pf_sort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
     "pf_unsort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, ctypes.c_float, _VP]),
 }

@@ -180,21 +180,19 @@ class _Call:
 
     def _split_ws(self, desc, ws):
         L = _lib.lib()
-        gb = ctypes.c_size_t()
-        _lib.check(L.pf_grid_size(self.native.handle, ctypes.byref(desc), ctypes.byref(gb)))
-        g = (gb.value + 255) // 256 * 256
-        C = desc.batch
-        sb = (self.interp.nframes * self.interp.npts * C * 8 + 255) // 256 * 256
-        return ws[:g], ws[g : g + sb], ws[g + sb :]
+        off = [ctypes.c_size_t() for _ in range(4)]
+        _lib.check(L.pf_workspace_offsets(self.native.handle, ctypes.byref(desc), *[ctypes.byref(o) for o in off]))
+        g, s, f, p = (o.value for o in off)
+        return ws[g:s], ws[s:f], ws[f:p], ws[p:]
 
     def _staged_forward(self, desc, x, y, ws, norm_code):
         L = _lib.lib()
         h, d, st = self.native.handle, ctypes.byref(desc), _lib.stream_of(self.device)
-        grid, ys, _ = self._split_ws(desc, ws)
+        grid, ys, fws, _ = self._split_ws(desc, ws)
         t = stage_timer.active
         dev = self.device
-        t.record("deapod_pad", lambda: _lib.check(L.pf_deapod_pad(h, d, _lib.ptr(x), _lib.ptr(grid), st)), dev)
-        t.record("fft_fwd", lambda: _lib.check(L.pf_fft(h, d, _lib.ptr(grid), 0, st)), dev)
+        t.record("pad_fft", lambda: _lib.check(
+            L.pf_pad_fft(h, d, _lib.ptr(x), _lib.ptr(grid), _lib.ptr(fws), st)), dev)
         t.record("interp", lambda: _lib.check(L.pf_interp(h, d, _lib.ptr(grid), _lib.ptr(ys), st)), dev)
         fac = self._factor(norm_code, False)
         t.record("unsort", lambda: _lib.check(
@@ -203,17 +201,16 @@ class _Call:
     def _staged_adjoint(self, desc, y, x, ws, norm_code):
         L = _lib.lib()
         h, d, st = self.native.handle, ctypes.byref(desc), _lib.stream_of(self.device)
-        grid, ys, part = self._split_ws(desc, ws)
+        grid, ys, fws, part = self._split_ws(desc, ws)
         t = stage_timer.active
         dev = self.device
         t.record("sort", lambda: _lib.check(
             L.pf_sort_samples(h, d, _lib.ptr(y), _lib.ptr(ys), _lib.ptr(self.weight_flat), st)), dev)
         t.record("spread", lambda: _lib.check(
             L.pf_spread(h, d, _lib.ptr(ys), _lib.ptr(grid), _lib.ptr(part), st)), dev)
-        t.record("fft_adj", lambda: _lib.check(L.pf_fft(h, d, _lib.ptr(grid), 1, st)), dev)
         fac = self._factor(norm_code, True)
-        t.record("crop_deapod", lambda: _lib.check(
-            L.pf_crop_deapod(h, d, _lib.ptr(grid), _lib.ptr(x), fac, st)), dev)
+        t.record("ifft_crop", lambda: _lib.check(
+            L.pf_ifft_crop(h, d, _lib.ptr(grid), _lib.ptr(x), fac, _lib.ptr(fws), st)), dev)
 
     def adjoint(self, y: torch.Tensor, norm_code: int) -> torch.Tensor:
         C = y.shape[0]

@@ -149,7 +149,8 @@ def kb_beta(width, oversamp) -> np.ndarray:
     w = np.asarray(width, dtype=np.int16)
     s = np.asarray(oversamp, dtype=np.float32)
     inner = (((w / s) * (s - np.float32(0.5))) ** 2 - np.float32(0.8)).astype(np.float32)
-    return (np.float32(math.pi) * inner ** np.float32(0.5)).astype(np.float32)
+    with np.errstate(invalid="ignore"):  # NaN on width-1 (Cartesian) axes, unused there
+        return (np.float32(math.pi) * inner ** np.float32(0.5)).astype(np.float32)
 
 
 def _kb_host(x, beta):

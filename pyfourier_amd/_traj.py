@@ -1,16 +1,37 @@
 """Trajectory normalisation.
 
-Intent of src/pyfourier/_subroutines/_traj.py:10-54: map coordinates of any
-unit (1/m, rad, [-0.5, 0.5], [-pi, pi)) to grid units [-N/2, N/2).  The
-reference divides every point by its *own* radius (defect A.11), which
-projects all samples onto one circle; here one global radius (the largest
-|k| of the trajectory) is mapped to N/2, so [-pi, pi) input becomes k*N/(2 pi)
-and input already in grid units with its largest radius at N/2 is unchanged.
+Intent of src/pyfourier/_subroutines/_traj.py:10-54: accept coordinates in
+any of the usual units and return grid units [-N/2, N/2).  The reference
+divides every point by its *own* radius (defect A.11), which projects all
+samples onto the circle |k| = N/2.  Here the unit is detected once for the
+whole trajectory from its largest radius r = max |k|:
+
+* r <= 0.5 (+1e-5)      cycles per field of view [-0.5, 0.5): k * N
+* r <= pi  (+1e-5)      radians [-pi, pi):                    k * N / (2 pi)
+* otherwise             already grid units:                   unchanged
+
+so radial and spiral trajectories in radians map exactly like the reference
+intends (largest radius pi -> N/2) and trajectories already in grid units,
+including mixed Cartesian/non-Cartesian ones (stack of stars), pass through
+unchanged.
 """
 
 from __future__ import annotations
 
+import math
+
 import torch
+
+_TOL = 1e-5
+
+
+def coordinate_scale(rmax: float) -> float:
+    """Factor (per unit of N) that maps the detected unit to grid units."""
+    if rmax <= 0.5 + _TOL:
+        return 1.0
+    if rmax <= math.pi + _TOL:
+        return 1.0 / (2.0 * math.pi)
+    return 0.0  # grid units
 
 
 def normalize_coordinates(coord: torch.Tensor, shape_xyz, center: bool = True) -> torch.Tensor:
@@ -18,19 +39,13 @@ def normalize_coordinates(coord: torch.Tensor, shape_xyz, center: bool = True) -
 
     coord: (..., d) with the last axis ordered (x, y, z).
     shape_xyz: matrix size per axis, ordered like the last axis of coord.
+    center: False shifts the result to [0, N) (reference keyword).
     """
     coord = coord.to(torch.float32)
     shape = torch.as_tensor(list(shape_xyz), dtype=torch.float32, device=coord.device)
-    cabs = coord.double().pow(2).sum(dim=-1).sqrt().max()
-    if float(cabs) == 0.0:
-        return coord.clone()
-    cabs = cabs.to(torch.float32)
-    if bool((coord < 0).any()):
-        out = (0.5 * coord / cabs) * shape
-        if not center:
-            out = out + torch.floor(shape / 2)
-        return out
-    out = (coord / cabs) * shape
-    if center:
-        out = out - torch.floor(shape / 2)
+    rmax = float(coord.double().pow(2).sum(dim=-1).sqrt().max()) if coord.numel() else 0.0
+    f = coordinate_scale(rmax)
+    out = coord.clone() if f == 0.0 else coord * (shape * f)
+    if not center:
+        out = out + torch.floor(shape / 2)
     return out

@@ -22,7 +22,7 @@ ROOT = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "libpfhip.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["pf_plan.hip", "pf_ops.hip", "pf_fft.cpp", "pf_api.cpp"]
+SOURCES = ["pf_plan.hip", "pf_ops.hip", "pf_fftk.hip", "pf_fft.cpp", "pf_api.cpp"]
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 
 
@@ -49,6 +49,9 @@ def compile_one(src: str) -> str:
     path = os.path.join(HERE, src)
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     cmd = [hipcc(), "-c", path, "-o", obj] + common_flags()
+    if src == "pf_plan.hip":
+        # plan arithmetic must round exactly like the reference (no FMA contraction)
+        cmd += ["-ffp-contract=off"]
     if src.endswith(".hip"):
         cmd += [
             f"--offload-arch={ARCH}",

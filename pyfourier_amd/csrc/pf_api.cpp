@@ -58,6 +58,23 @@ size_t samples_bytes(const pf_plan* p, const pf_batch& b) { return (size_t)p->mt
 
 double ghat(const pf_plan* p) { return (double)pf_gtot(p); }
 
+// workspace of the fused operators: [grid | sorted samples | FFT scratch | spread partial tiles]
+struct WsLayout {
+    size_t grid, samples, fft, spread, fwd_total, adj_total;
+};
+
+WsLayout layout(const pf_plan* p, const pf_batch& b) {
+    WsLayout w;
+    w.grid = 0;
+    w.samples = align_up(grid_bytes(p, b));
+    w.fft = w.samples + align_up(samples_bytes(p, b));
+    size_t extra = pf_fftk_supported(p) ? pf_fftk_extra_bytes(p, b) : 0;
+    w.spread = w.fft + align_up(extra);
+    w.fwd_total = w.spread;
+    w.adj_total = w.spread + align_up(pf_spread_partial_bytes(p, b));
+    return w;
+}
+
 }  // namespace
 
 extern "C" {
@@ -174,9 +191,8 @@ int pf_workspace_size(const pf_plan* p, const pf_batch_desc* d, int op, size_t* 
     if (rc != PF_OK) return rc;
     if (!bytes) return pf_fail(PF_ERR_ARG, "null output");
     if (op != 0 && op != 1) return pf_fail(PF_ERR_ARG, "op must be 0 (forward) or 1 (adjoint)");
-    size_t n = align_up(grid_bytes(p, b)) + align_up(samples_bytes(p, b));
-    if (op == 1) n += align_up(pf_spread_partial_bytes(p, b));
-    *bytes = n;
+    WsLayout w = layout(p, b);
+    *bytes = op == 0 ? w.fwd_total : w.adj_total;
     return PF_OK;
 }
 
@@ -246,6 +262,53 @@ int pf_unsort_samples(const pf_plan* p, const pf_batch_desc* d, const void* ksp_
                             factor, static_cast<hipStream_t>(stream));
 }
 
+int pf_workspace_offsets(const pf_plan* p, const pf_batch_desc* d, size_t* grid_off, size_t* samples_off,
+                         size_t* fft_off, size_t* spread_off) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!grid_off || !samples_off || !fft_off || !spread_off) return pf_fail(PF_ERR_ARG, "null output");
+    WsLayout w = layout(p, b);
+    *grid_off = w.grid;
+    *samples_off = w.samples;
+    *fft_off = w.fft;
+    *spread_off = w.spread;
+    return PF_OK;
+}
+
+int pf_pad_fft(pf_plan* p, const pf_batch_desc* d, const void* image, void* grid, void* fft_workspace,
+               void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!image || !grid) return pf_fail(PF_ERR_ARG, "null array");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (pf_fftk_supported(p)) {
+        if (pf_fftk_extra_bytes(p, b) && !fft_workspace) return pf_fail(PF_ERR_ARG, "missing FFT workspace");
+        return pf_fftk_forward(p, b, static_cast<const float2*>(image), static_cast<float2*>(grid),
+                               static_cast<float2*>(fft_workspace), s);
+    }
+    if ((rc = pf_launch_deapod_pad(p, b, static_cast<const float2*>(image), static_cast<float2*>(grid), s)) != PF_OK)
+        return rc;
+    return pf_fft_exec(p, b, static_cast<float2*>(grid), 0, s);
+}
+
+int pf_ifft_crop(pf_plan* p, const pf_batch_desc* d, void* grid, void* image, float factor, void* fft_workspace,
+                 void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!image || !grid) return pf_fail(PF_ERR_ARG, "null array");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (pf_fftk_supported(p)) {
+        if (pf_fftk_extra_bytes(p, b) && !fft_workspace) return pf_fail(PF_ERR_ARG, "missing FFT workspace");
+        return pf_fftk_adjoint(p, b, static_cast<float2*>(grid), static_cast<float2*>(image),
+                               static_cast<float2*>(fft_workspace), factor, s);
+    }
+    if ((rc = pf_fft_exec(p, b, static_cast<float2*>(grid), 1, s)) != PF_OK) return rc;
+    return pf_launch_crop_deapod(p, b, static_cast<const float2*>(grid), static_cast<float2*>(image), factor, s);
+}
+
 int pf_nufft(pf_plan* p, const pf_batch_desc* d, const void* image, void* kspace, const float* weight, int norm,
              void* workspace, size_t workspace_bytes, void* stream) {
     pf_batch b;
@@ -253,16 +316,22 @@ int pf_nufft(pf_plan* p, const pf_batch_desc* d, const void* image, void* kspace
     if (rc != PF_OK) return rc;
     if (!image || !kspace) return pf_fail(PF_ERR_ARG, "null array");
     if (norm < PF_NORM_NONE || norm > PF_NORM_ADJOINT) return pf_fail(PF_ERR_ARG, "norm must be 0, 1 or 2");
-    size_t need = align_up(grid_bytes(p, b)) + align_up(samples_bytes(p, b));
-    if (!workspace || workspace_bytes < need) return pf_fail(PF_ERR_ARG, "workspace too small");
+    WsLayout w = layout(p, b);
+    if (!workspace || workspace_bytes < w.fwd_total) return pf_fail(PF_ERR_ARG, "workspace too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    float2* grid = static_cast<float2*>(workspace);
-    float2* ys = reinterpret_cast<float2*>(static_cast<char*>(workspace) + align_up(grid_bytes(p, b)));
+    char* ws = static_cast<char*>(workspace);
+    float2* grid = reinterpret_cast<float2*>(ws + w.grid);
+    float2* ys = reinterpret_cast<float2*>(ws + w.samples);
+    float2* extra = reinterpret_cast<float2*>(ws + w.fft);
     double factor = 1.0 / p->scale;
     if (norm == PF_NORM_ORTHO) factor /= std::sqrt(ghat(p));
     if (norm == PF_NORM_ADJOINT) factor /= ghat(p);
-    if ((rc = pf_launch_deapod_pad(p, b, static_cast<const float2*>(image), grid, s)) != PF_OK) return rc;
-    if ((rc = pf_fft_exec(p, b, grid, 0, s)) != PF_OK) return rc;
+    if (pf_fftk_supported(p)) {
+        if ((rc = pf_fftk_forward(p, b, static_cast<const float2*>(image), grid, extra, s)) != PF_OK) return rc;
+    } else {
+        if ((rc = pf_launch_deapod_pad(p, b, static_cast<const float2*>(image), grid, s)) != PF_OK) return rc;
+        if ((rc = pf_fft_exec(p, b, grid, 0, s)) != PF_OK) return rc;
+    }
     if ((rc = pf_launch_gather(p, b, grid, ys, s)) != PF_OK) return rc;
     return pf_launch_unsort(p, b, ys, static_cast<float2*>(kspace), weight, (float)factor, s);
 }
@@ -274,19 +343,21 @@ int pf_nufft_adj(pf_plan* p, const pf_batch_desc* d, const void* kspace, void* i
     if (rc != PF_OK) return rc;
     if (!image || !kspace) return pf_fail(PF_ERR_ARG, "null array");
     if (norm < PF_NORM_NONE || norm > PF_NORM_ADJOINT) return pf_fail(PF_ERR_ARG, "norm must be 0, 1 or 2");
-    size_t gb = align_up(grid_bytes(p, b)), sb = align_up(samples_bytes(p, b));
-    size_t need = gb + sb + align_up(pf_spread_partial_bytes(p, b));
-    if (!workspace || workspace_bytes < need) return pf_fail(PF_ERR_ARG, "workspace too small");
+    WsLayout w = layout(p, b);
+    if (!workspace || workspace_bytes < w.adj_total) return pf_fail(PF_ERR_ARG, "workspace too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(workspace);
-    float2* grid = reinterpret_cast<float2*>(ws);
-    float2* ys = reinterpret_cast<float2*>(ws + gb);
-    float2* part = reinterpret_cast<float2*>(ws + gb + sb);
+    float2* grid = reinterpret_cast<float2*>(ws + w.grid);
+    float2* ys = reinterpret_cast<float2*>(ws + w.samples);
+    float2* extra = reinterpret_cast<float2*>(ws + w.fft);
+    float2* part = reinterpret_cast<float2*>(ws + w.spread);
     double factor = 1.0 / p->scale;
     if (norm == PF_NORM_ORTHO) factor /= std::sqrt(ghat(p));
     if (norm == PF_NORM_NONE) factor /= ghat(p);
     if ((rc = pf_launch_sort(p, b, static_cast<const float2*>(kspace), ys, weight, s)) != PF_OK) return rc;
     if ((rc = pf_launch_spread(p, b, ys, grid, part, s)) != PF_OK) return rc;
+    if (pf_fftk_supported(p))
+        return pf_fftk_adjoint(p, b, grid, static_cast<float2*>(image), extra, (float)factor, s);
     if ((rc = pf_fft_exec(p, b, grid, 1, s)) != PF_OK) return rc;
     return pf_launch_crop_deapod(p, b, grid, static_cast<float2*>(image), (float)factor, s);
 }

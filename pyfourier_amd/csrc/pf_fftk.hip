@@ -1,0 +1,568 @@
+// Pruned, fused Cartesian FFT stage of the NUFFT (gfx950).
+//
+// Forward  (replaces _apodize + _resize + centred fftn, src/pyfourier/_nufft/_nufft.py:234-241):
+//   X pass   image rows (coil-major user layout) -> apodise (x B0 map) ->
+//            place at the stored positions (p - N/2) mod G (pad + ifftshift)
+//            -> FFT along x -> T1 (nfg, Cb, Nz, Ny, Gx, R*Cv)
+//   Y pass   T1 (Ny compact rows) -> zero-padded FFT along y -> T2 / grid
+//   Z pass   T2 (Nz compact planes) -> zero-padded FFT along z -> grid
+// Only rows that hold image data are transformed: in 3-D at sigma=2 the X pass
+// works on 1/4 and the Y pass on 1/2 of the grid's lines.
+//
+// Adjoint  (replaces centred ifftn + _resize + _apodize, src/pyfourier/_nufft/_nufft.py:276-282):
+//   the mirror image, each pass keeping only the N outputs the crop needs,
+//   the X pass writing the deapodised image (sum over B0 segments with the
+//   conjugate maps) in the user layout.
+//
+// Each block transforms B columns of one line family held in LDS as
+// [G][B+1] complex, with an in-place mixed-radix (8, 5, 4, 3, 2) Stockham
+// FFT (Govindaraju et al., "High performance discrete Fourier transforms on
+// graphics processors", SC 2008); twiddles come from a per-axis table of
+// exp(-2 pi i k / G) computed in double on the host.  The inverse uses
+// conj(FFT(conj(x))).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pf_internal.h"
+
+namespace {
+
+constexpr int FT = 256;       // threads per block
+constexpr int GBMAX = 4096;   // complex values per block (G * B)
+
+struct AxisFFT {
+    int G;
+    int N;
+    int nrad;
+    int rad[16];
+    const float2* tw;  // (G,) exp(-2 pi i k / G)
+};
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 c_mulc(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 c_negi(float2 a) { return make_float2(a.y, -a.x); }  // -i * a
+__device__ __forceinline__ float2 c_posi(float2 a) { return make_float2(-a.y, a.x); }  // +i * a
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// forward DFTs (kernel exp(-2 pi i r k / R)), in place on v[0..R)
+__device__ __forceinline__ void dft2(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = c_add(a, b);
+    v[1] = c_sub(a, b);
+}
+__device__ __forceinline__ void dft3(float2* v) {
+    const float s = 0.86602540378443864676f;
+    float2 t = c_add(v[1], v[2]);
+    float2 d = c_sub(v[1], v[2]);
+    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    float2 r = c_negi(c_scale(d, s));
+    v[0] = c_add(v[0], t);
+    v[1] = c_add(m, r);
+    v[2] = c_sub(m, r);
+}
+__device__ __forceinline__ void dft4(float2* v) {
+    float2 a = c_add(v[0], v[2]), b = c_sub(v[0], v[2]);
+    float2 c = c_add(v[1], v[3]), d = c_negi(c_sub(v[1], v[3]));
+    v[0] = c_add(a, c);
+    v[2] = c_sub(a, c);
+    v[1] = c_add(b, d);
+    v[3] = c_sub(b, d);
+}
+__device__ __forceinline__ void dft5(float2* v) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    float2 p14 = c_add(v[1], v[4]), m14 = c_sub(v[1], v[4]);
+    float2 p23 = c_add(v[2], v[3]), m23 = c_sub(v[2], v[3]);
+    float2 x0 = v[0];
+    float2 a1 = make_float2(x0.x + c1 * p14.x + c2 * p23.x, x0.y + c1 * p14.y + c2 * p23.y);
+    float2 a2 = make_float2(x0.x + c2 * p14.x + c1 * p23.x, x0.y + c2 * p14.y + c1 * p23.y);
+    float2 b1 = make_float2(s1 * m14.x + s2 * m23.x, s1 * m14.y + s2 * m23.y);
+    float2 b2 = make_float2(s2 * m14.x - s1 * m23.x, s2 * m14.y - s1 * m23.y);
+    v[0] = make_float2(x0.x + p14.x + p23.x, x0.y + p14.y + p23.y);
+    v[1] = c_add(a1, c_negi(b1));
+    v[4] = c_add(a1, c_posi(b1));
+    v[2] = c_add(a2, c_negi(b2));
+    v[3] = c_add(a2, c_posi(b2));
+}
+__device__ __forceinline__ void dft8(float2* v) {
+    const float h = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft4(e);
+    dft4(o);
+    float2 o1 = make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));   // W8^1 = (1-i)/sqrt2
+    float2 o2 = c_negi(o[2]);                                               // W8^2 = -i
+    float2 o3 = make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));  // W8^3 = (-1-i)/sqrt2
+    v[0] = c_add(e[0], o[0]);
+    v[4] = c_sub(e[0], o[0]);
+    v[1] = c_add(e[1], o1);
+    v[5] = c_sub(e[1], o1);
+    v[2] = c_add(e[2], o2);
+    v[6] = c_sub(e[2], o2);
+    v[3] = c_add(e[3], o3);
+    v[7] = c_sub(e[3], o3);
+}
+
+template <int R>
+__device__ __forceinline__ void dftR(float2* v) {
+    if constexpr (R == 2) dft2(v);
+    else if constexpr (R == 3) dft3(v);
+    else if constexpr (R == 4) dft4(v);
+    else if constexpr (R == 5) dft5(v);
+    else dft8(v);
+}
+
+// One in-place Stockham pass of radix R over B columns (stride LD) of length G.
+template <int R>
+__device__ void stockham_pass(float2* buf, int LD, int B, int G, int Ns, const float2* tw) {
+    constexpr int CMAX = (GBMAX + R * FT - 1) / (R * FT);
+    const int nb = G / R;
+    const int total = nb * B;
+    const int tstep = G / (Ns * R);
+    float2 v[CMAX][R];
+#pragma unroll
+    for (int c = 0; c < CMAX; ++c) {
+        int q = threadIdx.x + c * FT;
+        if (q < total) {
+            int b = q % B, j = q / B;
+            int k = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float2 x = buf[(j + r * nb) * LD + b];
+                v[c][r] = r == 0 ? x : c_mul(x, tw[r * k * tstep]);
+            }
+            dftR<R>(v[c]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CMAX; ++c) {
+        int q = threadIdx.x + c * FT;
+        if (q < total) {
+            int b = q % B, j = q / B;
+            int k = j % Ns;
+            int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[(d + r * Ns) * LD + b] = v[c][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full FFT of the LDS block.  Caller has synchronised after filling it.
+__device__ void lds_fft(float2* buf, int LD, int B, const AxisFFT& A, const float2* tw) {
+    int Ns = 1;
+    for (int p = 0; p < A.nrad; ++p) {
+        switch (A.rad[p]) {
+            case 8: stockham_pass<8>(buf, LD, B, A.G, Ns, tw); break;
+            case 5: stockham_pass<5>(buf, LD, B, A.G, Ns, tw); break;
+            case 4: stockham_pass<4>(buf, LD, B, A.G, Ns, tw); break;
+            case 3: stockham_pass<3>(buf, LD, B, A.G, Ns, tw); break;
+            default: stockham_pass<2>(buf, LD, B, A.G, Ns, tw); break;
+        }
+        Ns *= A.rad[p];
+    }
+}
+
+__device__ __forceinline__ int stored_pos(int p, int N, int G) {
+    int n = p - N / 2;
+    return n < 0 ? n + G : n;
+}
+
+struct Geo {
+    int N[3], G[3];   // (z, y, x)
+    int64_t ntot;     // Nz*Ny*Nx
+};
+
+struct Batch {
+    int C, Cv, Cb, R, L, K, nfg, nimg;
+    const float2* b0s;  // (L, ntot)
+};
+
+// ---------------------------------------------------------------------------
+// X pass.  A block takes YG compact image rows (fg, cb, z, y) and a chunk of
+// KC (coefficient, coil) pairs kc = k*Cv + cv with all L B0 segments, so the
+// adjoint can sum the segments in-block.  Column j of a row holds value
+// v = l*(K*Cv) + kc of the grid vector (v = r*Cv + cv, r = l*K + k).
+// ---------------------------------------------------------------------------
+struct XBlk {
+    int YG;      // rows per block
+    int KC;      // (k, cv) pairs per block
+    int nkc;     // chunks over K*Cv
+};
+
+__device__ __forceinline__ void x_row_decode(const Batch& bt, int64_t row, int64_t nrows, int& fg, int& cb,
+                                             int64_t& zy) {
+    zy = row % nrows;
+    int64_t gc = row / nrows;
+    cb = (int)(gc % bt.Cb);
+    fg = (int)(gc / bt.Cb);
+}
+
+__global__ void __launch_bounds__(FT) k_fftx_fwd(Geo g, Batch bt, AxisFFT A, const float* __restrict__ apod,
+                                                 const float2* __restrict__ img, float2* __restrict__ out, XBlk xb) {
+    extern __shared__ float2 sh[];
+    const int RC = bt.R * bt.Cv;
+    const int KCv = bt.K * bt.Cv;
+    const int VC = bt.L * xb.KC;
+    const int B = xb.YG * VC, LD = B + 1;
+    const int G = A.G, N = A.N;
+    float2* buf = sh;
+    float2* tw = sh + G * LD;
+    const int64_t nrows = (int64_t)g.N[0] * g.N[1];
+    const int64_t rows_tot = (int64_t)bt.nfg * bt.Cb * nrows;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t row0 = (blk / xb.nkc) * xb.YG;
+    const int kc0 = (int)(blk % xb.nkc) * xb.KC;
+    for (int i = threadIdx.x; i < G; i += FT) tw[i] = A.tw[i];
+    for (int i = threadIdx.x; i < G * LD; i += FT) buf[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const float* apz = apod;
+    const float* apy = apod + g.N[0];
+    const float* apx = apod + g.N[0] + g.N[1];
+    for (int q = threadIdx.x; q < B * N; q += FT) {
+        const int p = q % N;
+        const int col = q / N;
+        const int yy = col / VC, j = col % VC;
+        const int l = j / xb.KC, kc = kc0 + j % xb.KC;
+        const int64_t row = row0 + yy;
+        if (row >= rows_tot || kc >= KCv) continue;
+        int fg, cb;
+        int64_t zy;
+        x_row_decode(bt, row, nrows, fg, cb, zy);
+        const int k = kc / bt.Cv, cv = kc % bt.Cv;
+        const int z = (int)(zy / g.N[1]), y = (int)(zy % g.N[1]);
+        const int ii = bt.nfg > 1 ? fg : (bt.nimg > 1 ? k : 0);
+        const int c = cb * bt.Cv + cv;
+        const int64_t pix = zy * N + p;
+        float2 x = img[((int64_t)c * bt.nimg + ii) * g.ntot + pix];
+        x = c_scale(x, apz[z] * apy[y] * apx[p]);
+        if (bt.b0s) x = c_mul(x, bt.b0s[(int64_t)l * g.ntot + pix]);
+        buf[stored_pos(p, N, G) * LD + col] = x;
+    }
+    __syncthreads();
+    lds_fft(buf, LD, B, A, tw);
+    for (int q = threadIdx.x; q < xb.YG * G * VC; q += FT) {
+        const int j = q % VC;
+        const int n = (q / VC) % G;
+        const int yy = q / (VC * G);
+        const int l = j / xb.KC, kc = kc0 + j % xb.KC;
+        const int64_t row = row0 + yy;
+        if (row >= rows_tot || kc >= KCv) continue;
+        out[(row * G + n) * RC + l * KCv + kc] = buf[n * LD + yy * VC + j];
+    }
+}
+
+__global__ void __launch_bounds__(FT) k_fftx_adj(Geo g, Batch bt, AxisFFT A, const float* __restrict__ apod,
+                                                 const float2* __restrict__ in, float2* __restrict__ img, XBlk xb,
+                                                 float factor) {
+    extern __shared__ float2 sh[];
+    const int RC = bt.R * bt.Cv;
+    const int KCv = bt.K * bt.Cv;
+    const int VC = bt.L * xb.KC;
+    const int B = xb.YG * VC, LD = B + 1;
+    const int G = A.G, N = A.N;
+    float2* buf = sh;
+    float2* tw = sh + G * LD;
+    const int64_t nrows = (int64_t)g.N[0] * g.N[1];
+    const int64_t rows_tot = (int64_t)bt.nfg * bt.Cb * nrows;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t row0 = (blk / xb.nkc) * xb.YG;
+    const int kc0 = (int)(blk % xb.nkc) * xb.KC;
+    for (int i = threadIdx.x; i < G; i += FT) tw[i] = A.tw[i];
+    for (int q = threadIdx.x; q < xb.YG * G * VC; q += FT) {
+        const int j = q % VC;
+        const int n = (q / VC) % G;
+        const int yy = q / (VC * G);
+        const int l = j / xb.KC, kc = kc0 + j % xb.KC;
+        const int64_t row = row0 + yy;
+        float2 x = make_float2(0.f, 0.f);
+        if (row < rows_tot && kc < KCv) x = in[(row * G + n) * RC + l * KCv + kc];
+        buf[n * LD + yy * VC + j] = make_float2(x.x, -x.y);  // conj: inverse via forward FFT
+    }
+    __syncthreads();
+    lds_fft(buf, LD, B, A, tw);
+    const float* apz = apod;
+    const float* apy = apod + g.N[0];
+    const float* apx = apod + g.N[0] + g.N[1];
+    for (int q = threadIdx.x; q < xb.YG * xb.KC * N; q += FT) {
+        const int p = q % N;
+        const int kcl = (q / N) % xb.KC;
+        const int yy = q / (N * xb.KC);
+        const int kc = kc0 + kcl;
+        const int64_t row = row0 + yy;
+        if (row >= rows_tot || kc >= KCv) continue;
+        int fg, cb;
+        int64_t zy;
+        x_row_decode(bt, row, nrows, fg, cb, zy);
+        const int k = kc / bt.Cv, cv = kc % bt.Cv;
+        const int z = (int)(zy / g.N[1]), y = (int)(zy % g.N[1]);
+        const int64_t pix = zy * N + p;
+        const int n = stored_pos(p, N, G);
+        float2 acc = make_float2(0.f, 0.f);
+        for (int l = 0; l < bt.L; ++l) {
+            float2 x = buf[n * LD + yy * VC + l * xb.KC + kcl];
+            x = make_float2(x.x, -x.y);
+            if (bt.b0s) x = c_mulc(x, bt.b0s[(int64_t)l * g.ntot + pix]);
+            acc = c_add(acc, x);
+        }
+        const int ii = bt.nfg > 1 ? fg : (bt.nimg > 1 ? k : 0);
+        const int c = cb * bt.Cv + cv;
+        img[((int64_t)c * bt.nimg + ii) * g.ntot + pix] = c_scale(acc, factor * apz[z] * apy[y] * apx[p]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass along a strided axis.  Buffer view: (O, rows, I) with I the
+// contiguous inner extent; a block takes B inner columns of one outer index.
+// fwd: N compact input rows -> G output rows (zero-padded FFT)
+// adj: G input rows -> inverse FFT -> N compact output rows
+// ---------------------------------------------------------------------------
+template <bool FWD>
+__global__ void __launch_bounds__(FT) k_fftcol(AxisFFT A, int64_t O, int64_t I, int B,
+                                               const float2* __restrict__ in, float2* __restrict__ out) {
+    extern __shared__ float2 sh[];
+    const int G = A.G, N = A.N;
+    const int LD = B + 1;
+    float2* buf = sh;
+    float2* tw = sh + G * LD;
+    const int64_t nchunk = (I + B - 1) / B;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t o = blk / nchunk;
+    const int64_t i0 = (blk % nchunk) * B;
+    const int bw = (int)std::min<int64_t>(B, I - i0);
+    for (int i = threadIdx.x; i < G; i += FT) tw[i] = A.tw[i];
+    if (FWD) {
+        for (int i = threadIdx.x; i < G * LD; i += FT) buf[i] = make_float2(0.f, 0.f);
+        __syncthreads();
+        const float2* src = in + o * (int64_t)N * I + i0;
+        for (int q = threadIdx.x; q < N * B; q += FT) {
+            int b = q % B, p = q / B;
+            if (b < bw) buf[stored_pos(p, N, G) * LD + b] = src[(int64_t)p * I + b];
+        }
+    } else {
+        const float2* src = in + o * (int64_t)G * I + i0;
+        for (int q = threadIdx.x; q < G * B; q += FT) {
+            int b = q % B, n = q / B;
+            float2 x = b < bw ? src[(int64_t)n * I + b] : make_float2(0.f, 0.f);
+            buf[n * LD + b] = make_float2(x.x, -x.y);
+        }
+    }
+    __syncthreads();
+    lds_fft(buf, LD, B, A, tw);
+    if (FWD) {
+        float2* dst = out + o * (int64_t)G * I + i0;
+        for (int q = threadIdx.x; q < G * B; q += FT) {
+            int b = q % B, n = q / B;
+            if (b < bw) dst[(int64_t)n * I + b] = buf[n * LD + b];
+        }
+    } else {
+        float2* dst = out + o * (int64_t)N * I + i0;
+        for (int q = threadIdx.x; q < N * B; q += FT) {
+            int b = q % B, p = q / B;
+            if (b < bw) {
+                float2 x = buf[stored_pos(p, N, G) * LD + b];
+                dst[(int64_t)p * I + b] = make_float2(x.x, -x.y);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct pf_fftk_axis {
+    int G = 0, N = 0;
+    std::vector<int> rad;
+    float2* tw = nullptr;
+};
+
+struct pf_fftk {
+    pf_fftk_axis ax[3];
+};
+
+static bool factorize(int G, std::vector<int>& rad) {
+    rad.clear();
+    int g = G;
+    for (int r : {8, 5, 4, 3, 2})
+        while (g % r == 0) {
+            rad.push_back(r);
+            g /= r;
+        }
+    return g == 1 && (int)rad.size() <= 16;
+}
+
+bool pf_fftk_supported(const pf_plan* p) {
+    std::vector<int> rad;
+    for (int a = 3 - p->ndim; a < 3; ++a) {
+        if (p->os[a] > GBMAX || !factorize(p->os[a], rad)) return false;
+        if (p->shape[a] > p->os[a]) return false;
+    }
+    return true;
+}
+
+int pf_fftk_init(pf_plan* p) {
+    if (p->fftk) return PF_OK;
+    pf_fftk* f = new pf_fftk();
+    for (int a = 3 - p->ndim; a < 3; ++a) {
+        pf_fftk_axis& X = f->ax[a];
+        X.G = p->os[a];
+        X.N = p->shape[a];
+        if (!factorize(X.G, X.rad)) {
+            pf_fftk_destroy(f);
+            return pf_fail(PF_ERR_UNSUPPORTED, "grid size has prime factors other than 2, 3, 5");
+        }
+        std::vector<float2> h(X.G);
+        for (int k = 0; k < X.G; ++k) {
+            double ang = -2.0 * M_PI * (double)k / (double)X.G;
+            h[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        if (hipMalloc(&X.tw, X.G * sizeof(float2)) != hipSuccess ||
+            hipMemcpy(X.tw, h.data(), X.G * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+            pf_fftk_destroy(f);
+            return pf_fail(PF_ERR_HIP, "twiddle table upload failed");
+        }
+    }
+    p->fftk = f;
+    return PF_OK;
+}
+
+void pf_fftk_destroy(pf_fftk* f) {
+    if (!f) return;
+    for (auto& X : f->ax)
+        if (X.tw) (void)hipFree(X.tw);
+    delete f;
+}
+
+static AxisFFT axis_desc(const pf_fftk_axis& X) {
+    AxisFFT A;
+    A.G = X.G;
+    A.N = X.N;
+    A.nrad = (int)X.rad.size();
+    for (int i = 0; i < A.nrad; ++i) A.rad[i] = X.rad[i];
+    A.tw = X.tw;
+    return A;
+}
+
+static Geo geo_of(const pf_plan* p) {
+    Geo g;
+    for (int a = 0; a < 3; ++a) {
+        g.N[a] = p->shape[a];
+        g.G[a] = p->os[a];
+    }
+    g.ntot = pf_ntot(p);
+    return g;
+}
+
+static Batch batch_of(const pf_batch& b) {
+    Batch t{b.C, b.Cv, b.Cb, b.R, b.L, b.K, b.nfg, b.nimg, b.b0s};
+    return t;
+}
+
+// columns per block for a line of length G
+static int cols_for(int G) {
+    int B = 1;
+    while (B * 2 * G <= GBMAX && B < 64) B *= 2;
+    return B;
+}
+
+size_t pf_fftk_extra_bytes(const pf_plan* p, const pf_batch& b) {
+    const size_t RC = (size_t)b.R * b.Cv;
+    const size_t blocks = (size_t)b.nfg * b.Cb * RC * sizeof(float2);
+    if (p->ndim == 3) return blocks * p->shape[0] * p->os[1] * p->os[2];  // T2
+    if (p->ndim == 2) return blocks * p->shape[1] * p->os[2];             // T1
+    return 0;
+}
+
+static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2* src, float2* dst, float factor,
+                    hipStream_t s) {
+    const pf_fftk_axis& X = p->fftk->ax[2];
+    AxisFFT A = axis_desc(X);
+    const int KCv = b.K * b.Cv;
+    if ((int64_t)b.L * X.G > GBMAX)
+        return pf_fail(PF_ERR_UNSUPPORTED, "too many B0 segments for the fused FFT at this grid size");
+    XBlk xb;
+    const int cols = std::max(b.L, cols_for(X.G));  // columns per block, >= L
+    xb.KC = std::max(1, std::min(KCv, cols / b.L));
+    xb.YG = std::max(1, cols / (b.L * xb.KC));
+    while (xb.YG > 1 && (int64_t)xb.YG * b.L * xb.KC * X.G > GBMAX) xb.YG /= 2;
+    while (xb.KC > 1 && (int64_t)xb.YG * b.L * xb.KC * X.G > GBMAX) xb.KC /= 2;
+    xb.nkc = (KCv + xb.KC - 1) / xb.KC;
+    const int64_t rows = (int64_t)b.nfg * b.Cb * p->shape[0] * p->shape[1];
+    const int64_t nblk = ((rows + xb.YG - 1) / xb.YG) * xb.nkc;
+    if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+    const int B = xb.YG * b.L * xb.KC;
+    size_t lds = ((size_t)X.G * (B + 1) + X.G) * sizeof(float2);
+    Geo g = geo_of(p);
+    Batch t = batch_of(b);
+    if (fwd)
+        hipLaunchKernelGGL(k_fftx_fwd, dim3((unsigned)nblk), dim3(FT), lds, s, g, t, A, p->apod, src, dst, xb);
+    else
+        hipLaunchKernelGGL(k_fftx_adj, dim3((unsigned)nblk), dim3(FT), lds, s, g, t, A, p->apod, src, dst, xb,
+                           factor);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, const float2* src, float2* dst,
+                      hipStream_t s) {
+    const pf_fftk_axis& X = p->fftk->ax[axis];
+    AxisFFT A = axis_desc(X);
+    const int B = cols_for(X.G);
+    const int64_t nblk = O * ((I + B - 1) / B);
+    if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+    size_t lds = ((size_t)X.G * (B + 1) + X.G) * sizeof(float2);
+    if (fwd)
+        hipLaunchKernelGGL(k_fftcol<true>, dim3((unsigned)nblk), dim3(FT), lds, s, A, O, I, B, src, dst);
+    else
+        hipLaunchKernelGGL(k_fftcol<false>, dim3((unsigned)nblk), dim3(FT), lds, s, A, O, I, B, src, dst);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+// image -> grid (deapodise, pad, FFT); `extra` holds pf_fftk_extra_bytes
+int pf_fftk_forward(pf_plan* p, const pf_batch& b, const float2* img, float2* grid, float2* extra, hipStream_t s) {
+    int rc = pf_fftk_init(p);
+    if (rc != PF_OK) return rc;
+    const int64_t RC = (int64_t)b.R * b.Cv;
+    const int64_t FC = (int64_t)b.nfg * b.Cb;
+    if (p->ndim == 1) return launch_x(p, b, true, img, grid, 1.0f, s);
+    if (p->ndim == 2) {
+        if ((rc = launch_x(p, b, true, img, extra, 1.0f, s)) != PF_OK) return rc;
+        return launch_col(p, 1, true, FC, (int64_t)p->os[2] * RC, extra, grid, s);
+    }
+    // 3-D: T1 lives in the grid buffer (dead until the Z pass writes it)
+    float2* t1 = grid;
+    float2* t2 = extra;
+    if ((rc = launch_x(p, b, true, img, t1, 1.0f, s)) != PF_OK) return rc;
+    if ((rc = launch_col(p, 1, true, FC * p->shape[0], (int64_t)p->os[2] * RC, t1, t2, s)) != PF_OK) return rc;
+    return launch_col(p, 0, true, FC, (int64_t)p->os[1] * p->os[2] * RC, t2, grid, s);
+}
+
+// grid -> image (inverse FFT, crop, deapodise, times factor); the grid is consumed
+int pf_fftk_adjoint(pf_plan* p, const pf_batch& b, float2* grid, float2* img, float2* extra, float factor,
+                    hipStream_t s) {
+    int rc = pf_fftk_init(p);
+    if (rc != PF_OK) return rc;
+    const int64_t RC = (int64_t)b.R * b.Cv;
+    const int64_t FC = (int64_t)b.nfg * b.Cb;
+    if (p->ndim == 1) return launch_x(p, b, false, grid, img, factor, s);
+    if (p->ndim == 2) {
+        if ((rc = launch_col(p, 1, false, FC, (int64_t)p->os[2] * RC, grid, extra, s)) != PF_OK) return rc;
+        return launch_x(p, b, false, extra, img, factor, s);
+    }
+    float2* t2 = extra;
+    float2* t1 = grid;
+    if ((rc = launch_col(p, 0, false, FC, (int64_t)p->os[1] * p->os[2] * RC, grid, t2, s)) != PF_OK) return rc;
+    if ((rc = launch_col(p, 1, false, FC * p->shape[0], (int64_t)p->os[2] * RC, t2, t1, s)) != PF_OK) return rc;
+    return launch_x(p, b, false, t1, img, factor, s);
+}

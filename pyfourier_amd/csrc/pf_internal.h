@@ -75,7 +75,8 @@ struct pf_plan {
     pf_slist sl[2];
     int max_chunk = 0;
 
-    pf_fft_cache* fft = nullptr;
+    pf_fft_cache* fft = nullptr;   // rocFFT plans (grid sizes with other prime factors)
+    struct pf_fftk* fftk = nullptr; // fused pruned FFT (sizes 2^a 3^b 5^c)
 };
 
 // ---- error handling --------------------------------------------------------
@@ -124,6 +125,15 @@ int pf_fft_exec(pf_plan* p, const pf_batch& b, float2* grid, int inverse, hipStr
 void pf_fft_destroy(pf_fft_cache* c);
 size_t pf_spread_partial_bytes(const pf_plan* p, const pf_batch& b);
 int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b);
+
+// fused deapodise/pad/FFT and inverse FFT/crop/deapodise (pf_fftk.hip)
+bool pf_fftk_supported(const pf_plan* p);
+int pf_fftk_init(pf_plan* p);
+void pf_fftk_destroy(struct pf_fftk* f);
+size_t pf_fftk_extra_bytes(const pf_plan* p, const pf_batch& b);
+int pf_fftk_forward(pf_plan* p, const pf_batch& b, const float2* img, float2* grid, float2* extra, hipStream_t s);
+int pf_fftk_adjoint(pf_plan* p, const pf_batch& b, float2* grid, float2* img, float2* extra, float factor,
+                    hipStream_t s);
 
 // ---- small helpers --------------------------------------------------------
 static inline int64_t pf_gtot(const pf_plan* p) { return (int64_t)p->os[0] * p->os[1] * p->os[2]; }

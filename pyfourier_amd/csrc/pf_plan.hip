@@ -84,7 +84,19 @@ __global__ void k_taps(const float* __restrict__ coord, TapParams P, uint32_t* _
         }
         // coordinate axis order is (x, y, z): grid slot 2 <- coord[..., 0]
         float c = coord[i * P.ndim + (2 - a)];
-        float cos_ = __fadd_rn(__fmul_rn(c, P.sc[a]), P.sh[a]);
+        // two roundings like the reference's `c *= scale; c += shift`
+        // (_subroutines/_traj.py:64-65); a fused multiply-add would round once
+        // and move samples that sit on a tap boundary to the next tap
+        float prod;
+        {
+#pragma clang fp contract(off)
+            prod = c * P.sc[a];
+        }
+        float cos_;
+        {
+#pragma clang fp contract(off)
+            cos_ = prod + P.sh[a];
+        }
         cos_out[i * 3 + a] = cos_;
         double cd = (double)cos_;
         if (W == 1) {
@@ -420,5 +432,6 @@ void pf_plan_free(pf_plan* p) {
             if (q) (void)hipFree(q);
     }
     if (p->fft) pf_fft_destroy(p->fft);
+    if (p->fftk) pf_fftk_destroy(p->fftk);
     delete p;
 }

@@ -1,0 +1,177 @@
+"""GPU parity of the operator variants against the CPU restatement:
+1-D, anisotropic 3-D, Cartesian axes, rocFFT fallback grid sizes,
+multi-frame trajectories, the fused low-rank subspace, the time-segmented
+B0 model, density weights, real inputs and numpy inputs.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import pyfourier_amd as pf
+from oracle import nudft as U
+from oracle import restate as R
+from pyfourier_amd import demo
+
+pytestmark = pytest.mark.gpu
+
+PARITY = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _check_pair(dev, coord, shape, x, width=4, oversamp=2.0, **kw):
+    P = R.Plan(coord, shape, width=width, oversamp=oversamp)
+    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=width, oversamp=oversamp)
+    y = pf.nufft(torch.from_numpy(x).to(dev), nufft_plan=plan, device=dev, **kw)
+    F = P.nframes
+    B = x.shape[0] if x.ndim > len(P.shape) + (1 if F > 1 else 0) else 1
+    xr = x.reshape((B, F) + tuple(P.shape))
+    yr = R.nufft(xr, P, norm=kw.get("norm"))
+    assert rel(y.cpu().numpy(), yr) < PARITY
+    ya = demo.complex_gaussian(np.random.default_rng(9), tuple(y.shape))
+    xa = pf.nufft_adj(torch.from_numpy(ya).to(dev), nufft_plan=plan, device=dev, **kw)
+    xar = R.nufft_adj(ya.reshape(B, F, -1), P, norm=kw.get("norm"))
+    assert rel(xa.cpu().numpy(), xar) < PARITY
+    return plan
+
+
+def test_1d(cuda_device):
+    rng = np.random.default_rng(0)
+    coord = np.sort(rng.uniform(-np.pi, np.pi, size=(3000, 1))).astype(np.float32)
+    coord[0, 0] = -np.pi
+    x = demo.complex_gaussian(rng, (4, 200))
+    _check_pair(cuda_device, coord, 200, x)
+
+
+def test_3d_anisotropic(cuda_device):
+    rng = np.random.default_rng(1)
+    coord = demo.golden_means_radial_3d(200, 48)
+    shape = (20, 24, 32)
+    x = demo.complex_gaussian(rng, (3,) + shape)
+    _check_pair(cuda_device, coord, shape, x, width=(4, 5, 6), oversamp=(2.0, 1.5, 1.25))
+
+
+def test_rocfft_fallback_size(cuda_device):
+    # oversampled size 147 = 3 * 7^2 is outside the fused FFT's radix set
+    rng = np.random.default_rng(2)
+    coord = demo.golden_angle_radial_2d(40, 120)
+    x = demo.complex_gaussian(rng, (2, 98, 98))
+    _check_pair(cuda_device, coord, 98, x, oversamp=1.5, norm="ortho")
+
+
+def test_cartesian_axis_stack_of_stars(cuda_device):
+    # z on integer grid positions -> width 1, oversamp 1 on that axis
+    rng = np.random.default_rng(3)
+    Nz, N = 8, 32
+    c2 = demo.golden_angle_radial_2d(24, 64) / np.pi * (N / 2)
+    kz = np.arange(Nz) - Nz // 2
+    coord = np.zeros((Nz, 24, 64, 3), dtype=np.float32)
+    coord[..., :2] = c2[None]
+    coord[..., 2] = kz[:, None, None]
+    coord = coord.reshape(Nz * 24, 64, 3)
+    plan = pf.plan_nufft(torch.from_numpy(coord), (Nz, N, N), width=4, oversamp=2.0)
+    assert plan.width[0] == 1 and plan.oversamp[0] == 1.0
+    x = demo.complex_gaussian(rng, (2, Nz, N, N))
+    _check_pair(cuda_device, coord, (Nz, N, N), x)
+
+
+def test_cartesian_exact_fft(cuda_device):
+    # integer coordinates with W=1, sigma=1 reduce exactly to a centred FFT
+    rng = np.random.default_rng(4)
+    N = 32
+    ky, kx = np.meshgrid(np.arange(N) - N // 2, np.arange(N) - N // 2, indexing="ij")
+    coord = np.stack([kx, ky], -1).reshape(N, N, 2).astype(np.float32)
+    x = demo.complex_gaussian(rng, (N, N))
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), torch.from_numpy(coord), shape=N, device=cuda_device)
+    ref = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x)))
+    assert rel(y.cpu().numpy(), ref) < 1e-6
+
+
+def test_multiframe(cuda_device):
+    rng = np.random.default_rng(5)
+    F, N = 3, 48
+    coord = np.stack([demo.golden_angle_radial_2d(16, 96) * (1.0 - 0.1 * f) for f in range(F)])
+    coord[:, 0, 0] = [-np.pi, 0]
+    x = demo.complex_gaussian(rng, (2, F, N, N))
+    _check_pair(cuda_device, coord, N, x)
+
+
+def test_subspace(cuda_device):
+    rng = np.random.default_rng(6)
+    F, K, N, C = 20, 4, 48, 3
+    coord = np.stack([demo.golden_angle_radial_2d(1, 128)[0] for _ in range(F)])[:, None]
+    ang = np.arange(F) * 2.4
+    rot = np.stack([np.cos(ang), -np.sin(ang), np.sin(ang), np.cos(ang)], -1).reshape(F, 1, 1, 2, 2)
+    coord = np.einsum("fvsij,fvsj->fvsi", np.broadcast_to(rot, coord.shape[:3] + (2, 2)), coord).astype(np.float32)
+    basis = demo.t2_subspace_basis(rng, F, K)
+    P = R.Plan(coord, N, width=4, oversamp=2.0)
+    plan = pf.plan_nufft(torch.from_numpy(coord), N, width=4, oversamp=2.0)
+    x = demo.complex_gaussian(rng, (C, K, N, N))
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, basis=torch.from_numpy(basis),
+                 device=cuda_device)
+    assert tuple(y.shape) == (C, F, 1, 128)
+    yr = R.nufft(x, P, basis=basis)
+    assert rel(y.cpu().numpy().reshape(C, F, -1), yr) < PARITY
+    ya = demo.complex_gaussian(rng, (C, F, 1, 128))
+    xa = pf.nufft_adj(torch.from_numpy(ya).to(cuda_device), nufft_plan=plan, basis=torch.from_numpy(basis),
+                      device=cuda_device)
+    assert tuple(xa.shape) == (C, K, N, N)
+    xar = R.nufft_adj(ya.reshape(C, F, -1), P, basis=basis)
+    assert rel(xa.cpu().numpy(), xar) < PARITY
+
+
+def test_b0_segments(cuda_device):
+    rng = np.random.default_rng(7)
+    N, C, L = 48, 2, 6
+    coord = demo.variable_density_spiral_2d(8, 512, N)
+    zmap = demo.smooth_field_map(rng, (N, N), 100.0)
+    t = np.linspace(0, 10e-3, 512).astype(np.float32)
+    plan = pf.plan_nufft(torch.from_numpy(coord), N, width=4, oversamp=2.0, zmap=torch.from_numpy(zmap),
+                         T=torch.from_numpy(t), L=L, nbins=40)
+    bt = plan.zmap_t_kernel.cpu().numpy().astype(np.complex128)
+    ct = plan.zmap_s_kernel.cpu().numpy().astype(np.complex128)
+    P = R.Plan(coord, N, width=4, oversamp=2.0)
+    x = demo.complex_gaussian(rng, (C, N, N))
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    yr = R.nufft(x[:, None], P, b0=(bt, ct))
+    assert rel(y.cpu().numpy().reshape(C, 1, -1), yr) < PARITY
+    # the segmented model approximates the exact field-map signal
+    ye = U.nudft_b0(x[:1], P.coord.reshape(-1, 2), (N, N), zmap, t)
+    assert rel(y.cpu().numpy()[:1].reshape(1, -1), ye) < 2e-2
+    ya = demo.complex_gaussian(rng, tuple(y.shape))
+    xa = pf.nufft_adj(torch.from_numpy(ya).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    xar = R.nufft_adj(ya.reshape(C, 1, -1), P, b0=(bt, ct))
+    assert rel(xa.cpu().numpy().reshape(C, 1, N, N), xar) < PARITY
+    # segment chunking (L_batch_size) gives the same operator
+    plan.zmap_batch_size = 4
+    y2 = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    assert rel(y2.cpu().numpy(), y.cpu().numpy()) < 1e-6
+
+
+def test_weights_real_and_numpy(cuda_device):
+    rng = np.random.default_rng(8)
+    N = 40
+    coord = demo.golden_angle_radial_2d(20, 80)
+    w = demo.spiral_density(coord)
+    P = R.Plan(coord, N, width=4, oversamp=1.25)  # nufft's default oversampling
+    xr = rng.standard_normal((N, N)).astype(np.float32)
+    y = pf.nufft(xr, coord, shape=N, weight=w, device="cuda:0")
+    assert isinstance(y, np.ndarray) and y.dtype == np.float32
+    yr = R.nufft(xr[None, None].astype(np.complex128), P, weight=w)
+    assert rel(y, yr.real.reshape(y.shape)) < PARITY
+    yc = demo.complex_gaussian(rng, coord.shape[:-1])
+    xa = pf.nufft_adj(yc, coord, shape=N, weight=w, device="cuda:0")
+    assert isinstance(xa, np.ndarray) and xa.dtype == np.complex64
+    xar = R.nufft_adj(yc.reshape(1, 1, -1), P, weight=w)
+    assert rel(xa, xar) < PARITY
+
+
+def test_cpu_device_raises():
+    coord = demo.golden_angle_radial_2d(4, 16)
+    with pytest.raises(RuntimeError, match="GPU only"):
+        pf.nufft(np.zeros((8, 8), np.complex64), coord, shape=8, device="cpu")
