@@ -1,4 +1,5 @@
 // extern "C" entry points of libpfhip (declared in include/pfhip.h).
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <string>
@@ -36,7 +37,14 @@ int make_batch(const pf_plan* p, const pf_batch_desc* d, pf_batch* b) {
         return pf_fail(PF_ERR_ARG, "a field map needs both b0_temporal and b0_spatial");
     if (has_b0 && d->nt < 1) return pf_fail(PF_ERR_ARG, "nt must be >= 1");
     b->C = d->batch;
-    b->Cv = (d->batch % 8 == 0) ? 8 : (d->batch % 4 == 0) ? 4 : (d->batch % 2 == 0) ? 2 : 1;
+    // all batch items innermost (one contiguous vector per grid point) up to
+    // 64; larger batches split into blocks of the largest divisor <= 64
+    b->Cv = 1;
+    for (int v = std::min(64, d->batch); v >= 1; --v)
+        if (d->batch % v == 0) {
+            b->Cv = v;
+            break;
+        }
     b->Cb = b->C / b->Cv;
     b->L = d->nseg;
     b->K = d->ncoeff > 0 ? d->ncoeff : 1;

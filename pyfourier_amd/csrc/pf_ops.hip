@@ -242,86 +242,158 @@ __global__ void __launch_bounds__(TB) k_gather(GeoParams g, BatchParams B, int64
 }
 
 // -------------------------------------------------------------------------
-// spread: one block per (tile work item, coil block, value chunk).  Waves
-// own disjoint value subsets of the LDS tile; inside a wave all lanes work on
-// the same sample, on distinct (tap line, value) pairs, so no two lanes ever
-// add to the same LDS word in one instruction and per-wave program order
-// fixes the summation order: run-to-run bit-identical.
+// spread: one block per (tile work item, value chunk).  A value chunk is a
+// range of VC entries of the flat per-point vector (Cb, R, Cv); the waves of
+// the block own disjoint parts of it.  Samples are staged into LDS in batches
+// of SB (tile-clipped tap box, in-tile weights, values times conj of the
+// subspace/B0 coefficient); every wave then walks the batch in order, its
+// lanes taking distinct (in-tile tap, value) pairs, so no two lanes add to
+// the same LDS word in one instruction and per-wave program order fixes the
+// summation order: run-to-run bit-identical with no global atomics.
+// The accumulator rows are padded to an odd number of words (no LDS bank
+// conflicts between neighbouring taps).
 // -------------------------------------------------------------------------
-constexpr int SPREAD_LDS_BYTES = 64 * 1024;
+constexpr int SPREAD_ACC_BYTES = 64 * 1024;
+constexpr int SB = 32;       // samples staged per batch
+constexpr int NWV = TB / 64; // waves per block
 
-__global__ void __launch_bounds__(TB) k_spread(GeoParams g, BatchParams B, int VC,
-                                               const int4* __restrict__ items, int nitems,
+// floor(t / c) for 0 <= t < 4096, 1 <= c <= 64, with m = ceil(65536 / c)
+__device__ __forceinline__ int fdiv(int t, int m) { return (t * m) >> 16; }
+
+struct SpreadLds {
+    int VC, PS, tpts;
+    __device__ float* acc(float* base) const { return base; }
+};
+
+__global__ void __launch_bounds__(TB) k_spread(GeoParams g, BatchParams B, int VC, const int4* __restrict__ items,
                                                const int32_t* __restrict__ slist,
                                                const float* __restrict__ taprec,
                                                const int32_t* __restrict__ perm,
-                                               const float2* __restrict__ ys,
-                                               float2* __restrict__ grid, float2* __restrict__ partial) {
+                                               const float2* __restrict__ ys, float2* __restrict__ grid,
+                                               float2* __restrict__ partial) {
     extern __shared__ float lds[];
-    const int nvch = (B.R * B.Cv + VC - 1) / VC;
-    unsigned bid = pf_xcd_remap(blockIdx.x, gridDim.x);
-    int4 it = items[bid];
-    const int cb = blockIdx.y / nvch;
-    const int vch = blockIdx.y % nvch;
     const int RC = B.R * B.Cv;
+    const int V = B.Cb * RC;
+    const int nvch = (V + VC - 1) / VC;
+    const int PS = 2 * VC + 1;  // padded words per accumulator point
+    const int Tx = g.tile[2], Ty = g.tile[1], Tz = g.tile[0];
+    const int tpts = Tx * Ty * Tz;
+    float* acc = lds;                                        // tpts * PS
+    int* sbox = reinterpret_cast<int*>(acc + tpts * PS);     // SB * 8
+    float* sw = reinterpret_cast<float*>(sbox + SB * 8);     // SB * 3 * PF_WMAX
+    float2* sy = reinterpret_cast<float2*>(sw + SB * 3 * PF_WMAX + (SB * 3 * PF_WMAX) % 2);  // SB * VC
+
+    const unsigned bid = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int4 it = items[bid / nvch];
+    const int vch = bid % nvch;
     const int vbase = vch * VC;
-    const int vcnt = min(VC, RC - vbase);
+    const int vcnt = min(VC, V - vbase);
     const int64_t key = it.x;
     const int fg_frame = (int)(key / g.ntile_tot);
     const int64_t tile = key % g.ntile_tot;
     const int tx = (int)(tile % g.ntiles[2]);
     const int ty = (int)((tile / g.ntiles[2]) % g.ntiles[1]);
     const int tz = (int)(tile / ((int64_t)g.ntiles[2] * g.ntiles[1]));
-    const int ox = tx * g.tile[2], oy = ty * g.tile[1], oz = tz * g.tile[0];
-    const int Tx = g.tile[2], Ty = g.tile[1], Tz = g.tile[0];
-    const int ex = min(Tx, g.os[2] - ox), ey = min(Ty, g.os[1] - oy), ez = min(Tz, g.os[0] - oz);
-    const int tpts = Tx * Ty * Tz;
+    const int o[3] = {tz * Tz, ty * Ty, tx * Tx};
+    const int e[3] = {min(Tz, g.os[0] - o[0]), min(Ty, g.os[1] - o[1]), min(Tx, g.os[2] - o[2])};
 
-    for (int i = threadIdx.x; i < tpts * VC * 2; i += blockDim.x) lds[i] = 0.f;
-    __syncthreads();
+    for (int i = threadIdx.x; i < tpts * PS; i += TB) acc[i] = 0.f;
 
     const int wave = threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
-    const int nw = blockDim.x / 64;
-    // values owned by this wave: [vw0, vw1)
-    const int per = (vcnt + nw - 1) / nw;
-    const int vw0 = min(vcnt, wave * per), vw1 = min(vcnt, vw0 + per);
-    const int VW = vw1 - vw0;
-    const int Wz = g.width[0], Wy = g.width[1], Wx = g.width[2];
-    const int lines = Wz * Wy;
-    const int npairs = lines * VW;
+    const int VW = (vcnt + NWV - 1) / NWV;
+    const int vw0 = min(vcnt, wave * VW);
+    const int vwn = min(vcnt, vw0 + VW) - vw0;  // values of this wave
+    const int mvw = vwn > 0 ? (65536 + vwn - 1) / vwn : 1;
 
-    if (VW > 0) {
-        for (int e = it.y; e < it.z; ++e) {
-            const int64_t s = slist[e];
+    for (int eb = it.y; eb < it.z; eb += SB) {
+        const int nb = min(SB, it.z - eb);
+        __syncthreads();  // previous batch fully consumed (and accumulator zeroed)
+        // ---- stage A: one thread per sample: tap box and in-tile weights
+        if (threadIdx.x < nb) {
+            const int j = threadIdx.x;
+            const int64_t s = slist[eb + j];
             const float* rec = taprec + s * g.rec;
             const int* ri = reinterpret_cast<const int*>(rec);
-            const int f = ri[3];
-            int64_t tread = 0;
-            if (B.b0t) tread = ((int64_t)perm[s] % g.npts) % B.nt;
-            for (int q = lane; q < npairs; q += 64) {
-                const int line = q / VW;
-                const int v = vw0 + q % VW;
-                const int a = line / Wy, b = line % Wy;
-                int gz = ri[0] + a;
-                if (gz >= g.os[0]) gz -= g.os[0];
-                int gy = ri[1] + b;
-                if (gy >= g.os[1]) gy -= g.os[1];
-                const int lz = gz - oz, ly = gy - oy;
-                if (lz < 0 || lz >= ez || ly < 0 || ly >= ey) continue;
-                const int vg = vbase + v;
-                const int r = vg / B.Cv, cv = vg % B.Cv;
-                float2 val = ys[s * B.C + cb * B.Cv + cv];
-                if (B.basis || B.b0t) val = cmulc(val, sample_coef(B, r, f, tread));
-                const float wzy = rec[g.woff[0] + a] * rec[g.woff[1] + b];
-                float* rowp = lds + ((int64_t)(lz * Ty + ly) * Tx) * VC * 2 + v * 2;
-                for (int c = 0; c < Wx; ++c) {
-                    int gx = ri[2] + c;
-                    if (gx >= g.os[2]) gx -= g.os[2];
-                    const int lx = gx - ox;
-                    if (lx < 0 || lx >= ex) continue;
-                    const float w = wzy * rec[g.woff[2] + c];
-                    float* d = rowp + lx * VC * 2;
+            int lo[3], cn[3], l0[3];
+            for (int a = 0; a < 3; ++a) {
+                const int W = g.width[a], G = g.os[a];
+                int rel0 = ri[a] - o[a];
+                if (rel0 < 0) rel0 += G;
+                if (g.ntiles[a] == 1) {
+                    // one tile spans the axis: every tap is inside, local
+                    // coordinates wrap modulo G in stage C
+                    lo[a] = 0;
+                    l0[a] = rel0;
+                    cn[a] = W;
+                } else if (rel0 < e[a]) {
+                    lo[a] = 0;
+                    l0[a] = rel0;
+                    cn[a] = min(W, e[a] - rel0);
+                } else {
+                    int d = G - rel0;  // taps until the tile start
+                    lo[a] = d;
+                    l0[a] = 0;
+                    cn[a] = d < W ? min(W - d, e[a]) : 0;
+                }
+                for (int k = 0; k < cn[a]; ++k) sw[(j * 3 + a) * PF_WMAX + k] = rec[g.woff[a] + lo[a] + k];
+            }
+            int* bx = sbox + j * 8;
+            bx[0] = cn[0];
+            bx[1] = cn[1];
+            bx[2] = cn[2];
+            bx[3] = l0[0];
+            bx[4] = l0[1];
+            bx[5] = l0[2];
+            bx[6] = ri[3];          // frame
+            bx[7] = (int)s;
+        }
+        __syncthreads();
+        // ---- stage B: values (times conj coefficient)
+        for (int q = threadIdx.x; q < nb * vcnt; q += TB) {
+            const int j = q / vcnt, v = q % vcnt;
+            const int u = vbase + v;
+            const int cb = u / RC, rv = u % RC;
+            const int r = rv / B.Cv, cv = rv % B.Cv;
+            const int64_t s = sbox[j * 8 + 7];
+            float2 val = ys[s * B.C + cb * B.Cv + cv];
+            if (B.basis || B.b0t) {
+                int64_t tread = B.b0t ? ((int64_t)perm[s] % g.npts) % B.nt : 0;
+                val = cmulc(val, sample_coef(B, r, sbox[j * 8 + 6], tread));
+            }
+            sy[j * VC + v] = val;
+        }
+        __syncthreads();
+        // ---- stage C: per-wave accumulation in sample order
+        if (vwn > 0) {
+            for (int j = 0; j < nb; ++j) {
+                const int* bx = sbox + j * 8;
+                const int cz = bx[0], cy = bx[1], cx = bx[2];
+                const int n = cz * cy * cx;
+                if (n == 0) continue;
+                const int mcx = (65536 + cx - 1) / cx, mcy = (65536 + cy - 1) / cy;
+                const int lz0 = bx[3], ly0 = bx[4], lx0 = bx[5];
+                const float* wz = sw + (j * 3 + 0) * PF_WMAX;
+                const float* wy = sw + (j * 3 + 1) * PF_WMAX;
+                const float* wx = sw + (j * 3 + 2) * PF_WMAX;
+                const float2* yj = sy + j * VC + vw0;
+                const int npairs = n * vwn;
+                for (int q = lane; q < npairs; q += 64) {
+                    const int t = fdiv(q, mvw);
+                    const int v = q - t * vwn;
+                    const int t2 = fdiv(t, mcx);
+                    const int ix = t - t2 * cx;
+                    const int iz = fdiv(t2, mcy);
+                    const int iy = t2 - iz * cy;
+                    const float w = wz[iz] * wy[iy] * wx[ix];
+                    // wrap only happens on single-tile axes (e = G there)
+                    int lz = lz0 + iz, ly = ly0 + iy, lx = lx0 + ix;
+                    lz -= lz >= e[0] ? e[0] : 0;
+                    ly -= ly >= e[1] ? e[1] : 0;
+                    lx -= lx >= e[2] ? e[2] : 0;
+                    const int pt = (lz * Ty + ly) * Tx + lx;
+                    const float2 val = yj[v];
+                    float* d = acc + pt * PS + 2 * (vw0 + v);
                     atomicAdd(d, w * val.x);
                     atomicAdd(d + 1, w * val.y);
                 }
@@ -329,22 +401,26 @@ __global__ void __launch_bounds__(TB) k_spread(GeoParams g, BatchParams B, int V
         }
     }
     __syncthreads();
-
     if (it.w < 0) {
-        float2* slab = grid + ((int64_t)(B.nfg > 1 ? fg_frame : 0) * B.Cb + cb) * g.gtot * RC + vbase;
-        for (int i = threadIdx.x; i < tpts * vcnt; i += blockDim.x) {
-            int v = i % vcnt;
-            int pt = i / vcnt;
-            int lx = pt % Tx, ly = (pt / Tx) % Ty, lz = pt / (Tx * Ty);
-            if (lx >= ex || ly >= ey || lz >= ez) continue;
-            int64_t gpt = ((int64_t)(oz + lz) * g.os[1] + (oy + ly)) * g.os[2] + (ox + lx);
-            const float* src = lds + ((int64_t)pt * VC + v) * 2;
-            slab[gpt * RC + v] = make_float2(src[0], src[1]);
+        for (int i = threadIdx.x; i < tpts * vcnt; i += TB) {
+            const int v = i % vcnt;
+            const int pt = i / vcnt;
+            const int lx = pt % Tx, ly = (pt / Tx) % Ty, lz = pt / (Tx * Ty);
+            if (lx >= e[2] || ly >= e[1] || lz >= e[0]) continue;
+            const int u = vbase + v;
+            const int cb = u / RC, rv = u % RC;
+            const int64_t gpt = ((int64_t)(o[0] + lz) * g.os[1] + (o[1] + ly)) * g.os[2] + (o[2] + lx);
+            const int64_t slab = (int64_t)(B.nfg > 1 ? fg_frame : 0) * B.Cb + cb;
+            const float* src = acc + pt * PS + 2 * v;
+            grid[(slab * g.gtot + gpt) * RC + rv] = make_float2(src[0], src[1]);
         }
     } else {
-        float2* dst = partial + (((int64_t)it.w * B.Cb + cb) * nvch + vch) * (int64_t)tpts * VC;
-        for (int i = threadIdx.x; i < tpts * VC; i += blockDim.x)
-            dst[i] = make_float2(lds[2 * i], lds[2 * i + 1]);
+        float2* dst = partial + ((int64_t)it.w * nvch + vch) * (int64_t)tpts * VC;
+        for (int i = threadIdx.x; i < tpts * VC; i += TB) {
+            const int v = i % VC, pt = i / VC;
+            const float* src = acc + pt * PS + 2 * v;
+            dst[i] = make_float2(src[0], src[1]);
+        }
     }
 }
 
@@ -353,38 +429,172 @@ __global__ void __launch_bounds__(TB) k_spread_reduce(GeoParams g, BatchParams B
                                                       const int4* __restrict__ ritems,
                                                       const float2* __restrict__ partial,
                                                       float2* __restrict__ grid) {
-    const int nvch = (B.R * B.Cv + VC - 1) / VC;
-    int4 it = ritems[blockIdx.x];
-    const int cb = blockIdx.y / nvch;
-    const int vch = blockIdx.y % nvch;
     const int RC = B.R * B.Cv;
+    const int V = B.Cb * RC;
+    const int nvch = (V + VC - 1) / VC;
+    const int4 it = ritems[blockIdx.x / nvch];
+    const int vch = blockIdx.x % nvch;
     const int vbase = vch * VC;
-    const int vcnt = min(VC, RC - vbase);
+    const int vcnt = min(VC, V - vbase);
     const int64_t key = it.x;
     const int fg_frame = (int)(key / g.ntile_tot);
     const int64_t tile = key % g.ntile_tot;
     const int tx = (int)(tile % g.ntiles[2]);
     const int ty = (int)((tile / g.ntiles[2]) % g.ntiles[1]);
     const int tz = (int)(tile / ((int64_t)g.ntiles[2] * g.ntiles[1]));
-    const int ox = tx * g.tile[2], oy = ty * g.tile[1], oz = tz * g.tile[0];
     const int Tx = g.tile[2], Ty = g.tile[1], Tz = g.tile[0];
+    const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
     const int ex = min(Tx, g.os[2] - ox), ey = min(Ty, g.os[1] - oy), ez = min(Tz, g.os[0] - oz);
     const int tpts = Tx * Ty * Tz;
-    float2* slab = grid + ((int64_t)(B.nfg > 1 ? fg_frame : 0) * B.Cb + cb) * g.gtot * RC + vbase;
-    for (int i = threadIdx.x; i < tpts * vcnt; i += blockDim.x) {
-        int v = i % vcnt;
-        int pt = i / vcnt;
-        int lx = pt % Tx, ly = (pt / Tx) % Ty, lz = pt / (Tx * Ty);
+    for (int i = threadIdx.x; i < tpts * vcnt; i += TB) {
+        const int v = i % vcnt;
+        const int pt = i / vcnt;
+        const int lx = pt % Tx, ly = (pt / Tx) % Ty, lz = pt / (Tx * Ty);
         if (lx >= ex || ly >= ey || lz >= ez) continue;
-        float2 acc = make_float2(0.f, 0.f);
+        float2 a = make_float2(0.f, 0.f);
         for (int sl = 0; sl < it.z; ++sl) {
-            const float2* src = partial + (((int64_t)(it.y + sl) * B.Cb + cb) * nvch + vch) * (int64_t)tpts * VC;
-            float2 v2 = src[(int64_t)pt * VC + v];
-            acc.x += v2.x;
-            acc.y += v2.y;
+            const float2* src = partial + ((int64_t)(it.y + sl) * nvch + vch) * (int64_t)tpts * VC;
+            const float2 x = src[(int64_t)pt * VC + v];
+            a.x += x.x;
+            a.y += x.y;
         }
-        int64_t gpt = ((int64_t)(oz + lz) * g.os[1] + (oy + ly)) * g.os[2] + (ox + lx);
-        slab[gpt * RC + v] = acc;
+        const int u = vbase + v;
+        const int cb = u / RC, rv = u % RC;
+        const int64_t gpt = ((int64_t)(oz + lz) * g.os[1] + (oy + ly)) * g.os[2] + (ox + lx);
+        const int64_t slab = (int64_t)(B.nfg > 1 ? fg_frame : 0) * B.Cb + cb;
+        grid[(slab * g.gtot + gpt) * RC + rv] = a;
+    }
+}
+
+// -------------------------------------------------------------------------
+// Register-resident spread for batches of >= 16 values per grid point.
+// One wave owns one tile (TZ x TY x 2*TXL points of the plan's tiling) for
+// a chunk of 32 values of the flat per-point vector (Cb, R, Cv): lane =
+// (value, x half), each lane keeps its TZ*TY*TXL complex accumulators in
+// VGPRs and walks the tile's sample list in order.  Tap rows of a sample are
+// uniform across the wave (scalar branches skip rows outside the footprint);
+// each useful tap costs two FMAs.  No LDS and no atomics: the summation
+// order is the sample order, so results are run-to-run bit-identical.
+// -------------------------------------------------------------------------
+constexpr int SPREAD_VL = 32;
+
+template <int TZ, int TY, int TXL>
+__global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, const int4* __restrict__ items,
+                                                   const int32_t* __restrict__ slist,
+                                                   const float* __restrict__ taprec,
+                                                   const int32_t* __restrict__ perm,
+                                                   const float2* __restrict__ ys, float2* __restrict__ grid,
+                                                   float2* __restrict__ partial) {
+    constexpr int VL = SPREAD_VL;
+    const int RC = B.R * B.Cv;
+    const int V = B.Cb * RC;
+    const int nvch = (V + VL - 1) / VL;
+    const unsigned bid = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int4 it = items[bid / nvch];
+    const int vch = bid % nvch;
+    const int lane = threadIdx.x;
+    const int v = lane & (VL - 1);
+    const int half = lane / VL;
+    const int u = vch * VL + v;
+    const bool vok = u < V;
+    const int uu = vok ? u : V - 1;
+    const int cb = uu / RC, rv = uu % RC;
+    const int r = rv / B.Cv, cv = rv % B.Cv;
+    const int ycol = cb * B.Cv + cv;
+
+    const int64_t key = it.x;
+    const int fg_frame = (int)(key / g.ntile_tot);
+    const int64_t tile = key % g.ntile_tot;
+    const int tix = (int)(tile % g.ntiles[2]);
+    const int tiy = (int)((tile / g.ntiles[2]) % g.ntiles[1]);
+    const int tiz = (int)(tile / ((int64_t)g.ntiles[2] * g.ntiles[1]));
+    const int oz = tiz * g.tile[0], oy = tiy * g.tile[1], ox = tix * g.tile[2];
+    const int ez = min(g.tile[0], g.os[0] - oz), ey = min(g.tile[1], g.os[1] - oy);
+    const int ex = min(g.tile[2], g.os[2] - ox);
+    const int Wz = g.width[0], Wy = g.width[1], Wx = g.width[2];
+
+    float2 acc[TZ][TY][TXL];
+#pragma unroll
+    for (int a = 0; a < TZ; ++a)
+#pragma unroll
+        for (int b = 0; b < TY; ++b)
+#pragma unroll
+            for (int k = 0; k < TXL; ++k) acc[a][b][k] = make_float2(0.f, 0.f);
+
+    for (int e = it.y; e < it.z; ++e) {
+        const int64_t s = slist[e];
+        const float* rec = taprec + s * g.rec;
+        const int* ri = reinterpret_cast<const int*>(rec);
+        // tap index of tile position p along an axis: (o + p - start) mod G
+        int bz = oz - ri[0];
+        bz += bz < 0 ? g.os[0] : 0;
+        int by = oy - ri[1];
+        by += by < 0 ? g.os[1] : 0;
+        int bx = ox - ri[2];
+        bx += bx < 0 ? g.os[2] : 0;
+        float wxl[TXL];
+#pragma unroll
+        for (int k = 0; k < TXL; ++k) {
+            int i = bx + half * TXL + k;
+            i -= i >= g.os[2] ? g.os[2] : 0;
+            const float w = rec[g.woff[2] + min(i, Wx - 1)];
+            wxl[k] = i < Wx ? w : 0.f;
+        }
+        float2 val = vok ? ys[s * B.C + ycol] : make_float2(0.f, 0.f);
+        if (B.basis || B.b0t) {
+            const int64_t tread = B.b0t ? ((int64_t)perm[s] % g.npts) % B.nt : 0;
+            val = cmulc(val, sample_coef(B, r, ri[3], tread));
+        }
+#pragma unroll
+        for (int a = 0; a < TZ; ++a) {
+            int iz = bz + a;
+            iz -= iz >= g.os[0] ? g.os[0] : 0;
+            if (iz >= Wz) continue;
+            const float wz = rec[g.woff[0] + iz];
+#pragma unroll
+            for (int b = 0; b < TY; ++b) {
+                int iy = by + b;
+                iy -= iy >= g.os[1] ? g.os[1] : 0;
+                if (iy >= Wy) continue;
+                const float wzy = wz * rec[g.woff[1] + iy];
+                const float vx = wzy * val.x, vy = wzy * val.y;
+#pragma unroll
+                for (int k = 0; k < TXL; ++k) {
+                    acc[a][b][k].x += wxl[k] * vx;
+                    acc[a][b][k].y += wxl[k] * vy;
+                }
+            }
+        }
+    }
+
+    if (!vok) return;
+    if (it.w < 0) {
+        const int64_t slab = (int64_t)(B.nfg > 1 ? fg_frame : 0) * B.Cb + cb;
+        float2* base = grid + slab * g.gtot * RC + rv;
+#pragma unroll
+        for (int a = 0; a < TZ; ++a)
+#pragma unroll
+            for (int b = 0; b < TY; ++b)
+#pragma unroll
+                for (int k = 0; k < TXL; ++k) {
+                    const int lx = half * TXL + k;
+                    if (a < ez && b < ey && lx < ex) {
+                        const int64_t gpt = ((int64_t)(oz + a) * g.os[1] + (oy + b)) * g.os[2] + (ox + lx);
+                        base[gpt * RC] = acc[a][b][k];
+                    }
+                }
+    } else {
+        const int Tx = g.tile[2], Ty = g.tile[1], Tz = g.tile[0];
+        float2* dst = partial + ((int64_t)it.w * nvch + vch) * (int64_t)(Tx * Ty * Tz) * VL + v;
+#pragma unroll
+        for (int a = 0; a < TZ; ++a)
+#pragma unroll
+            for (int b = 0; b < TY; ++b)
+#pragma unroll
+                for (int k = 0; k < TXL; ++k) {
+                    const int lx = half * TXL + k;
+                    if (a < Tz && b < Ty && lx < Tx) dst[(int64_t)((a * Ty + b) * Tx + lx) * VL] = acc[a][b][k];
+                }
     }
 }
 
@@ -464,37 +674,91 @@ int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, fl
     return PF_OK;
 }
 
+// Register spread tiles (z, y, x) per dimensionality; the plan tiles match.
+static bool spread_reg_ok(const pf_plan* p, const pf_batch& b, int* tz, int* ty, int* txl) {
+    const int V = b.Cb * b.R * b.Cv;
+    int z = 1, y = 1, xl = 32;
+    if (p->ndim == 3) {
+        z = 4; y = 4; xl = 4;
+    } else if (p->ndim == 2) {
+        z = 1; y = 8; xl = 8;
+    }
+    if (tz) *tz = z;
+    if (ty) *ty = y;
+    if (txl) *txl = xl;
+    return V >= 16 && p->tile[0] <= z && p->tile[1] <= y && p->tile[2] <= 2 * xl;
+}
+
 int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b) {
-    int tpts = p->tile[0] * p->tile[1] * p->tile[2];
-    int vc = SPREAD_LDS_BYTES / (tpts * 8);
-    if (vc < 1) vc = 1;
-    return std::min(vc, b.R * b.Cv);
+    if (spread_reg_ok(p, b, nullptr, nullptr, nullptr)) return SPREAD_VL;
+    const int tpts = p->tile[0] * p->tile[1] * p->tile[2];
+    const int V = b.Cb * b.R * b.Cv;
+    // accumulator (2*VC+1 words per point) within the LDS budget, multiple of 4
+    int vc = (SPREAD_ACC_BYTES / 4 / tpts - 1) / 2;
+    vc = std::max(1, std::min(vc, 64));
+    if (vc >= 4) vc &= ~3;
+    return std::min(vc, V);
+}
+
+static size_t spread_lds_bytes(const pf_plan* p, int VC) {
+    const size_t tpts = (size_t)p->tile[0] * p->tile[1] * p->tile[2];
+    size_t words = tpts * (2 * VC + 1) + SB * 8 + SB * 3 * PF_WMAX;
+    words += words % 2;
+    return words * 4 + (size_t)SB * VC * sizeof(float2);
 }
 
 size_t pf_spread_partial_bytes(const pf_plan* p, const pf_batch& b) {
-    int VC = pf_spread_coil_chunk(p, b);
-    int nvch = (b.R * b.Cv + VC - 1) / VC;
-    size_t tpts = (size_t)p->tile[0] * p->tile[1] * p->tile[2];
-    return (size_t)pf_slist_for(p, b).nslots * b.Cb * nvch * tpts * VC * sizeof(float2);
+    const int VC = pf_spread_coil_chunk(p, b);
+    const int V = b.Cb * b.R * b.Cv;
+    const int nvch = (V + VC - 1) / VC;
+    const size_t tpts = (size_t)p->tile[0] * p->tile[1] * p->tile[2];
+    return (size_t)pf_slist_for(p, b).nslots * nvch * tpts * VC * sizeof(float2);
 }
 
 int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, float2* grid, float2* partial,
                      hipStream_t s) {
     GeoParams g = geo(p);
     BatchParams B = bparams(b);
-    int VC = pf_spread_coil_chunk(p, b);
-    int nvch = (b.R * b.Cv + VC - 1) / VC;
-    int tpts = p->tile[0] * p->tile[1] * p->tile[2];
-    size_t lds = (size_t)tpts * VC * 8;
+    const int VC = pf_spread_coil_chunk(p, b);
+    const int V = b.Cb * b.R * b.Cv;
+    const int nvch = (V + VC - 1) / VC;
+    const size_t lds = spread_lds_bytes(p, VC);
+    if (lds > 160 * 1024) return pf_fail(PF_ERR_UNSUPPORTED, "spread tile exceeds LDS");
     const pf_slist& L = pf_slist_for(p, b);
     if (L.nslots > 0 && !partial) return pf_fail(PF_ERR_ARG, "spread needs a partial-tile workspace");
-    if ((int64_t)b.Cb * nvch > 65535) return pf_fail(PF_ERR_UNSUPPORTED, "too many coil chunks");
-    hipLaunchKernelGGL(k_spread, dim3(L.nsitems, b.Cb * nvch), dim3(TB), lds, s, g, B, VC, L.sitems, L.nsitems,
-                       L.slist, p->taprec, p->perm, ys, grid, partial);
+    const int64_t nb = (int64_t)L.nsitems * nvch;
+    if (nb > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many spread blocks");
+    int TZ, TY, TXL;
+    if (spread_reg_ok(p, b, &TZ, &TY, &TXL)) {
+        if (p->ndim == 3)
+            hipLaunchKernelGGL((k_spread_reg<4, 4, 4>), dim3((unsigned)nb), dim3(64), 0, s, g, B, L.sitems, L.slist,
+                               p->taprec, p->perm, ys, grid, partial);
+        else if (p->ndim == 2)
+            hipLaunchKernelGGL((k_spread_reg<1, 8, 8>), dim3((unsigned)nb), dim3(64), 0, s, g, B, L.sitems, L.slist,
+                               p->taprec, p->perm, ys, grid, partial);
+        else
+            hipLaunchKernelGGL((k_spread_reg<1, 1, 32>), dim3((unsigned)nb), dim3(64), 0, s, g, B, L.sitems,
+                               L.slist, p->taprec, p->perm, ys, grid, partial);
+        PF_HIP(hipGetLastError());
+        if (L.nritems > 0) {
+            hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch)), dim3(TB), 0, s, g, B,
+                               VC, L.ritems, partial, grid);
+            PF_HIP(hipGetLastError());
+        }
+        return PF_OK;
+    }
+    static bool attr_done = false;
+    if (!attr_done) {
+        PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_spread),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_done = true;
+    }
+    hipLaunchKernelGGL(k_spread, dim3((unsigned)nb), dim3(TB), lds, s, g, B, VC, L.sitems, L.slist, p->taprec,
+                       p->perm, ys, grid, partial);
     PF_HIP(hipGetLastError());
     if (L.nritems > 0) {
-        hipLaunchKernelGGL(k_spread_reduce, dim3(L.nritems, b.Cb * nvch), dim3(TB), 0, s, g, B, VC, L.ritems,
-                           partial, grid);
+        hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch)), dim3(TB), 0, s, g, B, VC,
+                           L.ritems, partial, grid);
         PF_HIP(hipGetLastError());
     }
     return PF_OK;

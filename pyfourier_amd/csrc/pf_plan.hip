@@ -261,12 +261,17 @@ int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_
     }
     p->scale = scale;
     // tiles: 8^3 in 3-D, 32^2 in 2-D, 1024 in 1-D (clamped to the grid)
-    int t3[3] = {8, 8, 8}, t2[3] = {1, 32, 32}, t1[3] = {1, 1, 1024};
+    // spread/sort tiles: the register-resident spread holds one tile for
+    // 32 batch values in one wave (k_spread_reg)
+    int t3[3] = {4, 4, 8}, t2[3] = {1, 8, 16}, t1[3] = {1, 1, 64};
     const int* tt = ndim == 3 ? t3 : (ndim == 2 ? t2 : t1);
     p->ntile_tot = 1;
     int off = 4;
     for (int a = 0; a < 3; ++a) {
         p->tile[a] = std::min(tt[a], p->os[a]);
+        // a footprint must not leave a tile and re-enter it across the
+        // periodic wrap: with several tiles that needs G >= T + W
+        if (p->os[a] < p->tile[a] + p->width[a]) p->tile[a] = p->os[a];
         p->ntiles[a] = (p->os[a] + p->tile[a] - 1) / p->tile[a];
         p->ntile_tot *= p->ntiles[a];
         p->woff[a] = off;

@@ -1,0 +1,145 @@
+"""CPU tests of the C ABI boundary and the host-side logic (no kernel launch).
+
+* the library loads and exports every entry point include/pfhip.h declares;
+* host-side argument validation returns the documented status codes and a
+  message before anything reaches the GPU;
+* the Python host mirror (normalisation, plan parameters, field-map basis,
+  device handling) matches the oracle's statement of the same rules.
+"""
+
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import pyfourier_amd as pf
+from oracle import restate as R
+from pyfourier_amd import _lib, demo
+from pyfourier_amd._plan import kb_beta, resolve_device
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "pfhip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(pf_\w+)\s*\(", text, re.M)))
+
+
+def test_header_symbols_exported():
+    names = _declared()
+    assert len(names) >= 20
+    h = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in names if not hasattr(h, n)]
+    assert not missing, f"not exported: {missing}"
+    # the Python binding covers every declared entry point
+    assert set(names) <= set(_lib.EXPORTED)
+
+
+def test_version_and_error_string():
+    L = _lib.lib()
+    assert L.pf_version() == 100
+    assert isinstance(L.pf_last_error(), bytes)
+
+
+def _create(ndim, shape, width, oversamp, nframes=1, npts=16):
+    L = _lib.lib()
+    fake = ctypes.c_void_p(16)  # never dereferenced: validation fails first
+    sh = (ctypes.c_int * 3)(*shape)
+    wi = (ctypes.c_int * 3)(*width)
+    ov = (ctypes.c_float * 3)(*oversamp)
+    out = ctypes.c_void_p()
+    rc = L.pf_plan_create(fake, ndim, sh, wi, ov, nframes, npts, None, ctypes.byref(out))
+    return rc, L.pf_last_error().decode(), out
+
+
+@pytest.mark.parametrize(
+    "args,code,msg",
+    [
+        ((4, (8, 8, 8), (4, 4, 4), (2.0, 2.0, 2.0)), 1, "ndim"),
+        ((2, (8, 0, 0), (4, 4, 4), (2.0, 2.0, 2.0)), 1, "positive"),
+        ((2, (8, 8, 0), (9, 4, 4), (2.0, 2.0, 2.0)), 5, "width"),
+        ((2, (8, 8, 0), (4, 4, 4), (0.5, 2.0, 2.0)), 1, "oversampling"),
+        ((1, (2, 0, 0), (4, 0, 0), (1.0, 0, 0)), 1, "smaller than the kernel"),
+    ],
+)
+def test_plan_argument_validation(args, code, msg):
+    rc, err, out = _create(*args)
+    assert rc == code and msg in err and not out.value
+
+
+def test_empty_trajectory_rejected():
+    rc, err, _ = _create(2, (8, 8, 0), (4, 4, 0), (2.0, 2.0, 0), nframes=1, npts=0)
+    assert rc == 1 and "empty" in err
+
+
+def test_null_arguments():
+    L = _lib.lib()
+    d = _lib.BatchDesc()
+    d.batch, d.nseg = 1, 1
+    n = ctypes.c_size_t()
+    assert L.pf_workspace_size(None, ctypes.byref(d), 0, ctypes.byref(n)) == 1
+    assert L.pf_plan_create(None, 2, None, None, None, 1, 1, None, None) == 1
+    assert L.pf_nufft(None, ctypes.byref(d), None, None, None, 0, None, 0, None) == 1
+
+
+def test_device_resolution():
+    with pytest.raises(RuntimeError, match="GPU only"):
+        resolve_device("cpu")
+    with pytest.raises(RuntimeError, match="GPU only"):
+        resolve_device(-1)
+    assert resolve_device("cuda:3") == torch.device("cuda", 3)
+    assert resolve_device(2) == torch.device("cuda", 2)
+
+
+@pytest.mark.parametrize(
+    "maker,N",
+    [
+        (lambda: demo.golden_angle_radial_2d(64, 256), 128),
+        (lambda: demo.golden_means_radial_3d(64, 32), (20, 24, 32)),
+        (lambda: demo.variable_density_spiral_2d(8, 512, 48), 48),
+        (lambda: demo.golden_angle_radial_2d(8, 16) / (2 * np.pi), 32),  # cycles/FOV units
+        (lambda: demo.golden_angle_radial_2d(8, 16) * 10.0, 32),  # grid units
+    ],
+)
+def test_normalisation_matches_oracle(maker, N):
+    c = maker().astype(np.float32)
+    d = c.shape[-1]
+    shape = [N] * d if np.isscalar(N) else list(N)
+    mine = pf.normalize_coordinates(torch.from_numpy(c), shape[::-1]).numpy()
+    np.testing.assert_array_equal(mine, R.normalize_coordinates(c, shape[::-1]))
+
+
+def test_normalisation_units():
+    k = np.array([[-math.pi, 0.0], [0.5 * math.pi, 0.25]], dtype=np.float32)
+    out = pf.normalize_coordinates(torch.from_numpy(k), [64, 32]).numpy()
+    np.testing.assert_allclose(out, k * np.array([64, 32]) / (2 * math.pi), rtol=1e-6)
+    g = np.array([[-10.0, 3.0], [7.5, -20.0]], dtype=np.float32)  # grid units pass through
+    np.testing.assert_array_equal(pf.normalize_coordinates(torch.from_numpy(g), [64, 64]).numpy(), g)
+
+
+def test_plan_parameters_on_host():
+    coord = demo.golden_angle_radial_2d(16, 64)
+    p = pf.plan_nufft(torch.from_numpy(coord), 100, width=4, oversamp=1.25)
+    P = R.Plan(coord, 100, width=4, oversamp=1.25)
+    assert list(p.os_shape) == P.os_shape == [125, 125]
+    np.testing.assert_array_equal(np.asarray(p.beta, np.float32), np.asarray(P.beta, np.float32))
+    assert p.interpolator.scale == pytest.approx(P.scale, rel=1e-12)
+    np.testing.assert_array_equal(kb_beta([4, 6], [2.0, 1.25]), R.kb_beta([4, 6], [2.0, 1.25]))
+
+
+def test_field_map_basis_host():
+    rng = np.random.default_rng(0)
+    zmap = demo.smooth_field_map(rng, (24, 24), 100.0)
+    t = np.linspace(0, 10e-3, 200)
+    b, ct = pf.mri_exp_approx(torch.from_numpy(zmap), torch.from_numpy(t), L=8, nbins=40)
+    assert tuple(b.shape) == (8, 200) and tuple(ct.shape) == (8, 24, 24)
+    bo, co = R.mri_exp_approx(zmap, t, L=8, nbins=40)
+    assert np.linalg.norm(b.numpy() - bo) / np.linalg.norm(bo) < 1e-5
+    assert np.linalg.norm(ct.numpy() - co) / np.linalg.norm(co) < 1e-5
+    approx = b.numpy().astype(np.complex128).T @ ct.numpy().reshape(8, -1)
+    exact = np.exp(-np.outer(t, 2j * math.pi * zmap.ravel()))
+    assert np.linalg.norm(approx - exact) / np.linalg.norm(exact) < 5e-3

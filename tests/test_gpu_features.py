@@ -175,3 +175,16 @@ def test_cpu_device_raises():
     coord = demo.golden_angle_radial_2d(4, 16)
     with pytest.raises(RuntimeError, match="GPU only"):
         pf.nufft(np.zeros((8, 8), np.complex64), coord, shape=8, device="cpu")
+
+
+@pytest.mark.parametrize("shape,oversamp", [((6, 6, 6), 2.0), ((17, 17), 2.0), ((18, 18), 2.0), ((5, 40), 2.0)])
+def test_small_grids_periodic_wrap(cuda_device, shape, oversamp):
+    # grids of one spread tile (footprints wrap inside the tile) and grids
+    # just above a tile, where footprints cross the periodic boundary
+    rng = np.random.default_rng(10)
+    d = len(shape)
+    coord = rng.uniform(-np.pi, np.pi, size=(400, d)).astype(np.float32)
+    coord[0] = 0
+    coord[0, 0] = -np.pi
+    x = demo.complex_gaussian(rng, (3,) + shape)
+    _check_pair(cuda_device, coord, shape, x, oversamp=oversamp)

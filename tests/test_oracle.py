@@ -1,0 +1,228 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each check compares the restatement with something it does not share code
+or formulas with: scipy's modified Bessel function, numerical Fourier
+integrals of the kernel, direct DFT sums, the FP64 NUDFT, exact adjoint
+identities and hand-computed small cases.
+"""
+
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from oracle import nudft as U
+from oracle import restate as R
+from pyfourier_amd import demo
+
+
+def rel(a, b):
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# ---- kernel ------------------------------------------------------------------
+@pytest.mark.parametrize("beta", [2.0, 6.99666, 8.99615, 13.8551])
+def test_kb_matches_scipy_i0(beta):
+    x = np.linspace(-1, 1, 2001)
+    exact = special.i0(beta * np.sqrt(1 - x * x))
+    # Abramowitz & Stegun 9.8.1 / 9.8.2 relative accuracy is a few 1e-7
+    assert np.max(np.abs(R.kb(x, beta) / exact - 1)) < 6e-7
+
+
+def test_kb_special_values():
+    b = 8.99615
+    assert R.kb(1.0, b) == pytest.approx(1.0, abs=0)
+    assert R.kb(-1.0, b) == pytest.approx(1.0, abs=0)
+    assert R.kb(1.0000001, b) == 0.0
+    assert R.kb(0.0, b) == pytest.approx(special.i0(b), rel=6e-7)
+
+
+def test_beta_closed_form():
+    # pi * sqrt((W/s*(s-1/2))^2 - 0.8); W=4, s=2 -> pi*sqrt(8.2)
+    assert float(R.kb_beta([4], [2.0])[0]) == pytest.approx(math.pi * math.sqrt(8.2), rel=1e-6)
+    assert float(R.kb_beta([6], [1.25])[0]) == pytest.approx(math.pi * math.sqrt((6 / 1.25 * 0.75) ** 2 - 0.8), rel=1e-6)
+
+
+def test_os_shape():
+    assert R.os_shape([128, 100, 33], [1.25, 1.25, 1.5]) == [160, 125, 50]
+
+
+def test_kernel_scale_is_sum_of_half_open_taps():
+    b = 8.99615
+    per_axis = sum(special.i0(b * np.sqrt(1 - u * u)) for u in (-1.0, -0.5, 0.0, 0.5))
+    assert R.kernel_scale([b, b], [4, 4]) == pytest.approx(per_axis ** 2, rel=2e-6)
+    assert R.kernel_scale([b, b], [1, 4]) == pytest.approx(per_axis, rel=1e-6)
+
+
+def test_interp_tables_brute_force():
+    b = 8.99615
+    c = np.array([[3.0], [0.5]], dtype=np.float32)  # grid of 8, width 4
+    idx, val = R.interp_tables(c, [4], [b], [8])
+    # c=3: x0=ceil(1)=1 -> taps 1..4 at offsets -2,-1,0,1
+    assert idx[0][0].tolist() == [1, 2, 3, 4]
+    np.testing.assert_allclose(val[0][0], special.i0(b * np.sqrt(1 - np.array([1, .25, 0, .25]))), rtol=6e-7)
+    # c=0.5: x0=ceil(-1.5)=-1 -> taps -1..2, wrapped mod 8
+    assert idx[0][1].tolist() == [7, 0, 1, 2]
+    u = np.array([-1.5, -0.5, 0.5, 1.5]) / 2
+    np.testing.assert_allclose(val[0][1], special.i0(b * np.sqrt(1 - u * u)), rtol=6e-7)
+
+
+@pytest.mark.parametrize("W,s,n", [(4, 2.0, 64), (6, 2.0, 48), (4, 1.25, 40)])
+def test_apodization_inverts_kernel_transform(W, s, n):
+    """apod(i) * phi_hat((i - n//2)/G) is constant, with phi_hat the Fourier
+    transform of the exact KB kernel computed by quadrature."""
+    beta = float(R.kb_beta([W], [s])[0])
+    G = math.ceil(s * n)
+
+    def phi_hat(u):
+        f = lambda t: special.i0(beta * math.sqrt(max(0.0, 1 - (2 * t / W) ** 2))) * math.cos(2 * math.pi * u * t)
+        return 2 * integrate.quad(f, 0, W / 2, limit=200, epsabs=0, epsrel=1e-12)[0]
+
+    a = R.apod_vector(n, s, W, beta)
+    prod = np.array([a[i] * phi_hat((i - n // 2) / G) for i in range(n)])
+    assert np.max(np.abs(prod / prod[n // 2] - 1)) < 1e-6
+
+
+def test_resize_known_answer():
+    x = np.arange(1, 5)
+    np.testing.assert_array_equal(R.resize(x, [8]), [0, 0, 1, 2, 3, 4, 0, 0])
+    np.testing.assert_array_equal(R.resize(np.arange(8), [4]), [2, 3, 4, 5])
+    # the centre element n//2 lands on o//2
+    np.testing.assert_array_equal(R.resize(np.arange(3), [6]), [0, 0, 0, 1, 2, 0])
+
+
+@pytest.mark.parametrize("n", [6, 7])
+def test_centered_fft_is_centered_dft(n):
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    r = np.arange(n) - n // 2
+    direct = np.array([np.sum(x * np.exp(-2j * np.pi * k * r / n)) for k in r])
+    np.testing.assert_allclose(R.fftc(x, 1), direct, rtol=1e-12, atol=1e-12)
+
+
+# ---- operators ---------------------------------------------------------------
+@pytest.mark.parametrize("W,bound", [(4, 2e-3), (6, 5e-5)])
+def test_nufft_2d_vs_nudft(W, bound):
+    rng = np.random.default_rng(1)
+    N = 32
+    coord = demo.golden_angle_radial_2d(16, 64)
+    x = demo.complex_gaussian(rng, (N, N))
+    P = R.Plan(coord, N, width=W, oversamp=2.0)
+    y = R.nufft(x[None, None], P)[0, 0]
+    yt = U.nudft(x[None], P.coord.reshape(-1, 2), [N, N])[0]
+    assert rel(y, yt) < bound
+    k = demo.complex_gaussian(rng, y.shape)
+    xa = R.nufft_adj(k[None, None], P)[0, 0] * np.prod(P.os_shape)
+    xt = U.nudft_adj(k[None], P.coord.reshape(-1, 2), [N, N])[0]
+    assert rel(xa, xt) < bound
+
+
+def test_nufft_1d_3d_vs_nudft():
+    rng = np.random.default_rng(2)
+    c1 = np.sort(rng.uniform(-np.pi, np.pi, (200, 1))).astype(np.float32)
+    c1[0] = -np.pi
+    x1 = demo.complex_gaussian(rng, (40,))
+    P1 = R.Plan(c1, 40, width=6, oversamp=2.0)
+    assert rel(R.nufft(x1[None, None], P1)[0, 0], U.nudft(x1[None], P1.coord.reshape(-1, 1), [40])[0]) < 5e-5
+    c3 = demo.golden_means_radial_3d(64, 32)
+    x3 = demo.complex_gaussian(rng, (12, 12, 12))
+    P3 = R.Plan(c3, 12, width=6, oversamp=2.0)
+    y3 = R.nufft(x3[None, None], P3)[0, 0]
+    assert rel(y3, U.nudft(x3[None], P3.coord.reshape(-1, 3), [12, 12, 12])[0]) < 1e-4
+
+
+def test_cartesian_reduces_to_centered_fft():
+    rng = np.random.default_rng(3)
+    N = 16
+    ky, kx = np.meshgrid(np.arange(N) - N // 2, np.arange(N) - N // 2, indexing="ij")
+    coord = np.stack([kx, ky], -1).astype(np.float32)
+    P = R.Plan(coord, N, width=4, oversamp=2.0)
+    assert P.width == [1, 1] and P.oversamp == [1.0, 1.0]
+    x = demo.complex_gaussian(rng, (N, N))
+    y = R.nufft(x[None, None], P)[0, 0].reshape(N, N)
+    ref = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(x.astype(np.complex128))))
+    np.testing.assert_allclose(y, ref, rtol=1e-10, atol=1e-10)
+
+
+def test_adjointness_ortho():
+    rng = np.random.default_rng(4)
+    coord = demo.golden_angle_radial_2d(12, 40)
+    P = R.Plan(coord, 20, width=4, oversamp=1.5)
+    x = demo.complex_gaussian(rng, (1, 1, 20, 20))
+    y = demo.complex_gaussian(rng, (1, 1, P.npts))
+    lhs = np.vdot(R.nufft(x, P, norm="ortho"), y)
+    rhs = np.vdot(x, R.nufft_adj(y, P, norm="ortho"))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_subspace_equals_frame_loop():
+    rng = np.random.default_rng(5)
+    F, K, N = 6, 3, 20
+    coord = np.stack([demo.golden_angle_radial_2d(3, 32) for _ in range(F)])
+    for f in range(F):
+        th = 0.7 * f
+        rot = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+        coord[f] = coord[f] @ rot.T
+    basis = demo.t2_subspace_basis(rng, F, K)
+    P = R.Plan(coord, N, width=4, oversamp=2.0)
+    X = demo.complex_gaussian(rng, (1, K, N, N))
+    y = R.nufft(X, P, basis=basis)[0]
+    for f in range(F):
+        Pf = _replan(P.coord[f : f + 1], N)  # single-frame plan on the same grid coordinates
+        yf = sum(basis[f, k] * R.nufft(X[:, k : k + 1], Pf)[0, 0] for k in range(K))
+        assert rel(y[f], yf) < 1e-12
+        yt = U.nudft(np.tensordot(basis[f], X[0], axes=(0, 0))[None], P.coord[f].reshape(-1, 2), [N, N])[0]
+        assert rel(y[f], yt) < 3e-3
+
+
+def _replan(coord_grid, N):
+    return R.Plan(coord_grid, N, width=4, oversamp=2.0, normalize=False)
+
+
+def test_b0_model_vs_exact_field_map_signal():
+    rng = np.random.default_rng(6)
+    N = 24
+    coord = demo.variable_density_spiral_2d(4, 256, N)
+    zmap = demo.smooth_field_map(rng, (N, N), 100.0)
+    t = np.linspace(0, 10e-3, 256)
+    bt, ct = R.mri_exp_approx(zmap, t, L=8, nbins=40)
+    # temporal interpolation of exp(-z t) over the map's own values
+    z = 2j * math.pi * zmap.ravel()
+    approx = bt.T @ ct.reshape(8, -1)  # (nt, npix)
+    exact = np.exp(-np.outer(t, z))
+    assert rel(approx, exact) < 5e-3
+    P = R.Plan(coord, N, width=6, oversamp=2.0)
+    x = demo.complex_gaussian(rng, (N, N))
+    y = R.nufft(x[None, None], P, b0=(bt, ct))[0, 0]
+    ye = U.nudft_b0(x[None], P.coord.reshape(-1, 2), (N, N), zmap, t)[0]
+    assert rel(y, ye) < 1e-2
+
+
+def test_toeplitz_exact_kernel_equals_gram_matrix():
+    rng = np.random.default_rng(7)
+    N = 12
+    coord = demo.golden_angle_radial_2d(8, 24)
+    P = R.Plan(coord, N, width=4, oversamp=2.0)
+    cg = P.coord.reshape(-1, 2)
+    w = rng.uniform(0.5, 1.5, cg.shape[0])
+    T = R.toeplitz_kernel_exact(cg, (N, N), w)
+    x = demo.complex_gaussian(rng, (N, N))
+    gx = R.toeplitz_apply(x, T, (N, N))
+    explicit = U.nudft_adj((w * U.nudft(x[None], cg, [N, N])[0])[None], cg, [N, N])[0]
+    assert rel(gx, explicit) < 1e-10
+
+
+def test_port_matches_numpy_restatement():
+    from oracle import port as Pt
+
+    rng = np.random.default_rng(8)
+    coord = demo.golden_means_radial_3d(64, 32)
+    P = R.Plan(coord, 16, width=4, oversamp=2.0)
+    pp = Pt.PortPlan(P)
+    x = demo.complex_gaussian(rng, (2, 1, 16, 16, 16))
+    y0 = R.nufft(x, P)
+    assert rel(Pt.nufft(x, pp), y0) < 5e-6
+    assert rel(Pt.nufft_adj(y0.astype(np.complex64), pp), R.nufft_adj(y0, P)) < 5e-6
