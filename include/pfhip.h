@@ -189,6 +189,41 @@ int pf_sort_samples(const pf_plan* plan, const pf_batch_desc* desc, const void* 
 int pf_unsort_samples(const pf_plan* plan, const pf_batch_desc* desc, const void* ksp_sorted,
                       void* kspace, const float* weight, float factor, void* stream);
 
+/* Coordinate derivative of the forward gather (autograd of nufft/nufft_adj
+ * with respect to the trajectory).  For tile-sorted h (F*npts, C) and the
+ * forward grid (pf_pad_fft of the image), writes per sorted sample
+ *   dcoord[s, a] = Re(sum_c conj(h[s, c]) * d(gather_c)/d(c_os[s, a]))
+ * for the oversampled coordinate axes a = (z, y, x) (0 on unused axes), using
+ * the analytic derivative of the Kaiser-Bessel tap weights; the caller applies
+ * 1/scale, the norm factor and the coordinate unit scaling.  Replaces the
+ * incomplete _nufft_backward, src/pyfourier/_nufft/_nufft.py:175-209. */
+int pf_interp_grad(const pf_plan* plan, const pf_batch_desc* desc, const void* grid, const void* h_sorted,
+                   float* dcoord_sorted, void* stream);
+
+/* ---- Toeplitz-embedded Gram operator ------------------------------------ */
+
+typedef struct pf_toep pf_toep;
+
+/* Build a Gram operator from its kernel.  Replaces the GramMatrix of
+ * plan_spgram, src/pyfourier/_sparsegram/_plan.py:20-270.
+ * kernel: (prod(2N), R, R) complex64 device array, R = nseg * ncoeff, the
+ *         uncentred FFT of the wrapped point-spread functions on the 2N grid
+ *         (index order of an uncentred FFT), already scaled as the operator
+ *         needs; copied into the object.
+ * shape:  host (ndim,) image matrix N (z, y, x); 2N must factor into 2, 3, 5. */
+int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg, int ncoeff, void* stream,
+                       pf_toep** out);
+int pf_toeplitz_destroy(pf_toep* op);
+int pf_toeplitz_workspace_size(const pf_toep* op, int batch, size_t* bytes);
+
+/* y = crop_N(IFFT(T (x) FFT(pad_2N(Ct_l * x)))) summed with conj(Ct_l) over
+ * B0 segments; per-voxel R x R kernel for subspace/B0.  Replaces
+ * _sparsegram/_do_sparsegram and _bdot, src/pyfourier/_sparsegram/_sparsegram.py:11-81,
+ * src/pyfourier/_subroutines/_bmatvec/_bdot.py:16-157.
+ * image, out: (batch, ncoeff, N...) complex64; b0_spatial: (nseg, N...) or NULL. */
+int pf_toeplitz_apply(pf_toep* op, int batch, const void* image, void* out, const void* b0_spatial,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,7 @@ reference repository root):
 * subspace spread      src/pyfourier/_subroutines/_grid/_noncart2cart/_2D_subspace.py:13-53
 * operator pipeline    src/pyfourier/_nufft/_nufft.py:11-169,218-287
 * B0 segmentation      src/pyfourier/_subroutines/_b0.py:10-135
+* coordinate gradient  src/pyfourier/_nufft/_nufft.py:175-209 (intent)
 * Toeplitz Gram        src/pyfourier/_sparsegram/_plan.py:108-249,
                        src/pyfourier/_sparsegram/_sparsegram.py:11-81
 
@@ -77,6 +78,41 @@ def kb(x, beta):
         for c in reversed(_AS_LARGE):
             large = large * it + c
         large = np.where(u >= 3.75, u ** -0.5 * np.exp(u) * large, 0.0)
+    val = np.where(u < 3.75, small, large)
+    return np.where(inside, val, 0.0)
+
+
+def kb_deriv(x, beta):
+    """d kb(x, beta) / dx (float64), 0 for |x| > 1.
+
+    Differentiates the same two polynomial branches as ``kb``: with
+    u = beta sqrt(1 - x^2), du^2/dx = -2 beta^2 x and du/dx = -beta^2 x / u.
+    Used by the coordinate gradient (the reference's _nufft_backward,
+    src/pyfourier/_nufft/_nufft.py:175-209, states the intent only).
+    """
+    x = np.asarray(x, dtype=np.float64)
+    beta = float(beta)
+    inside = np.abs(x) <= 1.0
+    u2 = beta * beta * np.where(inside, 1.0 - x * x, 0.0)
+    u = np.sqrt(u2)
+    t2 = u2 / (3.75 * 3.75)
+    # small branch: P(t2) = sum_k a_k t2^k -> dP/dx = P'(t2) * dt2/dx
+    dp = 0.0
+    for k in range(len(_AS_SMALL) - 1, 0, -1):
+        dp = dp * t2 + k * _AS_SMALL[k]
+    small = dp * (-2.0 * beta * beta * x / (3.75 * 3.75))
+    # large branch: Q(u) = u^-1/2 e^u q(3.75/u)
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        us = np.where(u >= 3.75, u, 3.75)
+        s = 3.75 / us
+        q = 0.0
+        dq = 0.0
+        for k in range(len(_AS_LARGE) - 1, -1, -1):
+            q = q * s + _AS_LARGE[k]
+            if k > 0:
+                dq = dq * s + k * _AS_LARGE[k]
+        dQ = us ** -0.5 * np.exp(us) * (q * (1.0 - 0.5 / us) - dq * 3.75 / (us * us))
+        large = dQ * (-beta * beta * x / us)
     val = np.where(u < 3.75, small, large)
     return np.where(inside, val, 0.0)
 
@@ -265,13 +301,33 @@ class Plan:
 # Gather / spread kernels (per frame), all batch items at once.
 # _cart2noncart/_3D.py:13-51 and _noncart2cart/_3D.py:13-51 (and 1D/2D twins)
 # --------------------------------------------------------------------------
-def _tap_iter(plan, frame):
+def deriv_tables(plan):
+    """Per-axis d(tap weight)/d(c_os) tables, ordered (x, y, z): the analytic
+    derivative of KB((x0+i-c)/(W/2)) with the taps x0 held fixed, 0 on W=1
+    axes (their weight does not depend on c)."""
+    c = plan.coord_os.reshape(-1, plan.ndim).astype(np.float64)
+    out = []
+    for a, (w, b) in enumerate(zip(plan.width[::-1], plan.beta[::-1])):
+        if w == 1:
+            out.append(np.zeros((c.shape[0], 1)))
+            continue
+        x0 = np.ceil(c[:, a] - w / 2)
+        taps = x0[:, None] + np.arange(w)[None, :]
+        out.append(-kb_deriv((taps - c[:, a, None]) / (w / 2), b) / (w / 2))
+    return out
+
+
+def _tap_iter(plan, frame, daxis=None):
     """Yield (flat grid index array, weight array) for every tap combination
-    of the points of one frame, in (z, y, x) loop order."""
+    of the points of one frame, in (z, y, x) loop order.  With daxis (an
+    index into (x, y, z)) the weights are differentiated along that axis."""
     d = plan.ndim
     sl = slice(frame * plan.npts, (frame + 1) * plan.npts)
     idx = [i[sl] for i in plan.index][::-1]  # (z, y, x)
-    val = [v[sl].astype(np.float64) for v in plan.value][::-1]
+    val = [v[sl].astype(np.float64) for v in plan.value]
+    if daxis is not None:
+        val[daxis] = deriv_tables(plan)[daxis][sl]
+    val = val[::-1]
     gshape = plan.os_shape
     widths = [i.shape[1] for i in idx]
     for taps in np.ndindex(*widths):
@@ -283,14 +339,15 @@ def _tap_iter(plan, frame):
         yield flat, w
 
 
-def gather(grid, plan):
-    """grid (B, F, *G) -> samples (B, F, M); /scale applied."""
+def gather(grid, plan, daxis=None):
+    """grid (B, F, *G) -> samples (B, F, M); /scale applied.  With daxis the
+    derivative of the samples with respect to c_os[daxis] (x, y, z index)."""
     grid = np.asarray(grid)
     b = grid.shape[0]
     out = np.zeros((b, plan.nframes, plan.npts), dtype=np.complex128)
     for f in range(plan.nframes):
         g = grid[:, f].reshape(b, -1)
-        for flat, w in _tap_iter(plan, f):
+        for flat, w in _tap_iter(plan, f, daxis):
             out[:, f] += w[None] * g[:, flat]
     return out / plan.scale
 
@@ -343,12 +400,14 @@ def _from_grid(g, plan, norm):
     return apodize(x, plan)
 
 
-def nufft(x, plan, basis=None, weight=None, norm=None, b0=None):
+def nufft(x, plan, basis=None, weight=None, norm=None, b0=None, daxis=None):
     """Forward NUFFT.  x (B, F|K, *N) -> y (B, F, M).
 
     Without basis, frame f of x is sampled with frame f's taps.  With
     basis (F, K): y[f] = sum_k B[f,k] A_f X_k (B.8).  With
     b0 = (Bt (L, nsamples), Ct (L, *N)): y = sum_l Bt[l,t] A(Ct_l x) (B.9).
+    With daxis (x, y, z index) returns dy/dc_os[daxis] instead; norm may
+    also be numpy's 'forward' (1/G on the FFT, the scaling of the adjoint).
     """
     x = np.asarray(x, dtype=np.complex128)
     B = x.shape[0]
@@ -359,13 +418,13 @@ def nufft(x, plan, basis=None, weight=None, norm=None, b0=None):
     for bt, ct in segs:
         xs = x if ct is None else x * ct
         if basis is None:
-            ys = gather(_to_grid(xs, plan, norm), plan)
+            ys = gather(_to_grid(xs, plan, norm), plan, daxis)
         else:
             ys = np.zeros((B, F, plan.npts), dtype=np.complex128)
             for k in range(basis.shape[1]):
                 gk = _to_grid(xs[:, k : k + 1], plan, norm)
                 gk = np.broadcast_to(gk, (B, F) + gk.shape[2:])
-                ys += basis[None, :, k, None] * gather(gk, plan)
+                ys += basis[None, :, k, None] * gather(gk, plan, daxis)
         if bt is not None:
             t = np.asarray(bt).reshape(-1)
             ys = ys * np.tile(t, plan.npts // t.size)[None, None]
@@ -407,6 +466,31 @@ def nufft_adj(y, plan, basis=None, weight=None, norm=None, b0=None):
             xs = xs * np.conj(ct)
         x = xs if x is None else x + xs
     return x
+
+
+def coord_grad(x, plan, h, basis=None, weight=None, norm=None, b0=None, adjoint=False):
+    """Trajectory gradient Re(sum_B conj(h) dy/dk) per sample, (F*M, d) in
+    grid units ordered (x, y, z).
+
+    Forward operator y = nufft(x): h is the upstream k-space gradient.
+    Adjoint operator x = nufft_adj(y): pass x = the upstream image gradient,
+    h = y and adjoint=True (the derivative then carries the adjoint's FFT
+    scaling).  Intent of src/pyfourier/_nufft/_nufft.py:175-209 (whose
+    ramp-based formula drops the -2 pi i / N factor, defect A.14).
+    """
+    fnorm = ("forward" if norm is None else norm) if adjoint else norm
+    h = np.asarray(h, dtype=np.complex128)
+    if weight is not None:
+        w = np.asarray(weight).reshape(1, plan.nframes, -1)
+        h = h * (w if adjoint else np.conj(w))
+    out = np.zeros((plan.nframes * plan.npts, plan.ndim))
+    shape_xyz = plan.shape[::-1]
+    os_xyz = plan.os_shape[::-1]
+    for a in range(plan.ndim):
+        dy = nufft(x, plan, basis=basis, norm=fnorm, b0=b0, daxis=a)
+        g = np.real(np.conj(h) * dy).sum(axis=0).reshape(-1)
+        out[:, a] = g * os_xyz[a] / shape_xyz[a]
+    return out
 
 
 # --------------------------------------------------------------------------

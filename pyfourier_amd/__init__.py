@@ -4,6 +4,7 @@ Drop-in entry points (same names, signatures and shape rules as
 INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
 
     plan_nufft, nufft, nufft_adj      forward / adjoint NUFFT (+ subspace, B0)
+    plan_spgram, sparse_gram          Toeplitz-embedded Gram operator
 
 The compute path is libpfhip.so: hand-written HIP kernels for gfx950 behind a
 C ABI (include/pfhip.h), called through ctypes on PyTorch-ROCm streams.
@@ -14,6 +15,7 @@ from ._nufft import nufft, nufft_adj
 from ._plan import NUFFTPlan, plan_nufft
 from ._traj import normalize_coordinates
 from ._b0 import mri_exp_approx
+from ._sparsegram import GramMatrix, plan_spgram, sparse_gram
 
 __all__ = [
     "PFError",
@@ -23,6 +25,9 @@ __all__ = [
     "nufft_adj",
     "normalize_coordinates",
     "mri_exp_approx",
+    "GramMatrix",
+    "plan_spgram",
+    "sparse_gram",
 ]
 
 __version__ = "0.1.0"

@@ -83,6 +83,11 @@ _SIG = {
         [_VP, ctypes.POINTER(BatchDesc)] + [ctypes.POINTER(ctypes.c_size_t)] * 4,
     ),
     "pf_pad_fft": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
+    "pf_interp_grad": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
+    "pf_toeplitz_create":(_I, [_VP, _I, ctypes.POINTER(_I), _I, _I, _VP, ctypes.POINTER(_VP)]),
+    "pf_toeplitz_destroy": (_I, [_VP]),
+    "pf_toeplitz_workspace_size": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_size_t)]),
+    "pf_toeplitz_apply": (_I, [_VP, _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     "pf_ifft_crop": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, ctypes.c_float, _VP, _VP]),
     "pf_sort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
     "pf_unsort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, ctypes.c_float, _VP]),

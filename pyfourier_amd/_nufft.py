@@ -168,6 +168,42 @@ class _Call:
             acc = y if acc is None else acc.add_(y)
         return acc
 
+    # ---- coordinate gradient ------------------------------------------------
+    def coord_grad(self, img: torch.Tensor, h: torch.Tensor, factor: float) -> torch.Tensor:
+        """Re(sum_c conj(h) * d(A img)/d coord) per sample, in user units.
+
+        img (C, nimg, *N) is the image whose forward grid is differentiated;
+        h (C, F, npts) the paired k-space (weight applied here); factor the
+        operator's 1/scale and norm factor.  Returns (F*npts, d) ordered (x, y, z).
+        """
+        it = self.interp
+        C = img.shape[0]
+        M = it.nframes * it.npts
+        L = _lib.lib()
+        acc = torch.zeros((M, 3), dtype=torch.float32, device=self.device)
+        for seg in self.segments:
+            desc = self.desc(C, seg)
+            nbytes = ctypes.c_size_t()
+            _lib.check(L.pf_workspace_size(self.native.handle, ctypes.byref(desc), 1, ctypes.byref(nbytes)))
+            ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
+            grid, ys, fws, _ = self._split_ws(desc, ws)
+            ds = torch.empty((M, 3), dtype=torch.float32, device=self.device)
+            h_, d, st = self.native.handle, ctypes.byref(desc), _lib.stream_of(self.device)
+            with torch.cuda.device(self.device):
+                _lib.check(L.pf_pad_fft(h_, d, _lib.ptr(img), _lib.ptr(grid), _lib.ptr(fws), st))
+                _lib.check(L.pf_sort_samples(h_, d, _lib.ptr(h), _lib.ptr(ys), _lib.ptr(self.weight_flat), st))
+                _lib.check(L.pf_interp_grad(h_, d, _lib.ptr(grid), _lib.ptr(ys), _lib.ptr(ds), st))
+            acc += ds
+        out = torch.empty_like(acc)
+        out[self.native.permutation()] = acc
+        # oversampled-grid axes (z, y, x) -> user axes (x, y, z): d c_os / d coord
+        dd = it.ndim
+        scale = torch.tensor(
+            [factor * it.os_shape[a] / it.shape[a] * it.unit_scale[dd - 1 - a] for a in range(dd)][::-1],
+            dtype=torch.float32, device=self.device,
+        )
+        return out[:, 3 - dd:].flip(1) * scale
+
     # ---- stage-by-stage execution (timing) ---------------------------------
     def _factor(self, norm_code: int, adjoint: bool) -> float:
         g = float(np.prod(self.interp.os_shape))
@@ -287,11 +323,7 @@ class _NUFFTFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
-        call = ctx.call
-        code = _lib.PF_NORM_ORTHO if ctx.norm_code == _lib.PF_NORM_ORTHO else (
-            _lib.PF_NORM_ADJOINT if ctx.norm_code == _lib.PF_NORM_NONE else _lib.PF_NORM_NONE
-        )
-        return call.adjoint(g.contiguous().to(torch.complex64), code), None, None
+        return ctx.call.adjoint(g.contiguous().to(torch.complex64), _adjoint_code(ctx.norm_code)), None, None
 
 
 class _NUFFTAdjFn(torch.autograd.Function):
@@ -305,11 +337,64 @@ class _NUFFTAdjFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
+        return ctx.call.forward(g.contiguous().to(torch.complex64), _adjoint_code(ctx.norm_code)), None, None
+
+
+def _adjoint_code(norm_code: int) -> int:
+    if norm_code == _lib.PF_NORM_ORTHO:
+        return _lib.PF_NORM_ORTHO
+    return _lib.PF_NORM_ADJOINT if norm_code == _lib.PF_NORM_NONE else _lib.PF_NORM_NONE
+
+
+def _coord_grad(call: _Call, coord: torch.Tensor, img: torch.Tensor, h: torch.Tensor, adjoint: bool, norm_code: int):
+    g = call.coord_grad(img, h, call._factor(norm_code, adjoint))
+    return g.reshape(coord.shape).to(device=coord.device, dtype=coord.dtype)
+
+
+class _NUFFTTrajFn(torch.autograd.Function):
+    """y = A(coord) x with gradients for the image and the trajectory.
+
+    Replaces NUFFTTraj of src/pyfourier/_nufft/_fwd.py:275-319: the image
+    gradient is the exact adjoint, the coordinate gradient
+    Re(conj(g) dy/dcoord) comes from the Kaiser-Bessel-derivative gather.
+    """
+
+    @staticmethod
+    def forward(ctx, image, coord, call, norm_code):
+        ctx.call = call
+        ctx.norm_code = norm_code
+        ctx.save_for_backward(image, coord)
+        return call.forward(image, norm_code)
+
+    @staticmethod
+    def backward(ctx, g):
         call = ctx.call
-        code = _lib.PF_NORM_ORTHO if ctx.norm_code == _lib.PF_NORM_ORTHO else (
-            _lib.PF_NORM_ADJOINT if ctx.norm_code == _lib.PF_NORM_NONE else _lib.PF_NORM_NONE
-        )
-        return call.forward(g.contiguous().to(torch.complex64), code), None, None
+        image, coord = ctx.saved_tensors
+        g = g.contiguous().to(torch.complex64)
+        gi = call.adjoint(g, _adjoint_code(ctx.norm_code)) if ctx.needs_input_grad[0] else None
+        gc = _coord_grad(call, coord, image, g, False, ctx.norm_code) if ctx.needs_input_grad[1] else None
+        return gi, gc, None, None
+
+
+class _NUFFTAdjTrajFn(torch.autograd.Function):
+    """x = A(coord)^H y with gradients for k-space and the trajectory
+    (replaces NUFFTAdjointTraj of src/pyfourier/_nufft/_adj.py:261-306)."""
+
+    @staticmethod
+    def forward(ctx, kspace, coord, call, norm_code):
+        ctx.call = call
+        ctx.norm_code = norm_code
+        ctx.save_for_backward(kspace, coord)
+        return call.adjoint(kspace, norm_code)
+
+    @staticmethod
+    def backward(ctx, gx):
+        call = ctx.call
+        kspace, coord = ctx.saved_tensors
+        gx = gx.contiguous().to(torch.complex64)
+        gk = call.forward(gx, _adjoint_code(ctx.norm_code)) if ctx.needs_input_grad[0] else None
+        gc = _coord_grad(call, coord, gx, kspace, True, ctx.norm_code) if ctx.needs_input_grad[1] else None
+        return gk, gc, None, None
 
 
 def _restore(out: torch.Tensor, dtype, backend_numpy: bool, idevice):
@@ -319,6 +404,20 @@ def _restore(out: torch.Tensor, dtype, backend_numpy: bool, idevice):
     if backend_numpy:
         return out.detach().cpu().numpy()
     return out.to(idevice)
+
+
+def _traj_tensor(coord, call):
+    """The coordinate tensor to differentiate, when the caller asks for it.
+
+    As in the reference, the gradient is taken at the plan's trajectory (the
+    one ``coord`` built, or the one a given ``nufft_plan`` was built from).
+    """
+    if isinstance(coord, torch.Tensor) and coord.requires_grad and torch.is_grad_enabled():
+        it = call.interp
+        if coord.numel() != it.nframes * it.npts * it.ndim:
+            raise ValueError(f"coord {tuple(coord.shape)} does not match the plan's trajectory")
+        return coord
+    return None
 
 
 def _prepare(x, coord, plan, shape_arg, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, for_adj):
@@ -334,7 +433,7 @@ def _prepare(x, coord, plan, shape_arg, zmap, L, nbins, dt, T, L_batch_size, wid
             if for_adj:
                 raise ValueError("nufft_adj needs the image shape when no plan is given")
             shape_arg = tuple(x.shape[-c.shape[-1]:])
-        plan = plan_nufft(c.to(torch.float32), shape_arg, width, oversamp, zmap, L, nbins, dt, T, L_batch_size)
+        plan = plan_nufft(c.detach().to(torch.float32), shape_arg, width, oversamp, zmap, L, nbins, dt, T, L_batch_size)
     plan.to(dev)
     return x, backend_numpy, idevice, dev, plan
 
@@ -371,7 +470,8 @@ def nufft(
     call = _Call(plan, dev, basis, weight, norm)
     batch_shape, C, nimg = _image_layout(x, call)
     xin = x.to(device=dev, dtype=torch.complex64).reshape((C, nimg) + tuple(call.interp.shape)).contiguous()
-    y = _NUFFTFn.apply(xin, call, code)
+    tc = _traj_tensor(coord, call)
+    y = _NUFFTFn.apply(xin, call, code) if tc is None else _NUFFTTrajFn.apply(xin, tc, call, code)
     it = call.interp
     oshape = batch_shape + ((it.nframes,) if it.nframes > 1 else ()) + tuple(it.dshape)
     y = y.reshape(oshape)
@@ -415,7 +515,8 @@ def nufft_adj(
     if call.weight_full is not None:
         yin = yin * call.weight_full
     yin = yin.reshape(C, it.nframes, it.npts).contiguous()
-    x = _NUFFTAdjFn.apply(yin, call, code)
+    tc = _traj_tensor(coord, call)
+    x = _NUFFTAdjFn.apply(yin, call, code) if tc is None else _NUFFTAdjTrajFn.apply(yin, tc, call, code)
     keep = call.basis is not None or it.nframes > 1
     oshape = batch_shape + ((call.nimg,) if keep else ()) + tuple(it.shape)
     x = x.reshape(oshape)

@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 from ._b0 import mri_exp_approx
-from ._traj import normalize_coordinates
+from ._traj import normalize_coordinates, unit_factors
 
 
 def resolve_device(device) -> torch.device:
@@ -65,6 +65,17 @@ class NativePlan:
         _lib.check(L.pf_plan_get_info(handle, ctypes.byref(info)))
         self.info = info
         self._lib = L
+        self._perm = None
+
+    def permutation(self) -> torch.Tensor:
+        """Original sample index (frame * npts + j) of every tile-sorted position."""
+        if self._perm is None:
+            n = int(self.info.nframes) * int(self.info.npts)
+            perm = torch.empty(n, dtype=torch.int32, device=self.device)
+            with torch.cuda.device(self.device):
+                _lib.check(self._lib.pf_plan_get_permutation(self.handle, _lib.ptr(perm), _lib.stream_of(self.device)))
+            self._perm = perm.long()
+        return self._perm
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -95,6 +106,7 @@ class Interpolator:
     ndim: int
     dshape: tuple  # (nviews, nsamples)
     nframes: int
+    unit_scale: tuple = ()  # d(grid units)/d(input units) per coord axis (x, y, z)
     _native: dict = field(default_factory=dict, repr=False)
 
     @property
@@ -207,6 +219,7 @@ def plan_nufft(
     width = _per_axis(width, ndim, np.int16)
     oversamp = _per_axis(oversamp, ndim, np.float32)
     shape = _per_axis(shape, ndim, np.int64)
+    units = unit_factors(coord, shape[::-1].tolist())
     coord = normalize_coordinates(coord, shape[::-1].tolist())
     # Cartesian axes get oversamp 1 and width 1 (src/pyfourier/_nufft/_plan.py:139-146)
     is_cart = np.asarray(
@@ -227,6 +240,7 @@ def plan_nufft(
         ndim=ndim,
         dshape=tuple(int(s) for s in coord.shape[1:-1]),
         nframes=int(coord.shape[0]),
+        unit_scale=units,
     )
     if zmap is not None:
         if T is None:

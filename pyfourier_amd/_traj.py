@@ -34,6 +34,14 @@ def coordinate_scale(rmax: float) -> float:
     return 0.0  # grid units
 
 
+def unit_factors(coord: torch.Tensor, shape_xyz) -> tuple:
+    """d(grid-unit coordinate)/d(input coordinate) per axis (x, y, z order):
+    N_a * f for the detected unit, 1 for coordinates already in grid units."""
+    rmax = float(coord.detach().double().pow(2).sum(dim=-1).sqrt().max()) if coord.numel() else 0.0
+    f = coordinate_scale(rmax)
+    return tuple(1.0 if f == 0.0 else float(n) * f for n in shape_xyz)
+
+
 def normalize_coordinates(coord: torch.Tensor, shape_xyz, center: bool = True) -> torch.Tensor:
     """Return float32 coordinates in grid units.
 

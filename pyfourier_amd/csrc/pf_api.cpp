@@ -3,6 +3,7 @@
 #include <cmath>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/pfhip.h"
 #include "pf_internal.h"
@@ -85,6 +86,35 @@ WsLayout layout(const pf_plan* p, const pf_batch& b) {
 
 }  // namespace
 
+// Deapodisation vectors per axis (src/pyfourier/_subroutines/_apod.py:27-41):
+// a(i) = sqrt(beta^2 - (pi W (i - n//2) / G)^2), a/sinh(a), normalised at
+// n//2, NaN -> 1; all ones on width-1 axes.
+int pf_plan_upload_apod(pf_plan* p) {
+    int ntot = p->shape[0] + p->shape[1] + p->shape[2];
+    std::vector<float> h(ntot, 1.0f);
+    int o = 0;
+    for (int a = 0; a < 3; ++a) {
+        int n = p->shape[a];
+        if (p->width[a] > 1) {
+            auto raw = [&](int i) {
+                double t = M_PI * p->width[a] * (double)(i - n / 2) / p->os[a];
+                double v = std::sqrt((double)p->beta[a] * p->beta[a] - t * t);
+                return v / std::sinh(v);
+            };
+            double c = raw(n / 2);
+            for (int i = 0; i < n; ++i) {
+                double v = raw(i) / c;
+                h[o + i] = std::isnan(v) ? 1.0f : (float)v;
+            }
+        }
+        o += n;
+    }
+    hipError_t he = hipMalloc(&p->apod, ntot * sizeof(float));
+    if (he == hipSuccess) he = hipMemcpy(p->apod, h.data(), ntot * sizeof(float), hipMemcpyHostToDevice);
+    if (he != hipSuccess) return pf_fail(PF_ERR_HIP, std::string("apod upload: ") + hipGetErrorString(he));
+    return PF_OK;
+}
+
 extern "C" {
 
 const char* pf_last_error(void) { return g_err.c_str(); }
@@ -107,33 +137,9 @@ int pf_plan_create(const float* coord, int ndim, const int* shape, const int* wi
         pf_plan_free(p);
         return pf_fail(PF_ERR_HIP, "hipGetDevice failed");
     }
-    // deapodisation vectors (src/pyfourier/_subroutines/_apod.py:27-41)
-    int ntot = p->shape[0] + p->shape[1] + p->shape[2];
-    float* h = new float[ntot];
-    int o = 0;
-    for (int a = 0; a < 3; ++a) {
-        int n = p->shape[a];
-        for (int i = 0; i < n; ++i) h[o + i] = 1.0f;
-        if (p->width[a] > 1) {
-            auto raw = [&](int i) {
-                double t = M_PI * p->width[a] * (double)(i - n / 2) / p->os[a];
-                double v = std::sqrt((double)p->beta[a] * p->beta[a] - t * t);
-                return v / std::sinh(v);
-            };
-            double c = raw(n / 2);
-            for (int i = 0; i < n; ++i) {
-                double v = raw(i) / c;
-                h[o + i] = std::isnan(v) ? 1.0f : (float)v;
-            }
-        }
-        o += n;
-    }
-    hipError_t he = hipMalloc(&p->apod, ntot * sizeof(float));
-    if (he == hipSuccess) he = hipMemcpy(p->apod, h, ntot * sizeof(float), hipMemcpyHostToDevice);
-    delete[] h;
-    if (he != hipSuccess) {
+    if ((rc = pf_plan_upload_apod(p)) != PF_OK) {
         pf_plan_free(p);
-        return pf_fail(PF_ERR_HIP, std::string("apod upload: ") + hipGetErrorString(he));
+        return rc;
     }
     rc = pf_plan_build(p, coord, static_cast<hipStream_t>(stream));
     if (rc != PF_OK) {
@@ -282,6 +288,16 @@ int pf_workspace_offsets(const pf_plan* p, const pf_batch_desc* d, size_t* grid_
     *fft_off = w.fft;
     *spread_off = w.spread;
     return PF_OK;
+}
+
+int pf_interp_grad(const pf_plan* p, const pf_batch_desc* d, const void* grid, const void* h_sorted,
+                   float* dcoord_sorted, void* stream) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!grid || !h_sorted || !dcoord_sorted) return pf_fail(PF_ERR_ARG, "null array");
+    return pf_launch_gather_dcoord(p, b, static_cast<const float2*>(grid), static_cast<const float2*>(h_sorted),
+                                   dcoord_sorted, static_cast<hipStream_t>(stream));
 }
 
 int pf_pad_fft(pf_plan* p, const pf_batch_desc* d, const void* image, void* grid, void* fft_workspace,

@@ -156,3 +156,11 @@ __device__ __forceinline__ unsigned pf_xcd_remap(unsigned b, unsigned nb) {
     unsigned base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
     return base + i;
 }
+
+// plan helpers shared by the NUFFT and Toeplitz objects
+int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_zyx,
+                   const float* oversamp_zyx, int nframes, int64_t npts);
+int pf_plan_upload_apod(pf_plan* p);
+void pf_plan_free(pf_plan* p);
+int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* grid, const float2* h, float* out,
+                            hipStream_t s);

@@ -1,0 +1,178 @@
+// Toeplitz-embedded Gram operator (gfx950).
+//
+// Replaces the apply of src/pyfourier/_sparsegram/_sparsegram.py:11-81
+// (pad to 2N -> FFT -> (x) kernel -> IFFT -> crop, with the B0 segment loop)
+// and the per-voxel matvec of src/pyfourier/_subroutines/_bmatvec/_bdot.py:16-157.
+//
+// The object holds a sample-free geometry plan for the 2N embedding (width 1,
+// oversampling 2, unit apodisation), so the pruned fused FFT passes of the
+// NUFFT do the embedding: only rows that hold image data are transformed
+// forward, and the inverse keeps only the N outputs the crop needs.  Between
+// them one kernel applies the per-voxel R x R kernel (R = L segments x K
+// subspace coefficients; R = 1 is a pointwise multiply).  With B0 segments the
+// X pass multiplies by Ct_l on the way in and sums conj(Ct_l) on the way out.
+#include <algorithm>
+#include <new>
+
+#include "pf_internal.h"
+
+struct pf_toep {
+    pf_plan* geo = nullptr;   // 2N embedding geometry (no samples)
+    float2* T = nullptr;      // (Gtot, R, R) kernel in stored FFT order
+    int R = 1;
+    int L = 1;
+    int K = 1;
+};
+
+namespace {
+
+constexpr int TB = 256;
+
+// grid (Cb, Gtot, R, Cv) in place: u[r] <- sum_r' T[pt, r, r'] u[r']
+template <int RMAX>
+__global__ void __launch_bounds__(TB) k_toep_mul(const float2* __restrict__ T, float2* __restrict__ grid,
+                                                 int64_t gtot, int R, int Cv, int Cb) {
+    const int64_t total = (int64_t)Cb * gtot * Cv;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int cv = (int)(e % Cv);
+        const int64_t pt = (e / Cv) % gtot;
+        const int64_t cb = e / (Cv * gtot);
+        float2* g = grid + (cb * gtot + pt) * (int64_t)R * Cv + cv;
+        const float2* t = T + pt * (int64_t)R * R;
+        if (RMAX == 1) {
+            const float2 a = t[0], u = g[0];
+            g[0] = make_float2(a.x * u.x - a.y * u.y, a.x * u.y + a.y * u.x);
+            continue;
+        }
+        float2 u[RMAX];
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r)
+            if (r < R) u[r] = g[r * Cv];
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            if (r >= R) break;
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int q = 0; q < RMAX; ++q) {
+                if (q >= R) break;
+                const float2 a = t[r * R + q];
+                acc.x += a.x * u[q].x - a.y * u[q].y;
+                acc.y += a.x * u[q].y + a.y * u[q].x;
+            }
+            g[r * Cv] = acc;
+        }
+    }
+}
+
+pf_batch toep_batch(const pf_toep* t, int C, const float2* b0s) {
+    pf_batch b;
+    b.C = C;
+    b.Cv = 1;
+    for (int v = std::min(64, C); v >= 1; --v)
+        if (C % v == 0) {
+            b.Cv = v;
+            break;
+        }
+    b.Cb = C / b.Cv;
+    b.L = t->L;
+    b.K = t->K;
+    b.R = t->R;
+    b.nfg = 1;
+    b.nimg = t->K;
+    b.basis = nullptr;
+    b.b0t = nullptr;
+    b.b0s = b0s;
+    b.nt = 1;
+    return b;
+}
+
+size_t align_up(size_t n) { return (n + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg, int ncoeff, void* stream,
+                       pf_toep** out) {
+    if (!out) return pf_fail(PF_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!kernel || !shape) return pf_fail(PF_ERR_ARG, "null argument");
+    if (nseg < 1 || ncoeff < 1) return pf_fail(PF_ERR_ARG, "nseg and ncoeff must be >= 1");
+    const int R = nseg * ncoeff;
+    if (R > 16) return pf_fail(PF_ERR_UNSUPPORTED, "Toeplitz kernel blocks larger than 16x16");
+    pf_toep* t = new (std::nothrow) pf_toep();
+    if (!t) return pf_fail(PF_ERR_ALLOC, "host allocation failed");
+    t->R = R;
+    t->L = nseg;
+    t->K = ncoeff;
+    t->geo = new (std::nothrow) pf_plan();
+    int w[3] = {1, 1, 1};
+    float s[3] = {2.0f, 2.0f, 2.0f};
+    int rc = t->geo ? pf_plan_params(t->geo, ndim, shape, w, s, 1, 1) : pf_fail(PF_ERR_ALLOC, "host allocation");
+    if (rc == PF_OK && !pf_fftk_supported(t->geo))
+        rc = pf_fail(PF_ERR_UNSUPPORTED, "Toeplitz embedding grid must factor into 2, 3 and 5");
+    if (rc == PF_OK) rc = pf_plan_upload_apod(t->geo);
+    if (rc == PF_OK) rc = pf_fftk_init(t->geo);
+    if (rc == PF_OK) {
+        const size_t n = (size_t)pf_gtot(t->geo) * R * R * sizeof(float2);
+        hipError_t e = hipMalloc(&t->T, n);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(t->T, kernel, n, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream));
+        if (e == hipSuccess) e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+        if (e != hipSuccess) rc = pf_fail(PF_ERR_HIP, std::string("Toeplitz kernel upload: ") + hipGetErrorString(e));
+    }
+    if (rc != PF_OK) {
+        if (t->T) (void)hipFree(t->T);
+        if (t->geo) pf_plan_free(t->geo);
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return PF_OK;
+}
+
+int pf_toeplitz_destroy(pf_toep* t) {
+    if (!t) return PF_OK;
+    if (t->T) (void)hipFree(t->T);
+    if (t->geo) pf_plan_free(t->geo);
+    delete t;
+    return PF_OK;
+}
+
+int pf_toeplitz_workspace_size(const pf_toep* t, int batch, size_t* bytes) {
+    if (!t || !bytes) return pf_fail(PF_ERR_ARG, "null argument");
+    if (batch < 1) return pf_fail(PF_ERR_ARG, "batch must be >= 1");
+    pf_batch b = toep_batch(t, batch, nullptr);
+    *bytes = align_up((size_t)b.C * b.R * pf_gtot(t->geo) * sizeof(float2)) + align_up(pf_fftk_extra_bytes(t->geo, b));
+    return PF_OK;
+}
+
+int pf_toeplitz_apply(pf_toep* t, int batch, const void* image, void* out, const void* b0_spatial, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+    if (!t || !image || !out) return pf_fail(PF_ERR_ARG, "null argument");
+    if (batch < 1) return pf_fail(PF_ERR_ARG, "batch must be >= 1");
+    if (t->L > 1 && !b0_spatial) return pf_fail(PF_ERR_ARG, "B0 segments need the spatial maps");
+    pf_batch b = toep_batch(t, batch, static_cast<const float2*>(b0_spatial));
+    const size_t gb = align_up((size_t)b.C * b.R * pf_gtot(t->geo) * sizeof(float2));
+    const size_t need = gb + align_up(pf_fftk_extra_bytes(t->geo, b));
+    if (!workspace || workspace_bytes < need) return pf_fail(PF_ERR_ARG, "workspace too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    float2* grid = static_cast<float2*>(workspace);
+    float2* extra = reinterpret_cast<float2*>(static_cast<char*>(workspace) + gb);
+    int rc = pf_fftk_forward(t->geo, b, static_cast<const float2*>(image), grid, extra, s);
+    if (rc != PF_OK) return rc;
+    const int64_t gtot = pf_gtot(t->geo);
+    const int64_t total = (int64_t)b.Cb * gtot * b.Cv;
+    const unsigned nb = (unsigned)std::min<int64_t>((total + TB - 1) / TB, 1 << 20);
+    if (t->R == 1)
+        hipLaunchKernelGGL(k_toep_mul<1>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, 1, b.Cv, b.Cb);
+    else if (t->R <= 4)
+        hipLaunchKernelGGL(k_toep_mul<4>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, t->R, b.Cv, b.Cb);
+    else
+        hipLaunchKernelGGL(k_toep_mul<16>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, t->R, b.Cv, b.Cb);
+    PF_HIP(hipGetLastError());
+    return pf_fftk_adjoint(t->geo, b, grid, static_cast<float2*>(out), extra, (float)(1.0 / (double)gtot), s);
+}
+
+}  // extern "C"

@@ -226,3 +226,62 @@ def test_port_matches_numpy_restatement():
     y0 = R.nufft(x, P)
     assert rel(Pt.nufft(x, pp), y0) < 5e-6
     assert rel(Pt.nufft_adj(y0.astype(np.complex64), pp), R.nufft_adj(y0, P)) < 5e-6
+
+
+# ---- coordinate gradient -------------------------------------------------------
+@pytest.mark.parametrize("beta", [2.0, 8.99615, 13.8551])
+def test_kb_deriv_matches_finite_differences(beta):
+    x = np.linspace(-0.999, 0.999, 1999)
+    h = 1e-6
+    fd = (R.kb(x + h, beta) - R.kb(x - h, beta)) / (2 * h)
+    d = R.kb_deriv(x, beta)
+    assert np.max(np.abs(fd - d)) < 1e-8 * np.max(np.abs(d))
+    assert R.kb_deriv(1.5, beta) == 0.0
+
+
+def _nudft_torch(x, k, shape):
+    """Differentiable exact NUDFT (complex128): x (B, *N), k (M, d) -> (B, M)."""
+    import torch
+
+    d = len(shape)
+    ph = 0
+    for a in range(d):
+        n = shape[a]
+        r = torch.arange(n, dtype=torch.float64) - n // 2
+        view = [1] * d
+        view[a] = n
+        ph = ph + k[:, d - 1 - a].reshape(-1, *([1] * d)) * r.reshape(view) / n
+    return (torch.exp(-2j * math.pi * ph) * x[:, None]).flatten(2).sum(-1)
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (8, 8, 8)])
+def test_coord_grad_vs_autograd_nudft(shape):
+    """Trajectory gradient of the restated NUFFT (W=6, sigma=2, accurate to
+    ~1e-5) against torch autograd through the exact NUDFT, for the forward
+    and the adjoint (norm None: the adjoint carries 1/G)."""
+    import torch
+
+    rng = np.random.default_rng(11)
+    d, M, B = len(shape), 60, 2
+    k = (rng.uniform(-0.5, 0.5, (M, d)) * np.array(shape[::-1])).astype(np.float32)
+    P = R.Plan(k[None, None], shape, width=6, oversamp=2.0)
+    x = demo.complex_gaussian(rng, (B,) + shape).astype(np.complex128)
+    v = demo.complex_gaussian(rng, (B, M)).astype(np.complex128)
+    # forward: L = Re <v, y>
+    kt = torch.tensor(k.astype(np.float64), requires_grad=True)
+    y = _nudft_torch(torch.tensor(x), kt, shape)
+    y.retain_grad()
+    (torch.tensor(v).conj() * y).real.sum().backward()
+    got = R.coord_grad(x[:, None], P, y.grad.numpy()[:, None])
+    assert rel(got, kt.grad.numpy()) < 3e-4
+    # adjoint: x = A^H v / G, L = Re <u, x>
+    G = float(np.prod(P.os_shape))
+    kt = torch.tensor(k.astype(np.float64), requires_grad=True)
+    npix = int(np.prod(shape))
+    A = _nudft_torch(torch.eye(npix, dtype=torch.complex128).reshape((npix,) + shape), kt, shape)  # (npix, M)
+    xa = ((A.conj() @ torch.tensor(v).T).T / G).reshape((B,) + shape)
+    xa.retain_grad()
+    u = demo.complex_gaussian(rng, (B,) + shape).astype(np.complex128)
+    (torch.tensor(u).conj() * xa).real.sum().backward()
+    got = R.coord_grad(xa.grad.numpy()[:, None], P, v[:, None], adjoint=True)
+    assert rel(got, kt.grad.numpy()) < 3e-4
