@@ -6,6 +6,9 @@ INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
     plan_nufft, nufft, nufft_adj      forward / adjoint NUFFT (+ subspace, B0)
     plan_spgram, sparse_gram          Toeplitz-embedded Gram operator
 
+Beyond the reference: plan_nufft_batch / NUFFTBatch run an uneven list of
+problems (own matrix sizes and trajectories) as one HIP-graph replay.
+
 The compute path is libpfhip.so: hand-written HIP kernels for gfx950 behind a
 C ABI (include/pfhip.h), called through ctypes on PyTorch-ROCm streams.
 """
@@ -16,6 +19,7 @@ from ._plan import NUFFTPlan, plan_nufft
 from ._traj import normalize_coordinates
 from ._b0 import mri_exp_approx
 from ._sparsegram import GramMatrix, plan_spgram, sparse_gram
+from ._batch import NUFFTBatch, plan_nufft_batch
 
 __all__ = [
     "PFError",
@@ -28,6 +32,8 @@ __all__ = [
     "GramMatrix",
     "plan_spgram",
     "sparse_gram",
+    "NUFFTBatch",
+    "plan_nufft_batch",
 ]
 
 __version__ = "0.1.0"

@@ -140,10 +140,11 @@ class _Call:
         return d
 
     # ---- native operator calls on (C, nimg, *N) / (C, F, npts) ------------
-    def forward(self, x: torch.Tensor, norm_code: int) -> torch.Tensor:
+    def forward(self, x: torch.Tensor, norm_code: int, out: torch.Tensor | None = None) -> torch.Tensor:
         C = x.shape[0]
         it = self.interp
-        out = torch.empty((C, it.nframes, it.npts), dtype=torch.complex64, device=self.device)
+        if out is None:
+            out = torch.empty((C, it.nframes, it.npts), dtype=torch.complex64, device=self.device)
         if C == 0:
             return out
         L = _lib.lib()
@@ -248,10 +249,11 @@ class _Call:
         t.record("ifft_crop", lambda: _lib.check(
             L.pf_ifft_crop(h, d, _lib.ptr(grid), _lib.ptr(x), fac, _lib.ptr(fws), st)), dev)
 
-    def adjoint(self, y: torch.Tensor, norm_code: int) -> torch.Tensor:
+    def adjoint(self, y: torch.Tensor, norm_code: int, out: torch.Tensor | None = None) -> torch.Tensor:
         C = y.shape[0]
         it = self.interp
-        out = torch.empty((C, self.nimg) + tuple(it.shape), dtype=torch.complex64, device=self.device)
+        if out is None:
+            out = torch.empty((C, self.nimg) + tuple(it.shape), dtype=torch.complex64, device=self.device)
         if C == 0:
             return out
         L = _lib.lib()

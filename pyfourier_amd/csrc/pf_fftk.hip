@@ -29,6 +29,11 @@
 namespace {
 
 constexpr int FT = 256;       // threads per block
+// minimum waves per SIMD the FFT kernels are compiled for: 4 caps them at
+// 128 VGPRs, so four 256-thread blocks (the LDS limit at G*B = 4096) fit a CU
+#ifndef PF_FFT_WPE
+#define PF_FFT_WPE 4
+#endif
 constexpr int GBMAX = 4096;   // complex values per block (G * B)
 
 struct AxisFFT {
@@ -36,6 +41,7 @@ struct AxisFFT {
     int N;
     int nrad;
     int rad[16];
+    uint64_t nsm[16];  // division magic of Ns before each pass
     const float2* tw;  // (G,) exp(-2 pi i k / G)
 };
 
@@ -119,9 +125,15 @@ __device__ __forceinline__ void dftR(float2* v) {
     else dft8(v);
 }
 
+// floor(q / d) for q < 8192, d <= 4096 with m = ceil(2^32 / d) (exact in that range)
+__device__ __forceinline__ int mdiv(int q, uint64_t m) { return (int)(((uint64_t)(unsigned)q * m) >> 32); }
+__host__ __device__ inline uint64_t magic_of(int d) { return ((1ull << 32) + (uint64_t)d - 1) / (uint64_t)d; }
+
 // One in-place Stockham pass of radix R over B columns (stride LD) of length G.
+// bm, nm: division magics of B and Ns.
 template <int R>
-__device__ void stockham_pass(float2* buf, int LD, int B, int G, int Ns, const float2* tw) {
+__device__ void stockham_pass(float2* buf, int LD, int B, uint64_t bm, int G, int Ns, uint64_t nm,
+                              const float2* tw) {
     constexpr int CMAX = (GBMAX + R * FT - 1) / (R * FT);
     const int nb = G / R;
     const int total = nb * B;
@@ -129,13 +141,13 @@ __device__ void stockham_pass(float2* buf, int LD, int B, int G, int Ns, const f
     float2 v[CMAX][R];
 #pragma unroll
     for (int c = 0; c < CMAX; ++c) {
-        int q = threadIdx.x + c * FT;
+        const int q = threadIdx.x + c * FT;
         if (q < total) {
-            int b = q % B, j = q / B;
-            int k = j % Ns;
+            const int j = mdiv(q, bm), b = q - j * B;
+            const int k = j - mdiv(j, nm) * Ns;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float2 x = buf[(j + r * nb) * LD + b];
+                const float2 x = buf[(j + r * nb) * LD + b];
                 v[c][r] = r == 0 ? x : c_mul(x, tw[r * k * tstep]);
             }
             dftR<R>(v[c]);
@@ -144,11 +156,11 @@ __device__ void stockham_pass(float2* buf, int LD, int B, int G, int Ns, const f
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < CMAX; ++c) {
-        int q = threadIdx.x + c * FT;
+        const int q = threadIdx.x + c * FT;
         if (q < total) {
-            int b = q % B, j = q / B;
-            int k = j % Ns;
-            int d = (j / Ns) * Ns * R + k;
+            const int j = mdiv(q, bm), b = q - j * B;
+            const int jn = mdiv(j, nm), k = j - jn * Ns;
+            const int d = jn * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) buf[(d + r * Ns) * LD + b] = v[c][r];
         }
@@ -158,14 +170,16 @@ __device__ void stockham_pass(float2* buf, int LD, int B, int G, int Ns, const f
 
 // Full FFT of the LDS block.  Caller has synchronised after filling it.
 __device__ void lds_fft(float2* buf, int LD, int B, const AxisFFT& A, const float2* tw) {
+    const uint64_t bm = magic_of(B);
     int Ns = 1;
     for (int p = 0; p < A.nrad; ++p) {
+        const uint64_t nm = A.nsm[p];
         switch (A.rad[p]) {
-            case 8: stockham_pass<8>(buf, LD, B, A.G, Ns, tw); break;
-            case 5: stockham_pass<5>(buf, LD, B, A.G, Ns, tw); break;
-            case 4: stockham_pass<4>(buf, LD, B, A.G, Ns, tw); break;
-            case 3: stockham_pass<3>(buf, LD, B, A.G, Ns, tw); break;
-            default: stockham_pass<2>(buf, LD, B, A.G, Ns, tw); break;
+            case 8: stockham_pass<8>(buf, LD, B, bm, A.G, Ns, nm, tw); break;
+            case 5: stockham_pass<5>(buf, LD, B, bm, A.G, Ns, nm, tw); break;
+            case 4: stockham_pass<4>(buf, LD, B, bm, A.G, Ns, nm, tw); break;
+            case 3: stockham_pass<3>(buf, LD, B, bm, A.G, Ns, nm, tw); break;
+            default: stockham_pass<2>(buf, LD, B, bm, A.G, Ns, nm, tw); break;
         }
         Ns *= A.rad[p];
     }
@@ -206,7 +220,7 @@ __device__ __forceinline__ void x_row_decode(const Batch& bt, int64_t row, int64
     fg = (int)(gc / bt.Cb);
 }
 
-__global__ void __launch_bounds__(FT) k_fftx_fwd(Geo g, Batch bt, AxisFFT A, const float* __restrict__ apod,
+__global__ void __launch_bounds__(FT, PF_FFT_WPE) k_fftx_fwd(Geo g, Batch bt, AxisFFT A, const float* __restrict__ apod,
                                                  const float2* __restrict__ img, float2* __restrict__ out, XBlk xb) {
     extern __shared__ float2 sh[];
     const int RC = bt.R * bt.Cv;
@@ -260,7 +274,7 @@ __global__ void __launch_bounds__(FT) k_fftx_fwd(Geo g, Batch bt, AxisFFT A, con
     }
 }
 
-__global__ void __launch_bounds__(FT) k_fftx_adj(Geo g, Batch bt, AxisFFT A, const float* __restrict__ apod,
+__global__ void __launch_bounds__(FT, PF_FFT_WPE) k_fftx_adj(Geo g, Batch bt, AxisFFT A, const float* __restrict__ apod,
                                                  const float2* __restrict__ in, float2* __restrict__ img, XBlk xb,
                                                  float factor) {
     extern __shared__ float2 sh[];
@@ -326,7 +340,7 @@ __global__ void __launch_bounds__(FT) k_fftx_adj(Geo g, Batch bt, AxisFFT A, con
 // adj: G input rows -> inverse FFT -> N compact output rows
 // ---------------------------------------------------------------------------
 template <bool FWD>
-__global__ void __launch_bounds__(FT) k_fftcol(AxisFFT A, int64_t O, int64_t I, int B,
+__global__ void __launch_bounds__(FT, PF_FFT_WPE) k_fftcol(AxisFFT A, int64_t I, int B, int bshift,
                                                const float2* __restrict__ in, float2* __restrict__ out) {
     extern __shared__ float2 sh[];
     const int G = A.G, N = A.N;
@@ -340,18 +354,19 @@ __global__ void __launch_bounds__(FT) k_fftcol(AxisFFT A, int64_t O, int64_t I, 
     const int bw = (int)std::min<int64_t>(B, I - i0);
     for (int i = threadIdx.x; i < G; i += FT) tw[i] = A.tw[i];
     if (FWD) {
-        for (int i = threadIdx.x; i < G * LD; i += FT) buf[i] = make_float2(0.f, 0.f);
-        __syncthreads();
+        // rows that receive no image data (stored positions N - N/2 .. G - N/2 - 1)
+        const int z0 = N - N / 2;
+        for (int i = threadIdx.x; i < (G - N) * LD; i += FT) buf[z0 * LD + i] = make_float2(0.f, 0.f);
         const float2* src = in + o * (int64_t)N * I + i0;
-        for (int q = threadIdx.x; q < N * B; q += FT) {
-            int b = q % B, p = q / B;
-            if (b < bw) buf[stored_pos(p, N, G) * LD + b] = src[(int64_t)p * I + b];
+        for (int q = threadIdx.x; q < (N << bshift); q += FT) {
+            const int b = q & (B - 1), p = q >> bshift;
+            buf[stored_pos(p, N, G) * LD + b] = b < bw ? src[(int64_t)p * I + b] : make_float2(0.f, 0.f);
         }
     } else {
         const float2* src = in + o * (int64_t)G * I + i0;
-        for (int q = threadIdx.x; q < G * B; q += FT) {
-            int b = q % B, n = q / B;
-            float2 x = b < bw ? src[(int64_t)n * I + b] : make_float2(0.f, 0.f);
+        for (int q = threadIdx.x; q < (G << bshift); q += FT) {
+            const int b = q & (B - 1), n = q >> bshift;
+            const float2 x = b < bw ? src[(int64_t)n * I + b] : make_float2(0.f, 0.f);
             buf[n * LD + b] = make_float2(x.x, -x.y);
         }
     }
@@ -359,16 +374,16 @@ __global__ void __launch_bounds__(FT) k_fftcol(AxisFFT A, int64_t O, int64_t I, 
     lds_fft(buf, LD, B, A, tw);
     if (FWD) {
         float2* dst = out + o * (int64_t)G * I + i0;
-        for (int q = threadIdx.x; q < G * B; q += FT) {
-            int b = q % B, n = q / B;
+        for (int q = threadIdx.x; q < (G << bshift); q += FT) {
+            const int b = q & (B - 1), n = q >> bshift;
             if (b < bw) dst[(int64_t)n * I + b] = buf[n * LD + b];
         }
     } else {
         float2* dst = out + o * (int64_t)N * I + i0;
-        for (int q = threadIdx.x; q < N * B; q += FT) {
-            int b = q % B, p = q / B;
+        for (int q = threadIdx.x; q < (N << bshift); q += FT) {
+            const int b = q & (B - 1), p = q >> bshift;
             if (b < bw) {
-                float2 x = buf[stored_pos(p, N, G) * LD + b];
+                const float2 x = buf[stored_pos(p, N, G) * LD + b];
                 dst[(int64_t)p * I + b] = make_float2(x.x, -x.y);
             }
         }
@@ -448,7 +463,12 @@ static AxisFFT axis_desc(const pf_fftk_axis& X) {
     A.G = X.G;
     A.N = X.N;
     A.nrad = (int)X.rad.size();
-    for (int i = 0; i < A.nrad; ++i) A.rad[i] = X.rad[i];
+    int ns = 1;
+    for (int i = 0; i < A.nrad; ++i) {
+        A.rad[i] = X.rad[i];
+        A.nsm[i] = magic_of(ns);
+        ns *= X.rad[i];
+    }
     A.tw = X.tw;
     return A;
 }
@@ -518,13 +538,16 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
     const pf_fftk_axis& X = p->fftk->ax[axis];
     AxisFFT A = axis_desc(X);
     const int B = cols_for(X.G);
+    int bshift = 0;
+    while ((1 << bshift) < B) ++bshift;
     const int64_t nblk = O * ((I + B - 1) / B);
     if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+    if (nblk < 1) return PF_OK;
     size_t lds = ((size_t)X.G * (B + 1) + X.G) * sizeof(float2);
     if (fwd)
-        hipLaunchKernelGGL(k_fftcol<true>, dim3((unsigned)nblk), dim3(FT), lds, s, A, O, I, B, src, dst);
+        hipLaunchKernelGGL(k_fftcol<true>, dim3((unsigned)nblk), dim3(FT), lds, s, A, I, B, bshift, src, dst);
     else
-        hipLaunchKernelGGL(k_fftcol<false>, dim3((unsigned)nblk), dim3(FT), lds, s, A, O, I, B, src, dst);
+        hipLaunchKernelGGL(k_fftcol<false>, dim3((unsigned)nblk), dim3(FT), lds, s, A, I, B, bshift, src, dst);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }

@@ -471,12 +471,20 @@ __global__ void __launch_bounds__(TB) k_spread_reduce(GeoParams g, BatchParams B
 // One wave owns one tile (TZ x TY x 2*TXL points of the plan's tiling) for
 // a chunk of 32 values of the flat per-point vector (Cb, R, Cv): lane =
 // (value, x half), each lane keeps its TZ*TY*TXL complex accumulators in
-// VGPRs and walks the tile's sample list in order.  Tap rows of a sample are
-// uniform across the wave (scalar branches skip rows outside the footprint);
-// each useful tap costs two FMAs.  No LDS and no atomics: the summation
-// order is the sample order, so results are run-to-run bit-identical.
+// VGPRs and walks the tile's sample list in order.
+//
+// The list is consumed in batches of SPREAD_EB entries: first every lane
+// fetches one entry (sample index, tap offsets relative to the tile, tap
+// weights) into LDS, then the lanes fetch the batch's sample values (all
+// independent loads, issued back to back), then the wave accumulates from
+// LDS.  One global round trip per batch instead of two dependent ones per
+// entry.  Tap rows of an entry are uniform across the wave (scalar branches
+// skip rows outside the footprint); each useful tap costs two FMAs.  No
+// atomics: the summation order is the sample order, so results are
+// run-to-run bit-identical.
 // -------------------------------------------------------------------------
 constexpr int SPREAD_VL = 32;
+constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 
 template <int TZ, int TY, int TXL>
 __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, const int4* __restrict__ items,
@@ -486,6 +494,12 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
                                                    const float2* __restrict__ ys, float2* __restrict__ grid,
                                                    float2* __restrict__ partial) {
     constexpr int VL = SPREAD_VL;
+    constexpr int EB = SPREAD_EB;
+    __shared__ int s_idx[EB];
+    __shared__ int s_off[EB][4];               // tile-relative tap offsets (z, y, x), frame
+    __shared__ float s_w[EB][3 * PF_WMAX];     // tap weights (z, y, x)
+    __shared__ float2 s_val[EB][VL];           // values times conj(coefficient)
+
     const int RC = B.R * B.Cv;
     const int V = B.Cb * RC;
     const int nvch = (V + VL - 1) / VL;
@@ -521,50 +535,75 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
 #pragma unroll
             for (int k = 0; k < TXL; ++k) acc[a][b][k] = make_float2(0.f, 0.f);
 
-    for (int e = it.y; e < it.z; ++e) {
-        const int64_t s = slist[e];
-        const float* rec = taprec + s * g.rec;
-        const int* ri = reinterpret_cast<const int*>(rec);
-        // tap index of tile position p along an axis: (o + p - start) mod G
-        int bz = oz - ri[0];
-        bz += bz < 0 ? g.os[0] : 0;
-        int by = oy - ri[1];
-        by += by < 0 ? g.os[1] : 0;
-        int bx = ox - ri[2];
-        bx += bx < 0 ? g.os[2] : 0;
-        float wxl[TXL];
-#pragma unroll
-        for (int k = 0; k < TXL; ++k) {
-            int i = bx + half * TXL + k;
-            i -= i >= g.os[2] ? g.os[2] : 0;
-            const float w = rec[g.woff[2] + min(i, Wx - 1)];
-            wxl[k] = i < Wx ? w : 0.f;
+    for (int e0 = it.y; e0 < it.z; e0 += EB) {
+        const int n = min(EB, it.z - e0);
+        // phase 1: one entry per lane
+        if (lane < n) {
+            const int s = slist[e0 + lane];
+            const float* rec = taprec + (int64_t)s * g.rec;
+            const int* ri = reinterpret_cast<const int*>(rec);
+            // tap index of tile position p along an axis: (o + p - start) mod G
+            int bz = oz - ri[0];
+            bz += bz < 0 ? g.os[0] : 0;
+            int by = oy - ri[1];
+            by += by < 0 ? g.os[1] : 0;
+            int bx = ox - ri[2];
+            bx += bx < 0 ? g.os[2] : 0;
+            s_idx[lane] = s;
+            s_off[lane][0] = bz;
+            s_off[lane][1] = by;
+            s_off[lane][2] = bx;
+            s_off[lane][3] = ri[3];
+            for (int k = 0; k < Wz; ++k) s_w[lane][k] = rec[g.woff[0] + k];
+            for (int k = 0; k < Wy; ++k) s_w[lane][PF_WMAX + k] = rec[g.woff[1] + k];
+            for (int k = 0; k < Wx; ++k) s_w[lane][2 * PF_WMAX + k] = rec[g.woff[2] + k];
         }
-        float2 val = vok ? ys[s * B.C + ycol] : make_float2(0.f, 0.f);
-        if (B.basis || B.b0t) {
-            const int64_t tread = B.b0t ? ((int64_t)perm[s] % g.npts) % B.nt : 0;
-            val = cmulc(val, sample_coef(B, r, ri[3], tread));
+        __syncthreads();
+        // phase 2: the batch's values, lane (value, half) takes entries half, half+2, ...
+        for (int j = half; j < n; j += 2) {
+            const int s = s_idx[j];
+            float2 val = vok ? ys[(int64_t)s * B.C + ycol] : make_float2(0.f, 0.f);
+            if (B.basis || B.b0t) {
+                const int64_t tread = B.b0t ? ((int64_t)perm[s] % g.npts) % B.nt : 0;
+                val = cmulc(val, sample_coef(B, r, s_off[j][3], tread));
+            }
+            s_val[j][v] = val;
         }
+        __syncthreads();
+        // phase 3: accumulate
+        for (int j = 0; j < n; ++j) {
+            const int bz = s_off[j][0], by = s_off[j][1], bx = s_off[j][2];
+            const float* w = s_w[j];
+            float wxl[TXL];
 #pragma unroll
-        for (int a = 0; a < TZ; ++a) {
-            int iz = bz + a;
-            iz -= iz >= g.os[0] ? g.os[0] : 0;
-            if (iz >= Wz) continue;
-            const float wz = rec[g.woff[0] + iz];
+            for (int k = 0; k < TXL; ++k) {
+                int i = bx + half * TXL + k;
+                i -= i >= g.os[2] ? g.os[2] : 0;
+                wxl[k] = i < Wx ? w[2 * PF_WMAX + i] : 0.f;
+            }
+            const float2 val = s_val[j][v];
 #pragma unroll
-            for (int b = 0; b < TY; ++b) {
-                int iy = by + b;
-                iy -= iy >= g.os[1] ? g.os[1] : 0;
-                if (iy >= Wy) continue;
-                const float wzy = wz * rec[g.woff[1] + iy];
-                const float vx = wzy * val.x, vy = wzy * val.y;
+            for (int a = 0; a < TZ; ++a) {
+                int iz = bz + a;
+                iz -= iz >= g.os[0] ? g.os[0] : 0;
+                if (iz >= Wz) continue;
+                const float wz = w[iz];
 #pragma unroll
-                for (int k = 0; k < TXL; ++k) {
-                    acc[a][b][k].x += wxl[k] * vx;
-                    acc[a][b][k].y += wxl[k] * vy;
+                for (int b = 0; b < TY; ++b) {
+                    int iy = by + b;
+                    iy -= iy >= g.os[1] ? g.os[1] : 0;
+                    if (iy >= Wy) continue;
+                    const float wzy = wz * w[PF_WMAX + iy];
+                    const float vx = wzy * val.x, vy = wzy * val.y;
+#pragma unroll
+                    for (int k = 0; k < TXL; ++k) {
+                        acc[a][b][k].x += wxl[k] * vx;
+                        acc[a][b][k].y += wxl[k] * vy;
+                    }
                 }
             }
         }
+        __syncthreads();
     }
 
     if (!vok) return;

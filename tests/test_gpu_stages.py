@@ -66,7 +66,8 @@ def test_pad_fft_stage(cuda_device, shape, oversamp):
     _lib.check(L.pf_pad_fft(nat.handle, ctypes.byref(desc), _lib.ptr(xd), _lib.ptr(grid), _lib.ptr(fws), st))
     torch.cuda.synchronize()
     G = P.os_shape
-    Cv = 1 if C % 2 else 2
+    # grid layout (C / Cv, *G, Cv): Cv is the largest divisor of C up to 64
+    Cv = max(v for v in range(1, min(C, 64) + 1) if C % v == 0)
     g = grid[: C * int(np.prod(G))].cpu().numpy().reshape((C // Cv,) + tuple(G) + (Cv,))
     g = np.moveaxis(g, -1, 1).reshape((C,) + tuple(G))
     ref = R._to_grid(x[:, None], P, None)[:, 0]

@@ -37,11 +37,13 @@ def test_gram_vs_exact_and_explicit(cuda_device):
     T = R.toeplitz_kernel_exact(P.coord.reshape(-1, 2), (N, N), w.ravel(), ghat=float(np.prod(P.os_shape)))
     ref = np.stack([R.toeplitz_apply(x[c], T, (N, N)) for c in range(C)])
     assert rel(gx, ref) < 1e-4
-    # explicit normal operator of the engine's own NUFFT (W=6, sigma=2)
+    # explicit normal operator A^H W A of the engine's own NUFFT (W=6, sigma=2);
+    # the weight enters once, as in the reference PSF (weight * delta,
+    # src/pyfourier/_sparsegram/_plan.py:202)
     plan = pf.plan_nufft(torch.from_numpy(coord), N, width=6, oversamp=2.0)
     xd = torch.from_numpy(x).to(cuda_device)
     wd = torch.from_numpy(w).to(cuda_device)
-    ex = pf.nufft_adj(pf.nufft(xd, nufft_plan=plan, weight=wd, device=cuda_device), nufft_plan=plan, weight=wd,
+    ex = pf.nufft_adj(pf.nufft(xd, nufft_plan=plan, weight=wd, device=cuda_device), nufft_plan=plan,
                       device=cuda_device)
     assert rel(gx, ex.cpu().numpy()) < 1e-4
 
