@@ -186,6 +186,25 @@ def _time(fn, dev, steps, warmup=2):
     return (time.perf_counter() - t0) * 1000.0 / steps
 
 
+def stream_copy(dev, nbytes=2 << 30, reps=10):
+    """Achievable HBM bandwidth of a plain device copy (read + write bytes / s)."""
+    import torch
+
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize(dev)
+    gbs = 2 * nbytes * reps / (time.perf_counter() - t0) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(gbs, 1)
+
+
 def bench_uneven_batch(pf, dev, steps):
     """64 2-D slices, matrix sizes from {128, 192, 256}^2, 8 coils, golden-angle
     radial 16..256 spokes x 256 samples, oversampling 2, width 4: fwd+adj of
@@ -356,6 +375,8 @@ def main():
         cpu = cpu_baseline(coord, xc, w, N, args.width, min(args.cpu_coils, C), C)
         log(f"cpu baseline: {cpu['value']:.4f} fwd+adj/s on {cpu['cores']} cores")
 
+    copy_gbs = stream_copy(dev)
+    log(f"device copy: {copy_gbs} GB/s")
     extras = None
     if not args.no_extras and not args.small:
         extras = {}
@@ -390,6 +411,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
+            "copy_GBs": copy_gbs,
         },
         "stages": stage_info,
         "plan_s": round(plan_s, 3),

@@ -25,6 +25,7 @@ reference repository root):
 * operator pipeline    src/pyfourier/_nufft/_nufft.py:11-169,218-287
 * B0 segmentation      src/pyfourier/_subroutines/_b0.py:10-135
 * coordinate gradient  src/pyfourier/_nufft/_nufft.py:175-209 (intent)
+* Pipe-Menon DCF       src/pyfourier/_dcomp/pipemenon.py:83-115
 * Toeplitz Gram        src/pyfourier/_sparsegram/_plan.py:108-249,
                        src/pyfourier/_sparsegram/_sparsegram.py:11-81
 
@@ -170,17 +171,18 @@ def normalize_coordinates(coord, shape_xyz):
     return coord * (shape_xyz * np.float32(f))
 
 
-def scale_coord(coord, shape_xyz, oversamp_xyz):
-    """c_os = c*ceil(sN)/N + ceil(sN)//2 in float32: _subroutines/_traj.py:57-66."""
-    out = np.array(coord, dtype=np.float32, copy=True)
+def scale_coord(coord, shape_xyz, oversamp_xyz, dtype=np.float32):
+    """c_os = c*ceil(sN)/N + ceil(sN)//2 in float32: _subroutines/_traj.py:57-66.
+    dtype float64 gives the exact map (finite-difference checks only)."""
+    out = np.array(coord, dtype=dtype, copy=True)
     for a in range(out.shape[-1]):
         n = np.float32(shape_xyz[a])
         g = np.ceil(np.float32(oversamp_xyz[a]) * n)
-        out[..., a] = out[..., a] * np.float32(g / n) + np.float32(g // 2)
+        out[..., a] = out[..., a] * dtype(g / n) + dtype(g // 2)
     return out
 
 
-def interp_tables(coord_os, width_xyz, beta_xyz, grid_xyz):
+def interp_tables(coord_os, width_xyz, beta_xyz, grid_xyz, dtype=np.float32):
     """Per-axis tap index/value tables, ordered (x, y, z).
 
     index[a] (P, W_a) int32 and value[a] (P, W_a) float32 for P points, from
@@ -189,7 +191,7 @@ def interp_tables(coord_os, width_xyz, beta_xyz, grid_xyz):
     index (x0+i) mod G.  W=1 axes use the integer coordinate with value 1
     (_grid/_plan.py:68-72, intent of defect A.9).
     """
-    c = np.asarray(coord_os, dtype=np.float32).reshape(-1, coord_os.shape[-1])
+    c = np.asarray(coord_os, dtype=dtype).reshape(-1, coord_os.shape[-1])
     index, value = [], []
     for a in range(c.shape[-1]):
         w = int(width_xyz[a])
@@ -197,11 +199,11 @@ def interp_tables(coord_os, width_xyz, beta_xyz, grid_xyz):
         ca = c[:, a].astype(np.float64)
         if w == 1:
             index.append(np.mod(np.rint(ca).astype(np.int64), g).astype(np.int32)[:, None])
-            value.append(np.ones((c.shape[0], 1), dtype=np.float32))
+            value.append(np.ones((c.shape[0], 1), dtype=dtype))
             continue
         x0 = np.ceil(ca - w / 2)
         taps = x0[:, None] + np.arange(w)[None, :]
-        value.append(kb((taps - ca[:, None]) / (w / 2), beta_xyz[a]).astype(np.float32))
+        value.append(kb((taps - ca[:, None]) / (w / 2), beta_xyz[a]).astype(dtype))
         index.append(np.mod(taps, g).astype(np.int32))
     return index, value
 
@@ -256,8 +258,11 @@ class Plan:
     ``coord`` (F, ..., d) in any unit; ``shape`` the image matrix (z, y, x).
     """
 
-    def __init__(self, coord, shape, width=4, oversamp=1.25, normalize=True):
-        coord = np.asarray(coord, dtype=np.float32)
+    def __init__(self, coord, shape, width=4, oversamp=1.25, normalize=True, precise=False):
+        # precise: coordinates in grid units kept in float64 end to end (for
+        # finite-difference checks of derivatives; not the reference's rounding)
+        dt = np.float64 if precise else np.float32
+        coord = np.asarray(coord, dtype=dt)
         d = coord.shape[-1]
         while coord.ndim < 4:
             coord = coord[None]
@@ -270,7 +275,7 @@ class Plan:
         oversamp = [oversamp] * d if np.isscalar(oversamp) else list(oversamp)
         self.shape = [int(s) for s in shape]  # (z, y, x)
         shape_xyz = self.shape[::-1]
-        if normalize:
+        if normalize and not precise:
             coord = normalize_coordinates(coord, shape_xyz)
         self.coord = coord  # grid units, (F, nv, ns, d)
         # Cartesian axes (all integer) get sigma=1, W=1 (_nufft/_plan.py:139-146)
@@ -283,12 +288,13 @@ class Plan:
         self.oversamp = [float(s) for s in oversamp]
         self.beta = [float(b) for b in kb_beta(width, oversamp)]
         self.os_shape = os_shape(self.shape, oversamp)  # (z, y, x)
-        self.coord_os = scale_coord(coord, shape_xyz, list(oversamp[::-1]))
+        self.coord_os = scale_coord(coord, shape_xyz, list(oversamp[::-1]), dt)
         self.index, self.value = interp_tables(
             self.coord_os.reshape(-1, d),
             self.width[::-1],
             self.beta[::-1],
             self.os_shape[::-1],
+            dt,
         )
         self.scale = kernel_scale(self.beta[::-1], self.width[::-1])
 
@@ -491,6 +497,20 @@ def coord_grad(x, plan, h, basis=None, weight=None, norm=None, b0=None, adjoint=
         g = np.real(np.conj(h) * dy).sum(axis=0).reshape(-1)
         out[:, a] = g * os_xyz[a] / shape_xyz[a]
     return out
+
+
+def pipe_menon_dcf(coord, shape, niter=30, width=6, oversamp=1.5):
+    """Pipe-Menon density compensation, src/pyfourier/_dcomp/pipemenon.py:83-115:
+    w = 1, then niter times w <- w / |G G^H w| with the ortho NUFFT, each
+    operator's output cast back to real (its real part), as the reference's
+    float dtype round trip does.  Returns w of shape coord.shape[:-1]."""
+    P = Plan(coord, shape, width=width, oversamp=oversamp)
+    w = np.ones((1, P.nframes, P.npts))
+    for _ in range(niter):
+        x = nufft_adj(w.astype(np.complex128), P, norm="ortho").real
+        y = nufft(x.astype(np.complex128), P, norm="ortho").real
+        w = w / np.abs(y)
+    return w.reshape(np.asarray(coord).shape[:-1])
 
 
 # --------------------------------------------------------------------------

@@ -5,6 +5,7 @@ INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
 
     plan_nufft, nufft, nufft_adj      forward / adjoint NUFFT (+ subspace, B0)
     plan_spgram, sparse_gram          Toeplitz-embedded Gram operator
+    pipe_menon_dcf                    iterative density compensation
 
 Beyond the reference: plan_nufft_batch / NUFFTBatch run an uneven list of
 problems (own matrix sizes and trajectories) as one HIP-graph replay.
@@ -20,6 +21,7 @@ from ._traj import normalize_coordinates
 from ._b0 import mri_exp_approx
 from ._sparsegram import GramMatrix, plan_spgram, sparse_gram
 from ._batch import NUFFTBatch, plan_nufft_batch
+from ._dcomp import pipe_menon_dcf
 
 __all__ = [
     "PFError",
@@ -34,6 +36,7 @@ __all__ = [
     "sparse_gram",
     "NUFFTBatch",
     "plan_nufft_batch",
+    "pipe_menon_dcf",
 ]
 
 __version__ = "0.1.0"

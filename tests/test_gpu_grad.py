@@ -145,3 +145,25 @@ def test_grid_unit_coordinates_and_gradient_descent_step(cuda_device):
     with torch.no_grad():
         L1 = loss(ct - step * ct.grad)
     assert float(L1) < float(L0)
+
+
+def test_coord_grad_vs_nudft_gradient(cuda_device):
+    """Accurate kernel (W=6, sigma=2): the trajectory gradient of the engine
+    against torch autograd through the exact FP64 NUDFT."""
+    rng = np.random.default_rng(5)
+    N, C, M = 24, 2, 300
+    coord = (rng.uniform(-0.5, 0.5, (M, 2)) * N).astype(np.float32)
+    coord[0] = [-N / 2, -N / 2]
+    x = demo.complex_gaussian(rng, (C, N, N))
+    v = demo.complex_gaussian(rng, (C, M))
+    ct = torch.from_numpy(coord).requires_grad_()
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), ct, width=6, oversamp=2.0, device=cuda_device)
+    y.backward(gradient=torch.from_numpy(v).to(cuda_device).reshape(y.shape))
+    # exact NUDFT y_j = sum_r x_r exp(-2 pi i k_j . r / N), differentiated by torch in FP64
+    kt = torch.from_numpy(coord.astype(np.float64)).requires_grad_()
+    r = torch.arange(N, dtype=torch.float64) - N // 2
+    ph = kt[:, 0, None, None] * r[None, None, :] / N + kt[:, 1, None, None] * r[None, :, None] / N
+    E = torch.exp(-2j * math.pi * ph)
+    ye = (E[None] * torch.from_numpy(x.astype(np.complex128))[:, None]).flatten(2).sum(-1)
+    ye.backward(gradient=torch.from_numpy(v.astype(np.complex128)))
+    assert rel(ct.grad.numpy(), kt.grad.numpy()) < 1e-3

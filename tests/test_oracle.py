@@ -285,3 +285,91 @@ def test_coord_grad_vs_autograd_nudft(shape):
     (torch.tensor(u).conj() * xa).real.sum().backward()
     got = R.coord_grad(xa.grad.numpy()[:, None], P, v[:, None], adjoint=True)
     assert rel(got, kt.grad.numpy()) < 3e-4
+
+
+def test_coord_grad_vs_central_differences():
+    """Exact derivative of the discrete forward operator against FP64
+    central differences (step 1e-4 grid units) of the restated operator
+    itself; a sample whose tap window would move within the step is excluded
+    (the operator jumps there by KB(1)/scale)."""
+    rng = np.random.default_rng(12)
+    shape, M, B, h, W = (12, 10, 14), 40, 2, 1e-4, 4
+    k = rng.uniform(-0.5, 0.5, (M, 3)) * np.array(shape[::-1])
+    P = R.Plan(k[None, None], shape, width=W, oversamp=2.0, precise=True)
+    x = demo.complex_gaussian(rng, (B, 1) + shape).astype(np.complex128)
+    v = demo.complex_gaussian(rng, (B, 1, M)).astype(np.complex128)
+    got = R.coord_grad(x, P, v)
+    fd = np.zeros((M, 3))
+    for a in range(3):
+        kp, km = k.copy(), k.copy()
+        kp[:, a] += h
+        km[:, a] -= h
+        # samples are independent: shifting every sample at once gives each
+        # sample's own derivative
+        yp = R.nufft(x, R.Plan(kp[None, None], shape, width=W, oversamp=2.0, precise=True))
+        ym = R.nufft(x, R.Plan(km[None, None], shape, width=W, oversamp=2.0, precise=True))
+        fd[:, a] = np.real(np.conj(v) * (yp - ym)).sum(axis=0).reshape(-1) / (2 * h)
+    c = P.coord_os.reshape(-1, 3)
+    step = h * np.array(P.os_shape[::-1]) / np.array(shape[::-1])
+    ok = np.all(np.ceil(c + step - W / 2) == np.ceil(c - step - W / 2), axis=1)
+    assert ok.sum() >= M - 3
+    assert rel(got[ok], fd[ok]) < 1e-7
+
+
+def test_coord_grad_adjoint_vs_central_differences():
+    """Adjoint operator: d/dk_j of L = Re<u, nufft_adj(v)> by FP64 central
+    differences, one sample moved at a time."""
+    rng = np.random.default_rng(13)
+    shape, M, B, h, W = (10, 12), 24, 2, 1e-4, 5
+    k = rng.uniform(-0.5, 0.5, (M, 2)) * np.array(shape[::-1])
+    P = R.Plan(k[None, None], shape, width=W, oversamp=1.5, precise=True)
+    u = demo.complex_gaussian(rng, (B, 1) + shape).astype(np.complex128)
+    v = demo.complex_gaussian(rng, (B, 1, M)).astype(np.complex128)
+    got = R.coord_grad(u, P, v, adjoint=True)
+
+    def loss(kk):
+        Pk = R.Plan(kk[None, None], shape, width=W, oversamp=1.5, precise=True)
+        return float(np.real(np.sum(np.conj(u) * R.nufft_adj(v, Pk))))
+
+    c = P.coord_os.reshape(-1, 2)
+    step = h * np.array(P.os_shape[::-1]) / np.array(shape[::-1])
+    checked = 0
+    for j in range(0, M, 3):
+        if np.any(np.ceil(c[j] + step - W / 2) != np.ceil(c[j] - step - W / 2)):
+            continue
+        for a in range(2):
+            kp, km = k.copy(), k.copy()
+            kp[j, a] += h
+            km[j, a] -= h
+            fd = (loss(kp) - loss(km)) / (2 * h)
+            assert fd == pytest.approx(got[j, a], rel=1e-6, abs=1e-9 * np.abs(got).max())
+            checked += 1
+    assert checked >= 10
+
+
+# ---- Pipe-Menon density compensation ---------------------------------------------
+def test_pipe_menon_cartesian_fixed_point():
+    """A full Cartesian grid makes the ortho NUFFT a unitary DFT: G G^H = I,
+    so w = 1 is the fixed point."""
+    g = np.arange(16) - 8
+    cc = np.stack(np.meshgrid(g, g, indexing="xy"), -1).reshape(1, 256, 2).astype(np.float32)
+    w = R.pipe_menon_dcf(cc, 16, niter=4)
+    assert np.max(np.abs(w - 1.0)) < 1e-9
+
+
+@pytest.mark.parametrize("niter", [1, 2])
+def test_pipe_menon_vs_exact_operator(niter):
+    """The iteration with the FP64 NUDFT in place of the NUFFT (ortho scaling
+    1/sqrt(G) on each side), accurate to the width-6 kernel error."""
+    N = 24
+    coord = demo.golden_angle_radial_2d(20, 48)
+    P = R.Plan(coord, N, width=6, oversamp=1.5)
+    cg = P.coord.reshape(-1, 2)
+    G = float(np.prod(P.os_shape))
+    w = np.ones((1, cg.shape[0]))
+    for _ in range(niter):
+        x = U.nudft_adj(w.astype(np.complex128), cg, [N, N]).real / math.sqrt(G)
+        y = U.nudft(x.astype(np.complex128), cg, [N, N]).real / math.sqrt(G)
+        w = w / np.abs(y)
+    got = R.pipe_menon_dcf(coord, N, niter=niter)
+    assert rel(got.ravel(), w.ravel()) < 2e-3
