@@ -99,6 +99,23 @@ int pf_plan_create(const float* coord, int ndim, const int* shape, const int* wi
                    const float* oversamp, int nframes, long long npts, void* stream,
                    pf_plan** out);
 
+/* pf_plan_create with the sort/spread tile chosen by the caller (an
+ * autotuner) instead of the built-in default; same reference interface as
+ * pf_plan_create, whose plan_nufft has no such knob.
+ * tile:  host array of ndim extents (z, y, x order) or NULL for the default;
+ *        the register-resident spread runs for the tiles listed in
+ *        pf_spread_tiles, any other tile uses the LDS-staged spread.
+ * chunk: spread-list entries per work item before a tile is split into
+ *        chunks that a reduce pass sums in order (0: default 1024). */
+int pf_plan_create_tiled(const float* coord, int ndim, const int* shape, const int* width,
+                         const float* oversamp, int nframes, long long npts, const int* tile,
+                         int chunk, void* stream, pf_plan** out);
+
+/* Tiles (z, y, x) with a register-resident spread for ndim: writes up to
+ * `cap` of them as 3 ints each into the host array `tiles`, sets *count to
+ * the number available.  No reference counterpart (tuning aid). */
+int pf_spread_tiles(int ndim, int* tiles, int cap, int* count);
+
 /* Release a plan and its device buffers (src/pyfourier/_nufft/_plan.py:203-225). */
 int pf_plan_destroy(pf_plan* plan);
 

@@ -8,7 +8,8 @@ INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
     pipe_menon_dcf                    iterative density compensation
 
 Beyond the reference: plan_nufft_batch / NUFFTBatch run an uneven list of
-problems (own matrix sizes and trajectories) as one HIP-graph replay.
+problems (own matrix sizes and trajectories) as one HIP-graph replay;
+autotune_plan picks a plan's sort/spread tile by timing the candidates.
 
 The compute path is libpfhip.so: hand-written HIP kernels for gfx950 behind a
 C ABI (include/pfhip.h), called through ctypes on PyTorch-ROCm streams.
@@ -22,6 +23,7 @@ from ._b0 import mri_exp_approx
 from ._sparsegram import GramMatrix, plan_spgram, sparse_gram
 from ._batch import NUFFTBatch, plan_nufft_batch
 from ._dcomp import pipe_menon_dcf
+from ._tune import autotune_plan, spread_tiles
 
 __all__ = [
     "PFError",
@@ -37,6 +39,8 @@ __all__ = [
     "NUFFTBatch",
     "plan_nufft_batch",
     "pipe_menon_dcf",
+    "autotune_plan",
+    "spread_tiles",
 ]
 
 __version__ = "0.1.0"

@@ -45,7 +45,8 @@ def resolve_device(device) -> torch.device:
 class NativePlan:
     """Owns one pf_plan on one device."""
 
-    def __init__(self, coord_grid: torch.Tensor, nframes: int, npts: int, shape, width, oversamp, device):
+    def __init__(self, coord_grid: torch.Tensor, nframes: int, npts: int, shape, width, oversamp, device,
+                 tile=None, chunk=0):
         self.device = device
         d = len(shape)
         c = coord_grid.detach().to(device=device, dtype=torch.float32).reshape(-1, d).contiguous()
@@ -56,8 +57,10 @@ class NativePlan:
         handle = ctypes.c_void_p()
         with torch.cuda.device(device):
             _lib.check(
-                L.pf_plan_create(
-                    _lib.ptr(c), d, sh, wi, ov, int(nframes), int(npts), _lib.stream_of(device), ctypes.byref(handle)
+                L.pf_plan_create_tiled(
+                    _lib.ptr(c), d, sh, wi, ov, int(nframes), int(npts),
+                    None if tile is None else (ctypes.c_int * d)(*[int(t) for t in tile]), int(chunk),
+                    _lib.stream_of(device), ctypes.byref(handle),
                 )
             )
         self.handle = handle
@@ -107,6 +110,8 @@ class Interpolator:
     dshape: tuple  # (nviews, nsamples)
     nframes: int
     unit_scale: tuple = ()  # d(grid units)/d(input units) per coord axis (x, y, z)
+    tile: tuple = None  # sort/spread tile (z, y, x); None: the engine's default
+    chunk: int = 0  # spread-list entries per work item (0: default)
     _native: dict = field(default_factory=dict, repr=False)
 
     @property
@@ -117,7 +122,8 @@ class Interpolator:
         key = (device.type, device.index)
         if key not in self._native:
             self._native[key] = NativePlan(
-                self.coord, self.nframes, self.npts, self.shape, self.width, self.oversamp, device
+                self.coord, self.nframes, self.npts, self.shape, self.width, self.oversamp, device, self.tile,
+                self.chunk,
             )
         return self._native[key]
 
@@ -202,11 +208,15 @@ def plan_nufft(
     dt=None,
     T=None,
     L_batch_size=None,
+    tile=None,
+    chunk=0,
 ):
     """Precompute a NUFFT plan (same signature as the reference plan_nufft).
 
     coord: (ncontrasts, nviews, nsamples, ndim) in any unit, last axis (x, y[, z]);
     shape: image matrix (scalar or (z, y, x)); width/oversamp: scalar or per axis (z, y, x).
+    tile, chunk (extensions): the sort/spread tile (z, y, x) and the spread
+    work-item size, e.g. from pyfourier_amd.autotune_plan; None/0 = defaults.
     """
     coord = torch.as_tensor(coord)
     if torch.is_complex(coord):
@@ -241,6 +251,8 @@ def plan_nufft(
         dshape=tuple(int(s) for s in coord.shape[1:-1]),
         nframes=int(coord.shape[0]),
         unit_scale=units,
+        tile=None if tile is None else tuple(int(t) for t in tile),
+        chunk=int(chunk),
     )
     if zmap is not None:
         if T is None:

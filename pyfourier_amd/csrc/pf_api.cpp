@@ -8,10 +8,6 @@
 #include "../../include/pfhip.h"
 #include "pf_internal.h"
 
-int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_zyx,
-                   const float* oversamp_zyx, int nframes, int64_t npts);
-int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream);
-void pf_plan_free(pf_plan* p);
 
 static thread_local std::string g_err;
 
@@ -123,12 +119,17 @@ int pf_version(void) { return 100; }
 
 int pf_plan_create(const float* coord, int ndim, const int* shape, const int* width, const float* oversamp,
                    int nframes, long long npts, void* stream, pf_plan** out) {
+    return pf_plan_create_tiled(coord, ndim, shape, width, oversamp, nframes, npts, nullptr, 0, stream, out);
+}
+
+int pf_plan_create_tiled(const float* coord, int ndim, const int* shape, const int* width, const float* oversamp,
+                         int nframes, long long npts, const int* tile, int chunk, void* stream, pf_plan** out) {
     if (!out) return pf_fail(PF_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (!coord || !shape || !width || !oversamp) return pf_fail(PF_ERR_ARG, "null argument");
     pf_plan* p = new (std::nothrow) pf_plan();
     if (!p) return pf_fail(PF_ERR_ALLOC, "host allocation failed");
-    int rc = pf_plan_params(p, ndim, shape, width, oversamp, nframes, (int64_t)npts);
+    int rc = pf_plan_params(p, ndim, shape, width, oversamp, nframes, (int64_t)npts, tile, chunk);
     if (rc != PF_OK) {
         pf_plan_free(p);
         return rc;
@@ -147,6 +148,13 @@ int pf_plan_create(const float* coord, int ndim, const int* shape, const int* wi
         return rc;
     }
     *out = p;
+    return PF_OK;
+}
+
+int pf_spread_tiles(int ndim, int* tiles, int cap, int* count) {
+    if (ndim < 1 || ndim > 3) return pf_fail(PF_ERR_ARG, "ndim must be 1, 2 or 3");
+    if (!count || cap < 0 || (cap > 0 && !tiles)) return pf_fail(PF_ERR_ARG, "null argument");
+    *count = pf_spread_reg_tiles(ndim, tiles, cap);
     return PF_OK;
 }
 

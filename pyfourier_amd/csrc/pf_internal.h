@@ -159,8 +159,11 @@ __device__ __forceinline__ unsigned pf_xcd_remap(unsigned b, unsigned nb) {
 
 // plan helpers shared by the NUFFT and Toeplitz objects
 int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_zyx,
-                   const float* oversamp_zyx, int nframes, int64_t npts);
+                   const float* oversamp_zyx, int nframes, int64_t npts, const int* tile_zyx, int chunk);
 int pf_plan_upload_apod(pf_plan* p);
 void pf_plan_free(pf_plan* p);
+int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream);
+// register-spread tiles for ndim (3 ints each, z y x); returns how many exist
+int pf_spread_reg_tiles(int ndim, int* tiles, int cap);
 int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* grid, const float2* h, float* out,
                             hipStream_t s);

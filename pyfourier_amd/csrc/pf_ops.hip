@@ -743,23 +743,46 @@ int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, fl
     return PF_OK;
 }
 
-// Register spread tiles (z, y, x) per dimensionality; the plan tiles match.
-static bool spread_reg_ok(const pf_plan* p, const pf_batch& b, int* tz, int* ty, int* txl) {
-    const int V = b.Cb * b.R * b.Cv;
-    int z = 1, y = 1, xl = 32;
-    if (p->ndim == 3) {
-        z = 4; y = 4; xl = 4;
-    } else if (p->ndim == 2) {
-        z = 1; y = 8; xl = 8;
-    }
-    if (tz) *tz = z;
-    if (ty) *ty = y;
-    if (txl) *txl = xl;
-    return V >= 16 && p->tile[0] <= z && p->tile[1] <= y && p->tile[2] <= 2 * xl;
+// Register-spread instances: tile (z, y, x) held by one wave, x split in
+// two halves of TXL = x / 2 points.  64 complex accumulators per lane each.
+using RegLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchParams&, const pf_slist&, const pf_plan*,
+                           const float2*, float2*, float2*);
+
+template <int TZ, int TY, int TXL>
+void launch_reg(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_slist& L,
+                const pf_plan* p, const float2* ys, float2* out, float2* partial) {
+    hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist, p->taprec,
+                       p->perm, ys, out, partial);
 }
 
+struct RegTile {
+    int ndim, z, y, x;
+    RegLaunch fn;
+};
+
+const RegTile REG_TILES[] = {
+    {3, 4, 4, 8, &launch_reg<4, 4, 4>},   {3, 4, 8, 4, &launch_reg<4, 8, 2>},
+    {3, 8, 4, 4, &launch_reg<8, 4, 2>},   {3, 2, 4, 16, &launch_reg<2, 4, 8>},
+    {2, 1, 8, 16, &launch_reg<1, 8, 8>},  {2, 1, 16, 8, &launch_reg<1, 16, 4>},
+    {2, 1, 4, 32, &launch_reg<1, 4, 16>}, {2, 1, 32, 4, &launch_reg<1, 32, 2>},
+    {1, 1, 1, 64, &launch_reg<1, 1, 32>},
+};
+
+// The smallest register instance that holds the plan's tile, or null.
+static const RegTile* spread_reg_for(const pf_plan* p, const pf_batch& b) {
+    if (b.Cb * b.R * b.Cv < 16) return nullptr;
+    const RegTile* best = nullptr;
+    for (const RegTile& t : REG_TILES) {
+        if (t.ndim != p->ndim || p->tile[0] > t.z || p->tile[1] > t.y || p->tile[2] > t.x) continue;
+        if (!best || t.z * t.y * t.x < best->z * best->y * best->x) best = &t;
+    }
+    return best;
+}
+
+static bool spread_reg_ok(const pf_plan* p, const pf_batch& b) { return spread_reg_for(p, b) != nullptr; }
+
 int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b) {
-    if (spread_reg_ok(p, b, nullptr, nullptr, nullptr)) return SPREAD_VL;
+    if (spread_reg_ok(p, b)) return SPREAD_VL;
     const int tpts = p->tile[0] * p->tile[1] * p->tile[2];
     const int V = b.Cb * b.R * b.Cv;
     // accumulator (2*VC+1 words per point) within the LDS budget, multiple of 4
@@ -784,6 +807,20 @@ size_t pf_spread_partial_bytes(const pf_plan* p, const pf_batch& b) {
     return (size_t)pf_slist_for(p, b).nslots * nvch * tpts * VC * sizeof(float2);
 }
 
+int pf_spread_reg_tiles(int ndim, int* tiles, int cap) {
+    int n = 0;
+    for (const RegTile& t : REG_TILES) {
+        if (t.ndim != ndim) continue;
+        if (n < cap && tiles) {
+            tiles[3 * n] = t.z;
+            tiles[3 * n + 1] = t.y;
+            tiles[3 * n + 2] = t.x;
+        }
+        ++n;
+    }
+    return n;
+}
+
 int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, float2* grid, float2* partial,
                      hipStream_t s) {
     GeoParams g = geo(p);
@@ -797,17 +834,8 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     if (L.nslots > 0 && !partial) return pf_fail(PF_ERR_ARG, "spread needs a partial-tile workspace");
     const int64_t nb = (int64_t)L.nsitems * nvch;
     if (nb > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many spread blocks");
-    int TZ, TY, TXL;
-    if (spread_reg_ok(p, b, &TZ, &TY, &TXL)) {
-        if (p->ndim == 3)
-            hipLaunchKernelGGL((k_spread_reg<4, 4, 4>), dim3((unsigned)nb), dim3(64), 0, s, g, B, L.sitems, L.slist,
-                               p->taprec, p->perm, ys, grid, partial);
-        else if (p->ndim == 2)
-            hipLaunchKernelGGL((k_spread_reg<1, 8, 8>), dim3((unsigned)nb), dim3(64), 0, s, g, B, L.sitems, L.slist,
-                               p->taprec, p->perm, ys, grid, partial);
-        else
-            hipLaunchKernelGGL((k_spread_reg<1, 1, 32>), dim3((unsigned)nb), dim3(64), 0, s, g, B, L.sitems,
-                               L.slist, p->taprec, p->perm, ys, grid, partial);
+    if (const RegTile* rt = spread_reg_for(p, b)) {
+        rt->fn(dim3((unsigned)nb), s, g, B, L, p, ys, grid, partial);
         PF_HIP(hipGetLastError());
         if (L.nritems > 0) {
             hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch)), dim3(TB), 0, s, g, B,

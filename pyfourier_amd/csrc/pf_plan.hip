@@ -217,7 +217,7 @@ double pf_kb_value_host(double x, double beta) { return kb_value(x, beta); }
 
 // Fill the scalar plan parameters; coordinates are in grid units.
 int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_zyx,
-                   const float* oversamp_zyx, int nframes, int64_t npts) {
+                   const float* oversamp_zyx, int nframes, int64_t npts, const int* tile_zyx, int chunk) {
     if (ndim < 1 || ndim > 3) return pf_fail(PF_ERR_ARG, "ndim must be 1, 2 or 3");
     if (nframes < 1 || npts < 1) return pf_fail(PF_ERR_ARG, "empty trajectory");
     if ((int64_t)nframes * npts >= (1ll << 31))
@@ -264,7 +264,17 @@ int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_
     // spread/sort tiles: the register-resident spread holds one tile for
     // 32 batch values in one wave (k_spread_reg)
     int t3[3] = {4, 4, 8}, t2[3] = {1, 8, 16}, t1[3] = {1, 1, 64};
+    int tu[3] = {1, 1, 1};
     const int* tt = ndim == 3 ? t3 : (ndim == 2 ? t2 : t1);
+    if (tile_zyx) {
+        // caller-chosen tile (ndim entries, z, y, x order), e.g. by an autotuner
+        for (int a = 3 - ndim; a < 3; ++a) {
+            const int t = tile_zyx[a - (3 - ndim)];
+            if (t < 1 || t > 4096) return pf_fail(PF_ERR_ARG, "tile extents must be in [1, 4096]");
+            tu[a] = t;
+        }
+        tt = tu;
+    }
     p->ntile_tot = 1;
     int off = 4;
     for (int a = 0; a < 3; ++a) {
@@ -280,7 +290,8 @@ int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_
     if ((int64_t)nframes * p->ntile_tot >= (1ll << 31))
         return pf_fail(PF_ERR_UNSUPPORTED, "too many tiles");
     p->rec = (off + 3) & ~3;
-    p->max_chunk = 1024;
+    if (chunk < 0) return pf_fail(PF_ERR_ARG, "chunk must be >= 0 (0: default)");
+    p->max_chunk = chunk > 0 ? chunk : 1024;
     return PF_OK;
 }
 

@@ -109,7 +109,7 @@ int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg,
     t->geo = new (std::nothrow) pf_plan();
     int w[3] = {1, 1, 1};
     float s[3] = {2.0f, 2.0f, 2.0f};
-    int rc = t->geo ? pf_plan_params(t->geo, ndim, shape, w, s, 1, 1) : pf_fail(PF_ERR_ALLOC, "host allocation");
+    int rc = t->geo ? pf_plan_params(t->geo, ndim, shape, w, s, 1, 1, nullptr, 0) : pf_fail(PF_ERR_ALLOC, "host allocation");
     if (rc == PF_OK && !pf_fftk_supported(t->geo))
         rc = pf_fail(PF_ERR_UNSUPPORTED, "Toeplitz embedding grid must factor into 2, 3 and 5");
     if (rc == PF_OK) rc = pf_plan_upload_apod(t->geo);

@@ -143,3 +143,15 @@ def test_field_map_basis_host():
     approx = b.numpy().astype(np.complex128).T @ ct.numpy().reshape(8, -1)
     exact = np.exp(-np.outer(t, 2j * math.pi * zmap.ravel()))
     assert np.linalg.norm(approx - exact) / np.linalg.norm(exact) < 5e-3
+
+
+def test_spread_tile_listing_host_only():
+    """pf_spread_tiles is host-only: it lists the register-spread tiles."""
+    import pyfourier_amd as pf
+
+    assert pf.spread_tiles(3)[0] == (4, 4, 8)
+    assert all(len(t) == 2 for t in pf.spread_tiles(2))
+    assert pf.spread_tiles(1) == [(64,)]
+    L = _lib.lib()
+    n = ctypes.c_int()
+    assert L.pf_spread_tiles(4, None, 0, ctypes.byref(n)) == 1  # PF_ERR_ARG
