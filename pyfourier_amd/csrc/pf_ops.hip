@@ -486,6 +486,10 @@ __global__ void __launch_bounds__(TB) k_spread_reduce(GeoParams g, BatchParams B
 constexpr int SPREAD_VL = 32;
 constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 
+#ifndef PF_SPREAD_UNROLL
+#define PF_SPREAD_UNROLL 1
+#endif
+
 template <int TZ, int TY, int TXL>
 __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, const int4* __restrict__ items,
                                                    const int32_t* __restrict__ slist,
@@ -495,9 +499,14 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
                                                    float2* __restrict__ partial) {
     constexpr int VL = SPREAD_VL;
     constexpr int EB = SPREAD_EB;
+    constexpr int TX = 2 * TXL;
+    constexpr int NW = TZ + TY + TX;  // tile-local tap weights of one entry
+    static_assert(TZ <= 32 && TY <= 32, "row masks hold 32 bits per axis");
     __shared__ int s_idx[EB];
-    __shared__ int s_off[EB][4];               // tile-relative tap offsets (z, y, x), frame
-    __shared__ float s_w[EB][3 * PF_WMAX];     // tap weights (z, y, x)
+    __shared__ int s_frame[EB];
+    __shared__ unsigned s_zmask[EB];           // tile z rows inside the footprint
+    __shared__ unsigned s_ymask[EB];           // tile y rows inside the footprint
+    __shared__ float s_w[EB][NW];              // weights at the tile's z, y, x positions (0 outside)
     __shared__ float2 s_val[EB][VL];           // values times conj(coefficient)
 
     const int RC = B.R * B.Cv;
@@ -525,7 +534,6 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
     const int oz = tiz * g.tile[0], oy = tiy * g.tile[1], ox = tix * g.tile[2];
     const int ez = min(g.tile[0], g.os[0] - oz), ey = min(g.tile[1], g.os[1] - oy);
     const int ex = min(g.tile[2], g.os[2] - ox);
-    const int Wz = g.width[0], Wy = g.width[1], Wx = g.width[2];
 
     float2 acc[TZ][TY][TXL];
 #pragma unroll
@@ -537,7 +545,7 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
 
     for (int e0 = it.y; e0 < it.z; e0 += EB) {
         const int n = min(EB, it.z - e0);
-        // phase 1: one entry per lane
+        // phase 1: one entry per lane -> its weights at this tile's positions
         if (lane < n) {
             const int s = slist[e0 + lane];
             const float* rec = taprec + (int64_t)s * g.rec;
@@ -549,14 +557,33 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
             by += by < 0 ? g.os[1] : 0;
             int bx = ox - ri[2];
             bx += bx < 0 ? g.os[2] : 0;
+            unsigned mz = 0, my = 0;
+#pragma unroll
+            for (int a = 0; a < TZ; ++a) {
+                int i = bz + a;
+                i -= i >= g.os[0] ? g.os[0] : 0;
+                const bool in = i < g.width[0];
+                s_w[lane][a] = in ? rec[g.woff[0] + i] : 0.f;
+                mz |= in ? 1u << a : 0u;
+            }
+#pragma unroll
+            for (int b = 0; b < TY; ++b) {
+                int i = by + b;
+                i -= i >= g.os[1] ? g.os[1] : 0;
+                const bool in = i < g.width[1];
+                s_w[lane][TZ + b] = in ? rec[g.woff[1] + i] : 0.f;
+                my |= in ? 1u << b : 0u;
+            }
+#pragma unroll
+            for (int x = 0; x < TX; ++x) {
+                int i = bx + x;
+                i -= i >= g.os[2] ? g.os[2] : 0;
+                s_w[lane][TZ + TY + x] = i < g.width[2] ? rec[g.woff[2] + i] : 0.f;
+            }
             s_idx[lane] = s;
-            s_off[lane][0] = bz;
-            s_off[lane][1] = by;
-            s_off[lane][2] = bx;
-            s_off[lane][3] = ri[3];
-            for (int k = 0; k < Wz; ++k) s_w[lane][k] = rec[g.woff[0] + k];
-            for (int k = 0; k < Wy; ++k) s_w[lane][PF_WMAX + k] = rec[g.woff[1] + k];
-            for (int k = 0; k < Wx; ++k) s_w[lane][2 * PF_WMAX + k] = rec[g.woff[2] + k];
+            s_frame[lane] = ri[3];
+            s_zmask[lane] = mz;
+            s_ymask[lane] = my;
         }
         __syncthreads();
         // phase 2: the batch's values, lane (value, half) takes entries half, half+2, ...
@@ -565,35 +592,30 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
             float2 val = vok ? ys[(int64_t)s * B.C + ycol] : make_float2(0.f, 0.f);
             if (B.basis || B.b0t) {
                 const int64_t tread = B.b0t ? ((int64_t)perm[s] % g.npts) % B.nt : 0;
-                val = cmulc(val, sample_coef(B, r, s_off[j][3], tread));
+                val = cmulc(val, sample_coef(B, r, s_frame[j], tread));
             }
             s_val[j][v] = val;
         }
         __syncthreads();
-        // phase 3: accumulate
+        // phase 3: accumulate; rows outside the footprint are skipped by
+        // scalar tests of the entry's row masks
+#pragma unroll PF_SPREAD_UNROLL
         for (int j = 0; j < n; ++j) {
-            const int bz = s_off[j][0], by = s_off[j][1], bx = s_off[j][2];
+            const unsigned mz = __builtin_amdgcn_readfirstlane(s_zmask[j]);
+            const unsigned my = __builtin_amdgcn_readfirstlane(s_ymask[j]);
             const float* w = s_w[j];
             float wxl[TXL];
 #pragma unroll
-            for (int k = 0; k < TXL; ++k) {
-                int i = bx + half * TXL + k;
-                i -= i >= g.os[2] ? g.os[2] : 0;
-                wxl[k] = i < Wx ? w[2 * PF_WMAX + i] : 0.f;
-            }
+            for (int k = 0; k < TXL; ++k) wxl[k] = w[TZ + TY + half * TXL + k];
             const float2 val = s_val[j][v];
 #pragma unroll
             for (int a = 0; a < TZ; ++a) {
-                int iz = bz + a;
-                iz -= iz >= g.os[0] ? g.os[0] : 0;
-                if (iz >= Wz) continue;
-                const float wz = w[iz];
+                if (!((mz >> a) & 1u)) continue;
+                const float wz = w[a];
 #pragma unroll
                 for (int b = 0; b < TY; ++b) {
-                    int iy = by + b;
-                    iy -= iy >= g.os[1] ? g.os[1] : 0;
-                    if (iy >= Wy) continue;
-                    const float wzy = wz * w[PF_WMAX + iy];
+                    if (!((my >> b) & 1u)) continue;
+                    const float wzy = wz * w[TZ + b];
                     const float vx = wzy * val.x, vy = wzy * val.y;
 #pragma unroll
                     for (int k = 0; k < TXL; ++k) {
