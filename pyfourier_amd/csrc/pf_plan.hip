@@ -53,7 +53,23 @@ struct TapParams {
     int64_t ntile_tot;
     int rec;
     int woff[3];
+    int mbits[3];  // Morton bits per axis (0 on all axes: row-major tile order)
 };
+
+// Interleave the bits of (tz, ty, tx), x fastest: consecutive codes are
+// spatially compact blocks of tiles (Z-order), which keeps the halo of the
+// tiles in flight within L2.
+__host__ __device__ inline uint32_t morton3(uint32_t tz, uint32_t ty, uint32_t tx, const int* mb) {
+    uint32_t code = 0;
+    int o = 0;
+    const int mx = mb[0] > mb[1] ? (mb[0] > mb[2] ? mb[0] : mb[2]) : (mb[1] > mb[2] ? mb[1] : mb[2]);
+    for (int b = 0; b < mx; ++b) {
+        if (b < mb[2]) code |= ((tx >> b) & 1u) << o++;
+        if (b < mb[1]) code |= ((ty >> b) & 1u) << o++;
+        if (b < mb[0]) code |= ((tz >> b) & 1u) << o++;
+    }
+    return code;
+}
 
 __device__ __forceinline__ int64_t pmod(int64_t a, int64_t m) {
     int64_t r = a % m;
@@ -114,8 +130,12 @@ __global__ void k_taps(const float* __restrict__ coord, TapParams P, uint32_t* _
         tcoord[a] = st / P.tile[a];
     }
     ri[3] = (int)f;
-    keys[i] = (uint32_t)(f * P.ntile_tot +
-                         (tcoord[0] * P.ntiles[1] + tcoord[1]) * P.ntiles[2] + tcoord[2]);
+    const int msum = P.mbits[0] + P.mbits[1] + P.mbits[2];
+    if (msum > 0)
+        keys[i] = ((uint32_t)f << msum) | morton3((uint32_t)tcoord[0], (uint32_t)tcoord[1], (uint32_t)tcoord[2], P.mbits);
+    else
+        keys[i] = (uint32_t)(f * P.ntile_tot +
+                             (tcoord[0] * P.ntiles[1] + tcoord[1]) * P.ntiles[2] + tcoord[2]);
     ids[i] = (int32_t)i;
 }
 
@@ -353,7 +373,23 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
     const int CH = p->max_chunk;
     std::vector<int4> items, ritems;
     int slot = 0;
-    for (uint64_t k = 0; k < nkeys; ++k) {
+    // visit tiles in the sample order's Z-order (when it is used), frame by frame
+    std::vector<uint64_t> order(nkeys);
+    {
+        const int msum = P.mbits[0] + P.mbits[1] + P.mbits[2];
+        std::vector<std::pair<uint64_t, uint64_t>> tk(p->ntile_tot);
+        for (int64_t t = 0; t < p->ntile_tot; ++t) {
+            const uint32_t tx = (uint32_t)(t % p->ntiles[2]);
+            const uint32_t ty = (uint32_t)((t / p->ntiles[2]) % p->ntiles[1]);
+            const uint32_t tz = (uint32_t)(t / ((int64_t)p->ntiles[2] * p->ntiles[1]));
+            tk[t] = {msum > 0 ? morton3(tz, ty, tx, P.mbits) : (uint64_t)t, (uint64_t)t};
+        }
+        std::sort(tk.begin(), tk.end());
+        uint64_t q = 0;
+        for (uint64_t f = 0; f < nkeys / p->ntile_tot; ++f)
+            for (int64_t t = 0; t < p->ntile_tot; ++t) order[q++] = f * p->ntile_tot + tk[t].second;
+    }
+    for (uint64_t k : order) {
         int e0 = hs[k], e1 = he[k];
         int n = e1 - e0;
         if (n <= CH) {
@@ -397,6 +433,15 @@ int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
     P.mtot = p->mtot;
     P.ntile_tot = p->ntile_tot;
     P.rec = p->rec;
+    int msum = 0;
+    for (int a = 0; a < 3; ++a) {
+        P.mbits[a] = 0;
+        while ((1 << P.mbits[a]) < p->ntiles[a]) ++P.mbits[a];
+        msum += P.mbits[a];
+    }
+    // Z-order keys need frame << msum to fit 32 bits; else row-major order
+    if (msum == 0 || (uint64_t)p->nframes << msum >= (1ull << 32))
+        P.mbits[0] = P.mbits[1] = P.mbits[2] = 0, msum = 0;
     const int64_t M = p->mtot;
     const int TB = 256;
 
@@ -416,7 +461,7 @@ int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
                        ids.as<int32_t>(), rec0.as<float>(), cos0.as<float>());
     PF_HIP(hipGetLastError());
 
-    const uint64_t nkeys = (uint64_t)p->nframes * p->ntile_tot;
+    const uint64_t nkeys = msum > 0 ? (uint64_t)p->nframes << msum : (uint64_t)p->nframes * p->ntile_tot;
     unsigned nbits = bits_for(nkeys + 1);
     size_t tbytes = 0;
     PF_HIP(rocprim::radix_sort_pairs(nullptr, tbytes, keys.as<uint32_t>(), keys2.as<uint32_t>(),
