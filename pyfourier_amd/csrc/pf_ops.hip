@@ -241,6 +241,101 @@ __global__ void __launch_bounds__(TB) k_gather(GeoParams g, BatchParams B, int64
         dst[0] = out[0];
 }
 
+// Gather with compile-time y and x widths: the WY x WX taps of a z row are
+// fully unrolled (all their loads in flight together, tap indices and weights
+// in registers by constant index), z stays a loop whose weight is read from
+// the record.  In-slab offsets are 32-bit (the host checks G * R * Cv < 2^31).
+template <int CPL, int WY, int WX>
+__global__ void __launch_bounds__(TB) k_gather_w(GeoParams g, BatchParams B, int64_t mtot,
+                                                 const float* __restrict__ taprec,
+                                                 const int32_t* __restrict__ perm,
+                                                 const float2* __restrict__ grid, float2* __restrict__ ys) {
+    const int LS = B.C / CPL;  // lanes per sample
+    const unsigned bid = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t t = (int64_t)bid * blockDim.x + threadIdx.x;
+    const int64_t s = t / LS;
+    if (s >= mtot) return;
+    const int lane = (int)(t % LS);
+    const int c0 = lane * CPL;
+    const int cb = c0 / B.Cv, cv = c0 % B.Cv;
+    const float* rec = taprec + s * g.rec;
+    const int* ri = reinterpret_cast<const int*>(rec);
+    const int f = ri[3];
+    const int fg = B.nfg > 1 ? f : 0;
+    const int64_t tread = B.b0t ? ((int64_t)perm[s] % g.npts) % B.nt : 0;
+    const int RC = B.R * B.Cv;
+    int yo[WY], xo[WX];
+    float wy[WY], wx[WX];
+#pragma unroll
+    for (int k = 0; k < WY; ++k) {
+        int v = ri[1] + k;
+        v -= v >= g.os[1] ? g.os[1] : 0;
+        yo[k] = v * g.os[2] * RC;
+        wy[k] = rec[g.woff[1] + k];
+    }
+#pragma unroll
+    for (int k = 0; k < WX; ++k) {
+        int v = ri[2] + k;
+        v -= v >= g.os[2] ? g.os[2] : 0;
+        xo[k] = v * RC;
+        wx[k] = rec[g.woff[2] + k];
+    }
+    const int zstride = g.os[1] * g.os[2] * RC;
+    const float2* slab = grid + ((int64_t)fg * B.Cb + cb) * g.gtot * RC + cv;
+    float2 out[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) out[q] = make_float2(0.f, 0.f);
+    for (int r = 0; r < B.R; ++r) {
+        float2 acc[CPL];
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) acc[q] = make_float2(0.f, 0.f);
+        const float2* sr = slab + r * B.Cv;
+        for (int a = 0; a < g.width[0]; ++a) {
+            int z = ri[0] + a;
+            z -= z >= g.os[0] ? g.os[0] : 0;
+            const float wz = rec[g.woff[0] + a];
+            const float2* pz = sr + z * zstride;
+#pragma unroll
+            for (int b = 0; b < WY; ++b) {
+                const float wzy = wz * wy[b];
+                const float2* py = pz + yo[b];
+#pragma unroll
+                for (int c = 0; c < WX; ++c) {
+                    const float w = wzy * wx[c];
+                    if constexpr (CPL == 2) {
+                        const float4 v = *reinterpret_cast<const float4*>(py + xo[c]);
+                        acc[0].x += w * v.x;
+                        acc[0].y += w * v.y;
+                        acc[CPL - 1].x += w * v.z;
+                        acc[CPL - 1].y += w * v.w;
+                    } else {
+                        const float2 v = py[xo[c]];
+                        acc[0].x += w * v.x;
+                        acc[0].y += w * v.y;
+                    }
+                }
+            }
+        }
+        if (B.R == 1 && !B.basis && !B.b0t) {
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) out[q] = acc[q];
+        } else {
+            const float2 cf = sample_coef(B, r, f, tread);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const float2 m = cmul(acc[q], cf);
+                out[q].x += m.x;
+                out[q].y += m.y;
+            }
+        }
+    }
+    float2* dst = ys + s * B.C + c0;
+    if constexpr (CPL == 2)
+        *reinterpret_cast<float4*>(dst) = make_float4(out[0].x, out[0].y, out[CPL - 1].x, out[CPL - 1].y);
+    else
+        dst[0] = out[0];
+}
+
 // -------------------------------------------------------------------------
 // spread: one block per (tile work item, value chunk).  A value chunk is a
 // range of VC entries of the flat per-point vector (Cb, R, Cv); the waves of
@@ -749,9 +844,44 @@ int pf_launch_crop_deapod(const pf_plan* p, const pf_batch& b, const float2* gri
     return PF_OK;
 }
 
+using GatherLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchParams&, const pf_plan*, const float2*,
+                              float2*);
+
+template <int CPL, int WY, int WX>
+void launch_gather_w(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_plan* p,
+                     const float2* in, float2* ys) {
+    hipLaunchKernelGGL((k_gather_w<CPL, WY, WX>), grid, dim3(TB), 0, s, g, B, p->mtot, p->taprec, p->perm, in, ys);
+}
+
+template <int CPL, int W>
+constexpr GatherLaunch gw_sq() { return &launch_gather_w<CPL, W, W>; }
+
+// [cpl - 1][wy == 1 ? 0 : 1][wx - 1]
+const GatherLaunch GATHER_W[2][2][PF_WMAX] = {
+    {{&launch_gather_w<1, 1, 1>, &launch_gather_w<1, 1, 2>, &launch_gather_w<1, 1, 3>, &launch_gather_w<1, 1, 4>,
+      &launch_gather_w<1, 1, 5>, &launch_gather_w<1, 1, 6>, &launch_gather_w<1, 1, 7>, &launch_gather_w<1, 1, 8>},
+     {nullptr, gw_sq<1, 2>(), gw_sq<1, 3>(), gw_sq<1, 4>(), gw_sq<1, 5>(), gw_sq<1, 6>(), gw_sq<1, 7>(),
+      gw_sq<1, 8>()}},
+    {{&launch_gather_w<2, 1, 1>, &launch_gather_w<2, 1, 2>, &launch_gather_w<2, 1, 3>, &launch_gather_w<2, 1, 4>,
+      &launch_gather_w<2, 1, 5>, &launch_gather_w<2, 1, 6>, &launch_gather_w<2, 1, 7>, &launch_gather_w<2, 1, 8>},
+     {nullptr, gw_sq<2, 2>(), gw_sq<2, 3>(), gw_sq<2, 4>(), gw_sq<2, 5>(), gw_sq<2, 6>(), gw_sq<2, 7>(),
+      gw_sq<2, 8>()}},
+};
+
 int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, float2* ys, hipStream_t s) {
     GeoParams g = geo(p);
     BatchParams B = bparams(b);
+    const int cpl = b.Cv % 2 == 0 ? 2 : 1;
+    const int wy = p->width[1], wx = p->width[2];
+    const bool idx32 = (int64_t)pf_gtot(p) * b.R * b.Cv < (1ll << 31);
+    if (idx32 && (wy == 1 || wy == wx)) {
+        if (GatherLaunch fn = GATHER_W[cpl - 1][wy == 1 ? 0 : 1][wx - 1]) {
+            const int64_t threads = p->mtot * (b.C / cpl);
+            fn(dim3(nblk(threads, TB)), s, g, B, p, grid, ys);
+            PF_HIP(hipGetLastError());
+            return PF_OK;
+        }
+    }
     if (b.Cv % 2 == 0) {
         int64_t threads = p->mtot * (b.C / 2);
         hipLaunchKernelGGL(k_gather<2>, dim3(nblk(threads, TB)), dim3(TB), 0, s, g, B, p->mtot, p->taprec,
