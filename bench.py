@@ -221,7 +221,7 @@ def bench_uneven_batch(pf, dev, steps):
     batch = pf.plan_nufft_batch(coords, sizes, nbatch=C, width=4, oversamp=2.0, device=dev)
     for buf in batch.images:
         buf.copy_(torch.randn(buf.shape, dtype=torch.complex64, device=dev))
-    plans = [c.plan for c in batch.calls]
+    plans = batch.plans
     imgs = [b[:, 0] for b in batch.images]
 
     def plain():
@@ -235,8 +235,8 @@ def bench_uneven_batch(pf, dev, steps):
     t_plain = _time(plain, dev, steps)
     t_batch = _time(batched, dev, steps)
     return {"workload": "64 uneven 2D slices {128,192,256}^2 x 8 coils, radial 16-256 spokes x 256, W=4",
-            "samples": int(sum(spokes) * 256), "plain_ms": round(t_plain, 3), "batched_ms": round(t_batch, 3),
-            "speedup": round(t_plain / t_batch, 2)}
+            "samples": int(sum(spokes) * 256), "groups": len(batch.groups), "plain_ms": round(t_plain, 3),
+            "batched_ms": round(t_batch, 3), "speedup": round(t_plain / t_batch, 2)}
 
 
 def bench_subspace(pf, dev, steps):

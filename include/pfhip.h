@@ -111,6 +111,20 @@ int pf_plan_create_tiled(const float* coord, int ndim, const int* shape, const i
                          const float* oversamp, int nframes, long long npts, const int* tile,
                          int chunk, void* stream, pf_plan** out);
 
+/* A plan over frames of unequal length ("ragged"): frame f owns samples
+ * frame_offsets[f] .. frame_offsets[f+1]-1 of coord (host array of
+ * nframes + 1 entries, frame_offsets[0] = 0, non-decreasing).  Each frame is
+ * an independent problem of the same image geometry (its own image plane in
+ * the (batch, nframes, *shape) layout, its own grid), so an uneven set of
+ * slices or trajectories runs in one call; k-space is (batch, total) with the
+ * frames concatenated.  Field-map segments are not supported on such plans.
+ * No reference counterpart: the reference loops over slices in Python with
+ * one plan_nufft (src/pyfourier/_nufft/_plan.py:20-199) per slice.
+ * pf_plan_get_info reports npts = total samples. */
+int pf_plan_create_ragged(const float* coord, int ndim, const int* shape, const int* width,
+                          const float* oversamp, int nframes, const long long* frame_offsets,
+                          const int* tile, int chunk, void* stream, pf_plan** out);
+
 /* Tiles (z, y, x) with a register-resident spread for ndim: writes up to
  * `cap` of them as 3 ints each into the host array `tiles`, sets *count to
  * the number available.  No reference counterpart (tuning aid). */

@@ -1,30 +1,72 @@
-"""Uneven batches of independent NUFFT problems replayed as one HIP graph.
+"""Uneven batches of independent NUFFT problems in a few launches.
 
 A reconstruction or training loop over many slices, each with its own
-matrix size, trajectory and sample count, pays the per-call host cost of the
-drop-in API (argument checks, plan lookup, workspace allocation, one ctypes
-round trip per native call) once per slice.  NUFFTBatch plans every item
-once, owns static image and k-space buffers for all items, records every
-item's native launches into one torch.cuda.CUDAGraph (a hipGraph on ROCm)
-per direction and norm, and replays it: one host call for the whole uneven
-batch.  Each item keeps its own plan; the items run back to back on one
-stream inside the graph.
+trajectory and sample count, pays per slice both the host cost of the
+drop-in API and, worse, kernels too small to fill the GPU.  NUFFTBatch
+plans every item once and groups the items that share an image geometry
+(matrix, kernel width, oversampling): a group runs as ONE ragged plan whose
+frames are the items (pf_plan_create_ragged), so each kernel of the
+pipeline covers every sample of the group in one launch.  Items that cannot
+join a group (several frames, or a subspace basis) run as their own calls.
+All launches of a direction are recorded once into a torch.cuda.CUDAGraph
+(a hipGraph on ROCm) and replayed.
 
 No counterpart in the reference (which loops in Python); the operator of
-each item is exactly the one nufft / nufft_adj apply with the same plan.
+each item is exactly the one nufft / nufft_adj apply with that item's plan.
 """
 
 from __future__ import annotations
 
+import ctypes
+
+import numpy as np
 import torch
 
 from . import _lib
 from ._nufft import _Call, _adjoint_code, _norm_code, _to_torch
-from ._plan import plan_nufft, resolve_device
+from ._plan import NativePlan, plan_nufft, resolve_device
+
+
+class _Group:
+    """Items of one geometry as the frames of one ragged native plan."""
+
+    def __init__(self, plans, weights, nbatch, device, tile, chunk):
+        it0 = plans[0].interpolator
+        d = it0.ndim
+        coords = [p.interpolator.coord.reshape(-1, d) for p in plans]
+        counts = [int(c.shape[0]) for c in coords]
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.device = device
+        self.native = NativePlan(torch.cat(coords), len(plans), 0, it0.shape, it0.width, it0.oversamp, device,
+                                 tile, chunk, frame_offsets=off.tolist())
+        total = int(off[-1])
+        self.images = torch.zeros((nbatch, len(plans)) + tuple(it0.shape), dtype=torch.complex64, device=device)
+        self.kspace = torch.zeros((nbatch, total), dtype=torch.complex64, device=device)
+        self.weight = None
+        if any(w is not None for w in weights):
+            parts = [torch.ones(n, dtype=torch.float32, device=device) if w is None else w
+                     for w, n in zip(weights, counts)]
+            self.weight = torch.cat(parts).contiguous()
+        self.image_views = [self.images[:, k:k + 1] for k in range(len(plans))]
+        self.kspace_views = [self.kspace[:, int(off[k]):int(off[k + 1])].unsqueeze(1) for k in range(len(plans))]
+        self.nbatch = nbatch
+
+    def _run(self, adjoint, code):
+        L = _lib.lib()
+        desc = _lib.BatchDesc()
+        desc.batch, desc.nseg, desc.ncoeff, desc.nt = self.nbatch, 1, 0, 1
+        nbytes = ctypes.c_size_t()
+        _lib.check(L.pf_workspace_size(self.native.handle, ctypes.byref(desc), int(adjoint), ctypes.byref(nbytes)))
+        ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
+        fn = L.pf_nufft_adj if adjoint else L.pf_nufft
+        src, dst = (self.kspace, self.images) if adjoint else (self.images, self.kspace)
+        with torch.cuda.device(self.device):
+            _lib.check(fn(self.native.handle, ctypes.byref(desc), _lib.ptr(src), _lib.ptr(dst),
+                          _lib.ptr(self.weight), code, _lib.ptr(ws), nbytes.value, _lib.stream_of(self.device)))
 
 
 class NUFFTBatch:
-    """Forward / adjoint NUFFT of an uneven list of problems in one launch.
+    """Forward / adjoint NUFFT of an uneven list of problems.
 
     coords[i]: trajectory of item i, (ncontrasts, nviews, nsamples, ndim) or
     fewer leading axes, in any unit; shapes[i]: its image matrix.  Every
@@ -33,14 +75,14 @@ class NUFFTBatch:
     shared) is a (ncontrasts, ncoeff) subspace basis.
 
     ``images[i]`` (nbatch, nimg_i, *shape_i) and ``kspace[i]`` (nbatch,
-    ncontrasts_i, npts_i) are the static buffers the graphs read and write:
-    fill them in place (or pass lists to forward/adjoint, which copy with one
-    multi-tensor copy) and read the results from the other side.  A replay
-    overwrites the previous results.
+    ncontrasts_i, npts_i) are (views of) the static buffers the launches
+    read and write: fill them in place, or pass lists to forward/adjoint,
+    which copy with one multi-tensor copy.  A call overwrites the previous
+    results.  group=False runs every item as its own calls (one graph).
     """
 
     def __init__(self, coords, shapes, nbatch=1, width=4, oversamp=1.25, weights=None, basis=None, norm=None,
-                 device="cuda", graph=True):
+                 device="cuda", graph=True, group=True, tile=None, chunk=0):
         if len(coords) != len(shapes):
             raise ValueError("coords and shapes must have the same length")
         if len(coords) == 0:
@@ -48,33 +90,54 @@ class NUFFTBatch:
         self.device = resolve_device(device)
         self.norm_code = _norm_code(norm)
         self.use_graph = bool(graph)
-        self.calls = []
-        self.images = []
-        self.kspace = []
-        for i, (c, sh) in enumerate(zip(coords, shapes)):
-            plan = plan_nufft(_to_torch(c).to(torch.float32), sh, width, oversamp).to(self.device)
+        self.plans = [plan_nufft(_to_torch(c).detach().to(torch.float32), sh, width, oversamp, tile=tile, chunk=chunk)
+                      for c, sh in zip(coords, shapes)]
+        self.images = [None] * len(coords)
+        self.kspace = [None] * len(coords)
+        self.calls = {}
+        self.groups = []
+        wts = [None] * len(coords)
+        groups = {}
+        for i, plan in enumerate(self.plans):
+            it = plan.interpolator
             w = None if weights is None else weights[i]
-            call = _Call(plan, self.device, basis, w, norm)
-            if call.weight_full is not None:
-                raise ValueError("NUFFTBatch takes real density weights broadcastable to each item's k-space")
-            it = call.interp
-            self.calls.append(call)
-            self.images.append(torch.zeros((nbatch, call.nimg) + tuple(it.shape), dtype=torch.complex64,
-                                           device=self.device))
-            self.kspace.append(torch.zeros((nbatch, it.nframes, it.npts), dtype=torch.complex64,
-                                           device=self.device))
+            if w is not None:
+                w = _to_torch(w).to(self.device)
+                if torch.is_complex(w):
+                    raise ValueError("NUFFTBatch takes real density weights")
+                w = torch.broadcast_to(w.to(torch.float32), (it.nframes,) + tuple(it.dshape)).reshape(-1)
+                wts[i] = w.contiguous()
+            if group and basis is None and it.nframes == 1:
+                groups.setdefault((it.ndim, it.shape, it.width, it.oversamp), []).append(i)
+            else:
+                plan.to(self.device)
+                call = _Call(plan, self.device, basis, None, norm)
+                call.weight_flat = wts[i]
+                self.calls[i] = call
+                self.images[i] = torch.zeros((nbatch, call.nimg) + tuple(it.shape), dtype=torch.complex64,
+                                             device=self.device)
+                self.kspace[i] = torch.zeros((nbatch, it.nframes, it.npts), dtype=torch.complex64,
+                                             device=self.device)
+        for idx in groups.values():
+            g = _Group([self.plans[i] for i in idx], [wts[i] for i in idx], nbatch, self.device, tile, chunk)
+            self.groups.append(g)
+            for k, i in enumerate(idx):
+                self.images[i] = g.image_views[k]
+                self.kspace[i] = g.kspace_views[k]
         self._graphs = {}
 
     def __len__(self):
-        return len(self.calls)
+        return len(self.plans)
 
     # ---- one pass over all items, eager ----------------------------------
     def _launch(self, adjoint: bool, code: int):
-        for call, x, y in zip(self.calls, self.images, self.kspace):
+        for g in self.groups:
+            g._run(adjoint, code)
+        for i, call in self.calls.items():
             if adjoint:
-                call.adjoint(y, code, out=x)
+                call.adjoint(self.kspace[i], code, out=self.images[i])
             else:
-                call.forward(x, code, out=y)
+                call.forward(self.images[i], code, out=self.kspace[i])
 
     def _replay(self, adjoint: bool, code: int):
         if not self.use_graph:
@@ -129,6 +192,7 @@ class NUFFTBatch:
 
 
 def plan_nufft_batch(coords, shapes, nbatch=1, width=4, oversamp=1.25, weights=None, basis=None, norm=None,
-                     device="cuda", graph=True) -> NUFFTBatch:
+                     device="cuda", graph=True, group=True, tile=None, chunk=0) -> NUFFTBatch:
     """Plan an uneven batch of NUFFT problems (see NUFFTBatch)."""
-    return NUFFTBatch(coords, shapes, nbatch, width, oversamp, weights, basis, norm, device, graph)
+    return NUFFTBatch(coords, shapes, nbatch, width, oversamp, weights, basis, norm, device, graph, group, tile,
+                      chunk)

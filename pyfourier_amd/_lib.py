@@ -71,6 +71,11 @@ _SIG = {
          ctypes.POINTER(_I), _I, _VP, ctypes.POINTER(_VP)],
     ),
     "pf_spread_tiles": (_I, [_I, ctypes.POINTER(_I), _I, ctypes.POINTER(_I)]),
+    "pf_plan_create_ragged": (
+        _I,
+        [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float), _I,
+         ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I), _I, _VP, ctypes.POINTER(_VP)],
+    ),
     "pf_plan_destroy": (_I, [_VP]),
     "pf_plan_get_info": (_I, [_VP, ctypes.POINTER(PlanInfo)]),
     "pf_plan_get_permutation": (_I, [_VP, _VP, _VP]),

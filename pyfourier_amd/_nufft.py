@@ -169,32 +169,64 @@ class _Call:
             acc = y if acc is None else acc.add_(y)
         return acc
 
-    # ---- coordinate gradient ------------------------------------------------
-    def coord_grad(self, img: torch.Tensor, h: torch.Tensor, factor: float) -> torch.Tensor:
-        """Re(sum_c conj(h) * d(A img)/d coord) per sample, in user units.
-
-        img (C, nimg, *N) is the image whose forward grid is differentiated;
-        h (C, F, npts) the paired k-space (weight applied here); factor the
-        operator's 1/scale and norm factor.  Returns (F*npts, d) ordered (x, y, z).
-        """
+    # ---- forward that keeps its grids (for the trajectory gradient) ----------
+    def forward_keep(self, x: torch.Tensor, norm_code: int):
+        """forward(x) that also returns, per B0 segment chunk, the workspace
+        holding the oversampled k-space grid of x, so the trajectory gradient
+        does not transform x again."""
+        C = x.shape[0]
         it = self.interp
-        C = img.shape[0]
-        M = it.nframes * it.npts
+        out = torch.empty((C, it.nframes, it.npts), dtype=torch.complex64, device=self.device)
         L = _lib.lib()
-        acc = torch.zeros((M, 3), dtype=torch.float32, device=self.device)
+        kept, acc = [], None
+        fac = self._factor(norm_code, False)
         for seg in self.segments:
             desc = self.desc(C, seg)
             nbytes = ctypes.c_size_t()
             _lib.check(L.pf_workspace_size(self.native.handle, ctypes.byref(desc), 1, ctypes.byref(nbytes)))
             ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
             grid, ys, fws, _ = self._split_ws(desc, ws)
+            y = out if acc is None else torch.empty_like(out)
+            h, d, st = self.native.handle, ctypes.byref(desc), _lib.stream_of(self.device)
+            with torch.cuda.device(self.device):
+                _lib.check(L.pf_pad_fft(h, d, _lib.ptr(x), _lib.ptr(grid), _lib.ptr(fws), st))
+                _lib.check(L.pf_interp(h, d, _lib.ptr(grid), _lib.ptr(ys), st))
+                _lib.check(L.pf_unsort_samples(h, d, _lib.ptr(ys), _lib.ptr(y), _lib.ptr(self.weight_flat), fac, st))
+            kept.append(ws)
+            acc = y if acc is None else acc.add_(y)
+        return acc, kept
+
+    # ---- coordinate gradient ------------------------------------------------
+    def coord_grad(self, img, h: torch.Tensor, factor: float, kept=None) -> torch.Tensor:
+        """Re(sum_c conj(h) * d(A img)/d coord) per sample, in user units.
+
+        img (C, nimg, *N) is the image whose forward grid is differentiated
+        (or None with `kept`, the workspaces forward_keep returned for it);
+        h (C, F, npts) the paired k-space (weight applied here); factor the
+        operator's 1/scale and norm factor.  Returns (F*npts, d) ordered (x, y, z).
+        """
+        it = self.interp
+        C = h.shape[0]
+        M = it.nframes * it.npts
+        L = _lib.lib()
+        acc = torch.zeros((M, 3), dtype=torch.float32, device=self.device)
+        for i, seg in enumerate(self.segments):
+            desc = self.desc(C, seg)
+            if kept is not None:
+                ws = kept[i]
+            else:
+                nbytes = ctypes.c_size_t()
+                _lib.check(L.pf_workspace_size(self.native.handle, ctypes.byref(desc), 1, ctypes.byref(nbytes)))
+                ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
+            grid, ys, fws, _ = self._split_ws(desc, ws)
             ds = torch.empty((M, 3), dtype=torch.float32, device=self.device)
             h_, d, st = self.native.handle, ctypes.byref(desc), _lib.stream_of(self.device)
             with torch.cuda.device(self.device):
-                _lib.check(L.pf_pad_fft(h_, d, _lib.ptr(img), _lib.ptr(grid), _lib.ptr(fws), st))
+                if kept is None:
+                    _lib.check(L.pf_pad_fft(h_, d, _lib.ptr(img), _lib.ptr(grid), _lib.ptr(fws), st))
                 _lib.check(L.pf_sort_samples(h_, d, _lib.ptr(h), _lib.ptr(ys), _lib.ptr(self.weight_flat), st))
                 _lib.check(L.pf_interp_grad(h_, d, _lib.ptr(grid), _lib.ptr(ys), _lib.ptr(ds), st))
-            acc += ds
+            acc = ds if len(self.segments) == 1 else acc.add_(ds)
         out = torch.empty_like(acc)
         out[self.native.permutation()] = acc
         # oversampled-grid axes (z, y, x) -> user axes (x, y, z): d c_os / d coord
@@ -348,8 +380,8 @@ def _adjoint_code(norm_code: int) -> int:
     return _lib.PF_NORM_ADJOINT if norm_code == _lib.PF_NORM_NONE else _lib.PF_NORM_NONE
 
 
-def _coord_grad(call: _Call, coord: torch.Tensor, img: torch.Tensor, h: torch.Tensor, adjoint: bool, norm_code: int):
-    g = call.coord_grad(img, h, call._factor(norm_code, adjoint))
+def _coord_grad(call: _Call, coord: torch.Tensor, img, h: torch.Tensor, adjoint: bool, norm_code: int, kept=None):
+    g = call.coord_grad(img, h, call._factor(norm_code, adjoint), kept)
     return g.reshape(coord.shape).to(device=coord.device, dtype=coord.dtype)
 
 
@@ -365,16 +397,19 @@ class _NUFFTTrajFn(torch.autograd.Function):
     def forward(ctx, image, coord, call, norm_code):
         ctx.call = call
         ctx.norm_code = norm_code
-        ctx.save_for_backward(image, coord)
-        return call.forward(image, norm_code)
+        ctx.save_for_backward(coord)
+        # keep the forward's k-space grid: the coordinate gradient reads it
+        y, ctx.kept = call.forward_keep(image, norm_code)
+        return y
 
     @staticmethod
     def backward(ctx, g):
         call = ctx.call
-        image, coord = ctx.saved_tensors
+        (coord,) = ctx.saved_tensors
         g = g.contiguous().to(torch.complex64)
         gi = call.adjoint(g, _adjoint_code(ctx.norm_code)) if ctx.needs_input_grad[0] else None
-        gc = _coord_grad(call, coord, image, g, False, ctx.norm_code) if ctx.needs_input_grad[1] else None
+        gc = _coord_grad(call, coord, None, g, False, ctx.norm_code, ctx.kept) if ctx.needs_input_grad[1] else None
+        ctx.kept = None
         return gi, gc, None, None
 
 

@@ -46,8 +46,11 @@ class NativePlan:
     """Owns one pf_plan on one device."""
 
     def __init__(self, coord_grid: torch.Tensor, nframes: int, npts: int, shape, width, oversamp, device,
-                 tile=None, chunk=0):
+                 tile=None, chunk=0, frame_offsets=None):
+        """frame_offsets (nframes + 1 ints): a ragged plan whose frame f owns
+        samples frame_offsets[f]..frame_offsets[f+1]-1 (npts is then ignored)."""
         self.device = device
+        self.ragged = frame_offsets is not None
         d = len(shape)
         c = coord_grid.detach().to(device=device, dtype=torch.float32).reshape(-1, d).contiguous()
         L = _lib.lib()
@@ -55,14 +58,25 @@ class NativePlan:
         wi = (ctypes.c_int * d)(*[int(w) for w in width])
         ov = (ctypes.c_float * d)(*[float(o) for o in oversamp])
         handle = ctypes.c_void_p()
+        ti = None if tile is None else (ctypes.c_int * d)(*[int(t) for t in tile])
         with torch.cuda.device(device):
-            _lib.check(
-                L.pf_plan_create_tiled(
-                    _lib.ptr(c), d, sh, wi, ov, int(nframes), int(npts),
-                    None if tile is None else (ctypes.c_int * d)(*[int(t) for t in tile]), int(chunk),
-                    _lib.stream_of(device), ctypes.byref(handle),
+            if self.ragged:
+                off = [int(o) for o in frame_offsets]
+                if len(off) != nframes + 1 or off[-1] != c.shape[0]:
+                    raise ValueError("frame_offsets must have nframes + 1 entries ending at the sample count")
+                _lib.check(
+                    L.pf_plan_create_ragged(
+                        _lib.ptr(c), d, sh, wi, ov, int(nframes), (ctypes.c_longlong * len(off))(*off), ti,
+                        int(chunk), _lib.stream_of(device), ctypes.byref(handle),
+                    )
                 )
-            )
+            else:
+                _lib.check(
+                    L.pf_plan_create_tiled(
+                        _lib.ptr(c), d, sh, wi, ov, int(nframes), int(npts), ti, int(chunk),
+                        _lib.stream_of(device), ctypes.byref(handle),
+                    )
+                )
         self.handle = handle
         info = _lib.PlanInfo()
         _lib.check(L.pf_plan_get_info(handle, ctypes.byref(info)))
@@ -73,7 +87,7 @@ class NativePlan:
     def permutation(self) -> torch.Tensor:
         """Original sample index (frame * npts + j) of every tile-sorted position."""
         if self._perm is None:
-            n = int(self.info.nframes) * int(self.info.npts)
+            n = int(self.info.npts) if self.ragged else int(self.info.nframes) * int(self.info.npts)
             perm = torch.empty(n, dtype=torch.int32, device=self.device)
             with torch.cuda.device(self.device):
                 _lib.check(self._lib.pf_plan_get_permutation(self.handle, _lib.ptr(perm), _lib.stream_of(self.device)))

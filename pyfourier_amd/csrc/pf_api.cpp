@@ -33,6 +33,7 @@ int make_batch(const pf_plan* p, const pf_batch_desc* d, pf_batch* b) {
     if ((d->nseg > 1 || has_b0) && (!d->b0_temporal || !d->b0_spatial))
         return pf_fail(PF_ERR_ARG, "a field map needs both b0_temporal and b0_spatial");
     if (has_b0 && d->nt < 1) return pf_fail(PF_ERR_ARG, "nt must be >= 1");
+    if (has_b0 && p->foff) return pf_fail(PF_ERR_UNSUPPORTED, "field-map segments need equal-length frames");
     b->C = d->batch;
     // all batch items innermost (one contiguous vector per grid point) up to
     // 64; larger batches split into blocks of the largest divisor <= 64
@@ -144,6 +145,46 @@ int pf_plan_create_tiled(const float* coord, int ndim, const int* shape, const i
     }
     rc = pf_plan_build(p, coord, static_cast<hipStream_t>(stream));
     if (rc != PF_OK) {
+        pf_plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return PF_OK;
+}
+
+int pf_plan_create_ragged(const float* coord, int ndim, const int* shape, const int* width, const float* oversamp,
+                          int nframes, const long long* frame_offsets, const int* tile, int chunk, void* stream,
+                          pf_plan** out) {
+    if (!out) return pf_fail(PF_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!coord || !shape || !width || !oversamp || !frame_offsets) return pf_fail(PF_ERR_ARG, "null argument");
+    if (nframes < 1) return pf_fail(PF_ERR_ARG, "nframes must be >= 1");
+    if (frame_offsets[0] != 0) return pf_fail(PF_ERR_ARG, "frame_offsets[0] must be 0");
+    for (int f = 0; f < nframes; ++f)
+        if (frame_offsets[f + 1] < frame_offsets[f]) return pf_fail(PF_ERR_ARG, "frame_offsets must not decrease");
+    const long long total = frame_offsets[nframes];
+    pf_plan* p = new (std::nothrow) pf_plan();
+    if (!p) return pf_fail(PF_ERR_ALLOC, "host allocation failed");
+    int rc = pf_plan_params(p, ndim, shape, width, oversamp, 1, (int64_t)total, tile, chunk);
+    if (rc == PF_OK && (int64_t)nframes * p->ntile_tot >= (1ll << 31))
+        rc = pf_fail(PF_ERR_UNSUPPORTED, "too many tiles");
+    if (rc != PF_OK) {
+        pf_plan_free(p);
+        return rc;
+    }
+    p->nframes = nframes;
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        pf_plan_free(p);
+        return pf_fail(PF_ERR_HIP, "hipGetDevice failed");
+    }
+    std::vector<int64_t> off(frame_offsets, frame_offsets + nframes + 1);
+    hipError_t he = hipMalloc(&p->foff, off.size() * sizeof(int64_t));
+    if (he == hipSuccess) he = hipMemcpy(p->foff, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        pf_plan_free(p);
+        return pf_fail(PF_ERR_HIP, std::string("frame offsets: ") + hipGetErrorString(he));
+    }
+    if ((rc = pf_plan_upload_apod(p)) != PF_OK || (rc = pf_plan_build(p, coord, static_cast<hipStream_t>(stream))) != PF_OK) {
         pf_plan_free(p);
         return rc;
     }

@@ -145,6 +145,169 @@ __global__ void __launch_bounds__(TB) k_gather_dcoord(GradParams P, const float*
     }
 }
 
+// Same result with compile-time y/x widths: the lanes of a sample (CPL batch
+// values each) first compute the sample's 3 x W derivative weights together
+// into LDS, then every lane walks the taps with the y/x loops unrolled (all
+// loads of a z row in flight) and 32-bit in-slab offsets.
+template <int CPL, int WY, int WX>
+__global__ void __launch_bounds__(TB) k_gather_dcoord_w(GradParams P, const float* __restrict__ taprec,
+                                                        const float* __restrict__ cos_sorted,
+                                                        const int32_t* __restrict__ perm,
+                                                        const float2* __restrict__ grid,
+                                                        const float2* __restrict__ h, float* __restrict__ dout) {
+    __shared__ float red[TB * 3];
+    __shared__ float sdw[TB][3 * PF_WMAX];  // indexed by the sample's slot in the block
+    const int LS = P.C / CPL;
+    const int SPB = TB / LS;
+    const int slot = threadIdx.x / LS;
+    const int64_t s = (int64_t)blockIdx.x * SPB + slot;
+    const int lane = threadIdx.x % LS;
+    const bool active = threadIdx.x < SPB * LS && s < P.mtot;
+    const float* rec = taprec + (active ? s : 0) * P.rec;
+    const int* ri = reinterpret_cast<const int*>(rec);
+    if (active) {
+        const int nw = P.width[0] + P.width[1] + P.width[2];
+        for (int q = lane; q < nw; q += LS) {
+            const int a = q < P.width[0] ? 0 : (q < P.width[0] + P.width[1] ? 1 : 2);
+            const int k = q - (a == 0 ? 0 : (a == 1 ? P.width[0] : P.width[0] + P.width[1]));
+            const int W = P.width[a];
+            float d = 0.f;
+            if (W > 1) {
+                const double cd = (double)cos_sorted[s * 3 + a];
+                const double half = 0.5 * W;
+                const double x0 = ceil(cd - half);
+                d = (float)(-kb_deriv((x0 + k - cd) / half, (double)P.beta[a]) / half);
+            }
+            sdw[slot][a * PF_WMAX + k] = d;
+        }
+    }
+    __syncthreads();
+    float part[3] = {0.f, 0.f, 0.f};
+    if (active) {
+        const int c0 = lane * CPL;
+        const int cb = c0 / P.Cv, cv = c0 % P.Cv;
+        const int f = ri[3];
+        const int fg = P.nfg > 1 ? f : 0;
+        const int64_t tread = P.b0t ? ((int64_t)perm[s] % P.npts) % P.nt : 0;
+        const int RC = P.R * P.Cv;
+        const float* dw = sdw[slot];
+        int yo[WY], xo[WX];
+        float wy[WY], wx[WX], dy[WY], dx[WX];
+#pragma unroll
+        for (int k = 0; k < WY; ++k) {
+            int v = ri[1] + k;
+            v -= v >= P.os[1] ? P.os[1] : 0;
+            yo[k] = v * P.os[2] * RC;
+            wy[k] = rec[P.woff[1] + k];
+            dy[k] = dw[PF_WMAX + k];
+        }
+#pragma unroll
+        for (int k = 0; k < WX; ++k) {
+            int v = ri[2] + k;
+            v -= v >= P.os[2] ? P.os[2] : 0;
+            xo[k] = v * RC;
+            wx[k] = rec[P.woff[2] + k];
+            dx[k] = dw[2 * PF_WMAX + k];
+        }
+        const int zstride = P.os[1] * P.os[2] * RC;
+        const float2* slab = grid + ((int64_t)fg * P.Cb + cb) * P.gtot * RC + cv;
+        float2 D[3][CPL];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) D[a][q] = make_float2(0.f, 0.f);
+        for (int r = 0; r < P.R; ++r) {
+            float2 acc[3][CPL];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) acc[a][q] = make_float2(0.f, 0.f);
+            const float2* sr = slab + r * P.Cv;
+            for (int i = 0; i < P.width[0]; ++i) {
+                int z = ri[0] + i;
+                z -= z >= P.os[0] ? P.os[0] : 0;
+                const float wz = rec[P.woff[0] + i], dz = dw[i];
+                const float2* pz = sr + z * zstride;
+#pragma unroll
+                for (int b = 0; b < WY; ++b) {
+                    const float2* py = pz + yo[b];
+                    const float wzy = wz * wy[b], dzy = dz * wy[b], wzdy = wz * dy[b];
+#pragma unroll
+                    for (int c = 0; c < WX; ++c) {
+                        const float tz = dzy * wx[c], ty = wzdy * wx[c], tx = wzy * dx[c];
+                        float2 g[CPL];
+                        if constexpr (CPL == 2) {
+                            const float4 v = *reinterpret_cast<const float4*>(py + xo[c]);
+                            g[0] = make_float2(v.x, v.y);
+                            g[CPL - 1] = make_float2(v.z, v.w);
+                        } else {
+                            g[0] = py[xo[c]];
+                        }
+#pragma unroll
+                        for (int q = 0; q < CPL; ++q) {
+                            acc[0][q].x += tz * g[q].x;
+                            acc[0][q].y += tz * g[q].y;
+                            acc[1][q].x += ty * g[q].x;
+                            acc[1][q].y += ty * g[q].y;
+                            acc[2][q].x += tx * g[q].x;
+                            acc[2][q].y += tx * g[q].y;
+                        }
+                    }
+                }
+            }
+            float2 cf = make_float2(1.f, 0.f);
+            if (P.basis || P.b0t) {
+                const int l = r / P.K, kk = r % P.K;
+                if (P.b0t) cf = P.b0t[(int64_t)l * P.nt + tread];
+                if (P.basis) cf = cmul(cf, P.basis[(int64_t)f * P.K + kk]);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const float2 m = cmul(acc[a][q], cf);
+                    D[a][q].x += m.x;
+                    D[a][q].y += m.y;
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const float2 hv = h[s * P.C + c0 + q];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) part[a] += hv.x * D[a][q].x + hv.y * D[a][q].y;  // Re(conj(h) D)
+        }
+    }
+    for (int a = 0; a < 3; ++a) red[threadIdx.x * 3 + a] = part[a];
+    __syncthreads();
+    if (active && lane == 0) {
+        float sum[3] = {0.f, 0.f, 0.f};
+        for (int q = 0; q < LS; ++q)
+            for (int a = 0; a < 3; ++a) sum[a] += red[(threadIdx.x + q) * 3 + a];
+        for (int a = 0; a < 3; ++a) dout[s * 3 + a] = sum[a];
+    }
+}
+
+using GradLaunch = void (*)(dim3, hipStream_t, const GradParams&, const pf_plan*, const float2*, const float2*, float*);
+
+template <int CPL, int WY, int WX>
+void launch_dcoord_w(dim3 grid, hipStream_t s, const GradParams& P, const pf_plan* p, const float2* g,
+                     const float2* h, float* out) {
+    hipLaunchKernelGGL((k_gather_dcoord_w<CPL, WY, WX>), grid, dim3(TB), 0, s, P, p->taprec, p->cos_sorted, p->perm,
+                       g, h, out);
+}
+
+// [cpl - 1][wy == 1 ? 0 : 1][wx - 1]
+const GradLaunch DCOORD_W[2][2][PF_WMAX] = {
+    {{&launch_dcoord_w<1, 1, 1>, &launch_dcoord_w<1, 1, 2>, &launch_dcoord_w<1, 1, 3>, &launch_dcoord_w<1, 1, 4>,
+      &launch_dcoord_w<1, 1, 5>, &launch_dcoord_w<1, 1, 6>, &launch_dcoord_w<1, 1, 7>, &launch_dcoord_w<1, 1, 8>},
+     {nullptr, &launch_dcoord_w<1, 2, 2>, &launch_dcoord_w<1, 3, 3>, &launch_dcoord_w<1, 4, 4>,
+      &launch_dcoord_w<1, 5, 5>, &launch_dcoord_w<1, 6, 6>, &launch_dcoord_w<1, 7, 7>, &launch_dcoord_w<1, 8, 8>}},
+    {{&launch_dcoord_w<2, 1, 1>, &launch_dcoord_w<2, 1, 2>, &launch_dcoord_w<2, 1, 3>, &launch_dcoord_w<2, 1, 4>,
+      &launch_dcoord_w<2, 1, 5>, &launch_dcoord_w<2, 1, 6>, &launch_dcoord_w<2, 1, 7>, &launch_dcoord_w<2, 1, 8>},
+     {nullptr, &launch_dcoord_w<2, 2, 2>, &launch_dcoord_w<2, 3, 3>, &launch_dcoord_w<2, 4, 4>,
+      &launch_dcoord_w<2, 5, 5>, &launch_dcoord_w<2, 6, 6>, &launch_dcoord_w<2, 7, 7>, &launch_dcoord_w<2, 8, 8>}},
+};
+
 }  // namespace
 
 int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* grid, const float2* h, float* out,
@@ -170,6 +333,17 @@ int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* g
     P.nt = b.nt;
     P.basis = b.basis;
     P.b0t = b.b0t;
+    const int cpl = (b.Cv % 2 == 0 && b.C >= 2) ? 2 : 1;
+    const int wy = p->width[1], wx = p->width[2];
+    const bool idx32 = (int64_t)pf_gtot(p) * b.R * b.Cv < (1ll << 31);
+    if (idx32 && (wy == 1 || wy == wx)) {
+        if (GradLaunch fn = DCOORD_W[cpl - 1][wy == 1 ? 0 : 1][wx - 1]) {
+            const int spb = TB / (b.C / cpl);
+            fn(dim3((unsigned)((p->mtot + spb - 1) / spb)), s, P, p, grid, h, out);
+            PF_HIP(hipGetLastError());
+            return PF_OK;
+        }
+    }
     const int SPB = TB / b.C;
     const int64_t nblk = (p->mtot + SPB - 1) / SPB;
     hipLaunchKernelGGL(k_gather_dcoord, dim3((unsigned)nblk), dim3(TB), 0, s, P, p->taprec, p->cos_sorted, p->perm,

@@ -54,8 +54,8 @@ struct pf_plan {
     float beta[3];
     double scale = 1.0;    // kernel-sum normalisation (divided out)
     int nframes = 1;
-    int64_t npts = 0;      // samples per frame
-    int64_t mtot = 0;      // nframes * npts
+    int64_t npts = 0;      // samples per frame (ragged plans: the total)
+    int64_t mtot = 0;      // nframes * npts (ragged plans: the total)
     int tile[3];           // spread/sort tile (z, y, x)
     int ntiles[3];
     int64_t ntile_tot = 0; // tiles per frame
@@ -69,6 +69,7 @@ struct pf_plan {
     float* taprec = nullptr;     // (mtot, rec): int start[3], int frame, weights
     float* cos_sorted = nullptr; // (mtot, 3) oversampled coordinate (z, y, x)
     float* apod = nullptr;       // deapodisation vectors, Nz + Ny + Nx floats
+    int64_t* foff = nullptr;     // ragged plans: frame f owns original samples [foff[f], foff[f+1])
     // spread lists: [0] keyed by (frame, tile) for per-frame grids,
     // [1] keyed by tile only for one grid shared by all frames (subspace);
     // identical (aliased) when nframes == 1

@@ -54,6 +54,8 @@ struct TapParams {
     int rec;
     int woff[3];
     int mbits[3];  // Morton bits per axis (0 on all axes: row-major tile order)
+    const int64_t* foff;  // ragged frame offsets (nframes + 1) or null
+    int nframes;
 };
 
 // Interleave the bits of (tz, ty, tx), x fastest: consecutive codes are
@@ -82,7 +84,19 @@ __global__ void k_taps(const float* __restrict__ coord, TapParams P, uint32_t* _
                        float* __restrict__ cos_out) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.mtot) return;
-    int64_t f = i / P.npts;
+    int64_t f;
+    if (P.foff) {
+        // last frame whose first sample is <= i
+        int lo = 0, hi = P.nframes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (P.foff[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        f = lo;
+    } else {
+        f = i / P.npts;
+    }
     int64_t tcoord[3];
     float* r = rec + i * P.rec;
     int* ri = reinterpret_cast<int*>(r);
@@ -433,6 +447,8 @@ int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
     P.mtot = p->mtot;
     P.ntile_tot = p->ntile_tot;
     P.rec = p->rec;
+    P.foff = p->foff;
+    P.nframes = p->nframes;
     int msum = 0;
     for (int a = 0; a < 3; ++a) {
         P.mbits[a] = 0;
@@ -483,7 +499,7 @@ int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
 
 void pf_plan_free(pf_plan* p) {
     if (!p) return;
-    void* ptrs[] = {p->perm, p->iperm, p->taprec, p->cos_sorted, p->apod};
+    void* ptrs[] = {p->perm, p->iperm, p->taprec, p->cos_sorted, p->apod, p->foff};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     const bool alias = p->sl[1].slist == p->sl[0].slist;
