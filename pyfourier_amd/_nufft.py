@@ -108,6 +108,7 @@ class _Call:
             if self.weight_flat is None:
                 self.weight_full = w.to(torch.complex64 if torch.is_complex(w) else torch.float32)
         self.segments = self._segments()
+        self._cache = {}
 
     def _segments(self):
         p = self.plan
@@ -231,10 +232,14 @@ class _Call:
         out[self.native.permutation()] = acc
         # oversampled-grid axes (z, y, x) -> user axes (x, y, z): d c_os / d coord
         dd = it.ndim
-        scale = torch.tensor(
-            [factor * it.os_shape[a] / it.shape[a] * it.unit_scale[dd - 1 - a] for a in range(dd)][::-1],
-            dtype=torch.float32, device=self.device,
-        )
+        key = ("coord_scale", factor)
+        scale = self._cache.get(key)
+        if scale is None:
+            scale = torch.tensor(
+                [factor * it.os_shape[a] / it.shape[a] * it.unit_scale[dd - 1 - a] for a in range(dd)][::-1],
+                dtype=torch.float32,
+            ).to(self.device, non_blocking=True)
+            self._cache[key] = scale
         return out[:, 3 - dd:].flip(1) * scale
 
     # ---- stage-by-stage execution (timing) ---------------------------------

@@ -34,7 +34,7 @@ def spread_tiles(ndim: int) -> list:
     return [tuple(buf[3 * i + 3 - ndim: 3 * i + 3]) for i in range(n.value)]
 
 
-def autotune_plan(coord, shape, width=4, oversamp=1.25, nbatch=1, basis=None, device="cuda", chunks=(1024, 256),
+def autotune_plan(coord, shape, width=4, oversamp=1.25, nbatch=1, basis=None, device="cuda", chunks=(0, 1024),
                   reps=3, return_table=False, **plan_kw):
     """plan_nufft(coord, shape, width, oversamp, **plan_kw) with the tile and
     work-item size that make nufft + nufft_adj fastest on this trajectory for

@@ -74,7 +74,7 @@ struct pf_plan {
     // [1] keyed by tile only for one grid shared by all frames (subspace);
     // identical (aliased) when nframes == 1
     pf_slist sl[2];
-    int max_chunk = 0;
+    int max_chunk = 0;    // spread entries per work item (0: automatic)
 
     pf_fft_cache* fft = nullptr;   // rocFFT plans (grid sizes with other prime factors)
     struct pf_fftk* fftk = nullptr; // fused pruned FFT (sizes 2^a 3^b 5^c)

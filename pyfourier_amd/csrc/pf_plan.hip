@@ -324,8 +324,8 @@ int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_
     if ((int64_t)nframes * p->ntile_tot >= (1ll << 31))
         return pf_fail(PF_ERR_UNSUPPORTED, "too many tiles");
     p->rec = (off + 3) & ~3;
-    if (chunk < 0) return pf_fail(PF_ERR_ARG, "chunk must be >= 0 (0: default)");
-    p->max_chunk = chunk > 0 ? chunk : 1024;
+    if (chunk < 0) return pf_fail(PF_ERR_ARG, "chunk must be >= 0 (0: automatic)");
+    p->max_chunk = chunk;  // 0: chosen from the spread list size when it is built
     return PF_OK;
 }
 
@@ -384,7 +384,18 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
     PF_HIP(hipStreamSynchronize(stream));
 
     // work items: heavy tiles split into chunks that a reduce pass sums in order
-    const int CH = p->max_chunk;
+    // entries per work item: a chunk is summed serially by one wave, so a
+    // tile far heavier than the rest (the k-space centre of radial and
+    // spiral trajectories) bounds the kernel unless it is split; splitting
+    // tiles of ordinary weight only adds partial-tile traffic.  Split above
+    // four times the mean entries of a non-empty tile, within 128..1024.
+    int CH = p->max_chunk;
+    if (CH <= 0) {
+        int64_t nonempty = 0;
+        for (uint64_t k = 0; k < nkeys; ++k) nonempty += he[k] > hs[k];
+        const int64_t c = 4 * NE / std::max<int64_t>(1, nonempty);
+        CH = (int)std::min<int64_t>(1024, std::max<int64_t>(128, c));
+    }
     std::vector<int4> items, ritems;
     int slot = 0;
     // visit tiles in the sample order's Z-order (when it is used), frame by frame
