@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--compare-width", type=int, default=4, help="second width timed beside it (0: none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the NUDFT error check")
-    ap.add_argument("--cpu-coils", type=int, default=8, help="coils in the CPU baseline sample")
+    ap.add_argument("--cpu-coils", type=int, default=32, help="coils in the CPU baseline sample")
     ap.add_argument("--small", action="store_true", help="64^3 x 8 coils quick run")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched / autograd comparisons")
     return ap.parse_args()
@@ -171,6 +171,25 @@ def time_width(pf, stage_timer, coord, N, width, xc_dev, w_dev, dev, args):
     log(f"width {width}: timed {args.steps} steps: {ms:.3f} ms/step")
     stages = {k: float(np.mean(v)) for k, v in timer.summary().items()}
     return plan, ms, stages, plan_s
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "nufft3d_128cube_32coil_w6_traffic.json")
+STAGE_KERNELS = {"spread": ("k_spread_reg<4, 4, 4>", "k_spread_reduce"), "interp": ("k_gather_w<2, 6, 6>",)}
+
+
+def measured_traffic(stage, args):
+    """HBM bytes per call of a stage at the headline workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated against a
+    device copy of known size); None for other workloads."""
+    if args.width != 6 or stage not in STAGE_KERNELS or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        k = json.load(f)["kernels"]
+    try:
+        return int(sum(k[n]["read_bytes"] + k[n]["write_bytes"] for n in STAGE_KERNELS[stage]))
+    except KeyError:
+        return None
 
 
 def _time(fn, dev, steps, warmup=2):
@@ -357,6 +376,7 @@ def main():
     dom = max(own, key=own.get)
     dom_bytes = stage_bytes(dom, C, N3, G3, M, 3)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(dom, args) if not args.small else None
 
     acc = None
     if not args.no_accuracy:
@@ -405,12 +425,12 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": dom + (" (" + " + ".join(STAGE_KERNELS[dom]) + ")" if dom in STAGE_KERNELS else ""),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "copy_GBs": copy_gbs,
         },
         "stages": stage_info,

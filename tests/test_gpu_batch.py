@@ -38,14 +38,15 @@ def _items(rng, n):
 @pytest.mark.parametrize("norm,group", [(None, True), ("ortho", True), (None, False)])
 def test_batch_replays_equal_per_item_calls(cuda_device, norm, group):
     """Grouped items run as frames of one ragged plan: per frame the same
-    sample order, tiles and chunks as the item's own plan, so the results
-    are bit-identical to the per-item drop-in calls."""
+    sample order, tiles and (with a fixed spread chunk size; the automatic
+    one depends on the whole list) chunks as the item's own plan, so the
+    results are bit-identical to the per-item drop-in calls."""
     rng = np.random.default_rng(0)
     C = 3
     coords, shapes = _items(rng, 7)
     weights = [demo.spiral_density(c) for c in coords]
     batch = pf.plan_nufft_batch(coords, shapes, nbatch=C, width=4, oversamp=2.0, weights=weights, norm=norm,
-                                device=cuda_device, group=group)
+                                device=cuda_device, group=group, chunk=1024)
     assert (len(batch.groups) > 0) == group
     imgs = [torch.from_numpy(demo.complex_gaussian(rng, (C, 1, N, N))).to(cuda_device) for N in shapes]
     ksps = [torch.from_numpy(demo.complex_gaussian(rng, (C, 1, c.shape[0] * c.shape[1]))).to(cuda_device)
