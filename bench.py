@@ -175,7 +175,7 @@ def time_width(pf, stage_timer, coord, N, width, xc_dev, w_dev, dev, args):
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "nufft3d_128cube_32coil_w6_traffic.json")
-STAGE_KERNELS = {"spread": ("k_spread_reg<4, 4, 4>", "k_spread_reduce"), "interp": ("k_gather_w<2, 6, 6>",)}
+STAGE_KERNELS = {"spread": ("k_spread_reg", "k_spread_reduce"), "interp": ("k_gather_w",)}
 
 
 def measured_traffic(stage, args):
@@ -186,10 +186,13 @@ def measured_traffic(stage, args):
         return None
     with open(TRAFFIC_FILE) as f:
         k = json.load(f)["kernels"]
-    try:
-        return int(sum(k[n]["read_bytes"] + k[n]["write_bytes"] for n in STAGE_KERNELS[stage]))
-    except KeyError:
-        return None
+    total = 0
+    for prefix in STAGE_KERNELS[stage]:
+        hits = [v for n, v in k.items() if n.split("<")[0] == prefix]
+        if len(hits) != 1:
+            return None
+        total += hits[0]["read_bytes"] + hits[0]["write_bytes"]
+    return int(total)
 
 
 def _time(fn, dev, steps, warmup=2):

@@ -28,13 +28,19 @@
 
 namespace {
 
-constexpr int FT = 256;       // threads per block
+#ifndef PF_FFT_THREADS
+#define PF_FFT_THREADS 256
+#endif
+#ifndef PF_FFT_GBMAX
+#define PF_FFT_GBMAX 4096
+#endif
+constexpr int FT = PF_FFT_THREADS;  // threads per block
 // minimum waves per SIMD the FFT kernels are compiled for: 4 caps them at
 // 128 VGPRs, so four 256-thread blocks (the LDS limit at G*B = 4096) fit a CU
 #ifndef PF_FFT_WPE
 #define PF_FFT_WPE 4
 #endif
-constexpr int GBMAX = 4096;   // complex values per block (G * B)
+constexpr int GBMAX = PF_FFT_GBMAX;  // complex values per block (G * B)
 
 struct AxisFFT {
     int G;
@@ -446,6 +452,17 @@ int pf_fftk_init(pf_plan* p) {
             pf_fftk_destroy(f);
             return pf_fail(PF_ERR_HIP, "twiddle table upload failed");
         }
+    }
+    if ((size_t)(GBMAX + GBMAX / 8 + 4096) * sizeof(float2) > 64 * 1024) {
+        // blocks of more than 64 KB of LDS must be allowed explicitly
+        const void* kernels[] = {reinterpret_cast<const void*>(k_fftx_fwd), reinterpret_cast<const void*>(k_fftx_adj),
+                                 reinterpret_cast<const void*>(k_fftcol<true>),
+                                 reinterpret_cast<const void*>(k_fftcol<false>)};
+        for (const void* k : kernels)
+            if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) {
+                pf_fftk_destroy(f);
+                return pf_fail(PF_ERR_HIP, "could not raise the FFT kernels' LDS limit");
+            }
     }
     p->fftk = f;
     return PF_OK;

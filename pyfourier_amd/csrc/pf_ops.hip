@@ -913,8 +913,8 @@ struct RegTile {
 };
 
 const RegTile REG_TILES[] = {
-    {3, 4, 4, 8, &launch_reg<4, 4, 4>},   {3, 4, 8, 4, &launch_reg<4, 8, 2>},
-    {3, 8, 4, 4, &launch_reg<8, 4, 2>},   {3, 2, 4, 16, &launch_reg<2, 4, 8>},
+    {3, 8, 4, 4, &launch_reg<8, 4, 2>},   {3, 4, 4, 8, &launch_reg<4, 4, 4>},
+    {3, 4, 8, 4, &launch_reg<4, 8, 2>},   {3, 2, 4, 16, &launch_reg<2, 4, 8>},
     {2, 1, 8, 16, &launch_reg<1, 8, 8>},  {2, 1, 16, 8, &launch_reg<1, 16, 4>},
     {2, 1, 4, 32, &launch_reg<1, 4, 16>}, {2, 1, 32, 4, &launch_reg<1, 32, 2>},
     {1, 1, 1, 64, &launch_reg<1, 1, 32>},

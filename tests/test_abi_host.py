@@ -149,7 +149,7 @@ def test_spread_tile_listing_host_only():
     """pf_spread_tiles is host-only: it lists the register-spread tiles."""
     import pyfourier_amd as pf
 
-    assert pf.spread_tiles(3)[0] == (4, 4, 8)
+    assert pf.spread_tiles(3)[0] == (8, 4, 4)
     assert all(len(t) == 2 for t in pf.spread_tiles(2))
     assert pf.spread_tiles(1) == [(64,)]
     L = _lib.lib()
