@@ -396,6 +396,118 @@ __global__ void __launch_bounds__(FT, PF_FFT_WPE) k_fftcol(AxisFFT A, int64_t I,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Column pass for G = 256 as a four-step FFT in registers: element n = t + 16k
+// of a column is held by thread t (k = 0..15); thread t takes the 16-point DFT
+// over k, multiplies by exp(-2 pi i t m1 / 256), the block transposes through
+// LDS once, and thread m1 takes the 16-point DFT over t, giving output
+// m1 + 16 m2.  Two LDS accesses per value instead of two per radix pass, and
+// zero rows of the padded input are not loaded.  Lane = (column b, t), b
+// fastest, so global accesses are 16 consecutive columns of one row.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dft4v(float2& a, float2& b, float2& c, float2& d) {
+    const float2 s0 = c_add(a, c), d0 = c_sub(a, c);
+    const float2 s1 = c_add(b, d), d1 = c_negi(c_sub(b, d));
+    a = c_add(s0, s1);
+    c = c_sub(s0, s1);
+    b = c_add(d0, d1);
+    d = c_sub(d0, d1);
+}
+
+// forward 16-point DFT in place (kernel exp(-2 pi i n m / 16)), v[n] -> v[m]
+__device__ __forceinline__ void dft16(float2* v) {
+    // n = 4a + b: DFT4 over a for each b, twiddle W16^(b c), DFT4 over b
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;  // W16^1 = c1 - i s1
+    const float h = 0.70710678118654752440f;                                  // W16^2 = h - i h
+#pragma unroll
+    for (int b = 0; b < 4; ++b) dft4v(v[b], v[4 + b], v[8 + b], v[12 + b]);
+    // Y_b[c] sits at v[4c + b]; multiply by W16^(b c)
+    v[4 * 1 + 1] = c_mul(v[5], make_float2(c1, -s1));
+    v[4 * 1 + 2] = c_mul(v[6], make_float2(h, -h));
+    v[4 * 1 + 3] = c_mul(v[7], make_float2(s1, -c1));
+    v[4 * 2 + 1] = c_mul(v[9], make_float2(h, -h));
+    v[4 * 2 + 2] = c_negi(v[10]);
+    v[4 * 2 + 3] = c_mul(v[11], make_float2(-h, -h));
+    v[4 * 3 + 1] = c_mul(v[13], make_float2(s1, -c1));
+    v[4 * 3 + 2] = c_mul(v[14], make_float2(-h, -h));
+    v[4 * 3 + 3] = c_mul(v[15], make_float2(-c1, s1));
+    // DFT4 over b for each c: X[c + 4d]
+    float2 o[16];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float2 a0 = v[4 * c], a1 = v[4 * c + 1], a2 = v[4 * c + 2], a3 = v[4 * c + 3];
+        dft4v(a0, a1, a2, a3);
+        o[c] = a0;
+        o[c + 4] = a1;
+        o[c + 8] = a2;
+        o[c + 12] = a3;
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = o[m];
+}
+
+constexpr int C256_PLANE = 16 * 16 + 8;  // LDS plane stride (float2) of the transpose
+
+template <bool FWD>
+__global__ void __launch_bounds__(256, PF_FFT_WPE) k_fftcol256(int N, int64_t I, const float2* __restrict__ twg,
+                                                             const float2* __restrict__ in, float2* __restrict__ out) {
+    constexpr int G = 256, B = 16;
+    __shared__ float2 sh[16 * C256_PLANE];
+    __shared__ float2 tw[G];
+    const int b = threadIdx.x & 15, t = threadIdx.x >> 4;
+    const int64_t nchunk = (I + B - 1) / B;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t o = blk / nchunk;
+    const int64_t i0 = (blk % nchunk) * B;
+    const bool bok = i0 + b < I;
+    tw[threadIdx.x] = twg[threadIdx.x];
+    const int lo = N - N / 2, hi = G - N / 2;  // stored rows [lo, hi) carry no image data
+    float2 v[16];
+    if (FWD) {
+        const float2* src = in + o * (int64_t)N * I + i0 + b;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int n = t + 16 * k;  // stored row
+            const int p = n < lo ? n + N / 2 : n - G + N / 2;
+            v[k] = (bok && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
+        }
+    } else {
+        const float2* src = in + o * (int64_t)G * I + i0 + b;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float2 x = bok ? src[(int64_t)(t + 16 * k) * I] : make_float2(0.f, 0.f);
+            v[k] = make_float2(x.x, -x.y);
+        }
+    }
+    dft16(v);
+    __syncthreads();  // twiddle table
+#pragma unroll
+    for (int m1 = 1; m1 < 16; ++m1) v[m1] = c_mul(v[m1], tw[t * m1]);
+#pragma unroll
+    for (int m1 = 0; m1 < 16; ++m1) sh[m1 * C256_PLANE + t * 16 + b] = v[m1];
+    __syncthreads();
+    // this thread now owns m1 = t and reads Y_t'[m1] for t' = 0..15
+#pragma unroll
+    for (int tp = 0; tp < 16; ++tp) v[tp] = sh[t * C256_PLANE + tp * 16 + b];
+    dft16(v);  // v[m2] = X[t + 16 m2]
+    if (!bok) return;
+    if (FWD) {
+        float2* dst = out + o * (int64_t)G * I + i0 + b;
+#pragma unroll
+        for (int m2 = 0; m2 < 16; ++m2) dst[(int64_t)(t + 16 * m2) * I] = v[m2];
+    } else {
+        float2* dst = out + o * (int64_t)N * I + i0 + b;
+#pragma unroll
+        for (int m2 = 0; m2 < 16; ++m2) {
+            const int n = t + 16 * m2;  // stored output index -> cropped row p
+            if (n < lo || n >= hi) {
+                const int p = n < lo ? n + N / 2 : n - G + N / 2;
+                dst[(int64_t)p * I] = make_float2(v[m2].x, -v[m2].y);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -553,6 +665,17 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
 static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, const float2* src, float2* dst,
                       hipStream_t s) {
     const pf_fftk_axis& X = p->fftk->ax[axis];
+    if (X.G == 256) {
+        const int64_t nblk = O * ((I + 15) / 16);
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+        if (nblk < 1) return PF_OK;
+        if (fwd)
+            hipLaunchKernelGGL(k_fftcol256<true>, dim3((unsigned)nblk), dim3(256), 0, s, X.N, I, X.tw, src, dst);
+        else
+            hipLaunchKernelGGL(k_fftcol256<false>, dim3((unsigned)nblk), dim3(256), 0, s, X.N, I, X.tw, src, dst);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
     AxisFFT A = axis_desc(X);
     const int B = cols_for(X.G);
     int bshift = 0;

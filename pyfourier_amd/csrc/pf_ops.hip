@@ -755,63 +755,61 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
 }
 
 // Sample permutation between the user layout (C, F*npts) and the tile-sorted
-// layout (Mtot, C), through an LDS transpose of PSB sorted samples x up to
-// PSC batch values per block: the sorted side moves whole contiguous rows,
-// the user side has consecutive lanes on consecutive sorted samples, whose
-// user indices perm[s] mostly run consecutively along a readout.
+// layout (Mtot, C), through an LDS transpose of PSB consecutive USER samples
+// x up to PSC batch values per block: the user side reads/writes contiguous
+// runs of each batch plane, the sorted side whole coil rows (C values of one
+// sample are contiguous there) at iperm[j].
 constexpr int PSB = 64;
 constexpr int PSC = 64;
 
 // user -> sorted, times the density weight
-__global__ void __launch_bounds__(TB) k_sort(int64_t mtot, int C, const int32_t* __restrict__ perm,
+__global__ void __launch_bounds__(TB) k_sort(int64_t mtot, int C, const int32_t* __restrict__ iperm,
                                              const float2* __restrict__ y, const float* __restrict__ w,
                                              float2* __restrict__ ys) {
     __shared__ float2 t[PSB][PSC + 1];
-    const int64_t s0 = (int64_t)blockIdx.x * PSB;
+    const int64_t j0 = (int64_t)blockIdx.x * PSB;
     const int c0 = blockIdx.y * PSC;
     const int cc = min(PSC, C - c0);
-    const int ns = (int)min<int64_t>(PSB, mtot - s0);
+    const int nj = (int)min<int64_t>(PSB, mtot - j0);
     for (int i = threadIdx.x; i < cc * PSB; i += TB) {
-        const int c = i / PSB, sl = i % PSB;
-        if (sl < ns) {
-            const int64_t j = perm[s0 + sl];
-            float2 v = y[(int64_t)(c0 + c) * mtot + j];
+        const int c = i / PSB, jl = i % PSB;
+        if (jl < nj) {
+            float2 v = y[(int64_t)(c0 + c) * mtot + j0 + jl];
             if (w) {
-                const float a = w[j];
+                const float a = w[j0 + jl];
                 v.x *= a;
                 v.y *= a;
             }
-            t[sl][c] = v;
+            t[jl][c] = v;
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < ns * cc; i += TB) {
-        const int sl = i / cc, c = i % cc;
-        ys[(s0 + sl) * C + c0 + c] = t[sl][c];
+    for (int i = threadIdx.x; i < nj * cc; i += TB) {
+        const int jl = i / cc, c = i % cc;
+        ys[(int64_t)iperm[j0 + jl] * C + c0 + c] = t[jl][c];
     }
 }
 
 // sorted -> user, times factor and the density weight
-__global__ void __launch_bounds__(TB) k_unsort(int64_t mtot, int C, const int32_t* __restrict__ perm,
+__global__ void __launch_bounds__(TB) k_unsort(int64_t mtot, int C, const int32_t* __restrict__ iperm,
                                                const float2* __restrict__ ys, const float* __restrict__ w,
                                                float factor, float2* __restrict__ y) {
     __shared__ float2 t[PSB][PSC + 1];
-    const int64_t s0 = (int64_t)blockIdx.x * PSB;
+    const int64_t j0 = (int64_t)blockIdx.x * PSB;
     const int c0 = blockIdx.y * PSC;
     const int cc = min(PSC, C - c0);
-    const int ns = (int)min<int64_t>(PSB, mtot - s0);
-    for (int i = threadIdx.x; i < ns * cc; i += TB) {
-        const int sl = i / cc, c = i % cc;
-        t[sl][c] = ys[(s0 + sl) * C + c0 + c];
+    const int nj = (int)min<int64_t>(PSB, mtot - j0);
+    for (int i = threadIdx.x; i < nj * cc; i += TB) {
+        const int jl = i / cc, c = i % cc;
+        t[jl][c] = ys[(int64_t)iperm[j0 + jl] * C + c0 + c];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < cc * PSB; i += TB) {
-        const int c = i / PSB, sl = i % PSB;
-        if (sl < ns) {
-            const int64_t j = perm[s0 + sl];
-            const float a = factor * (w ? w[j] : 1.0f);
-            const float2 v = t[sl][c];
-            y[(int64_t)(c0 + c) * mtot + j] = make_float2(v.x * a, v.y * a);
+        const int c = i / PSB, jl = i % PSB;
+        if (jl < nj) {
+            const float a = factor * (w ? w[j0 + jl] : 1.0f);
+            const float2 v = t[jl][c];
+            y[(int64_t)(c0 + c) * mtot + j0 + jl] = make_float2(v.x * a, v.y * a);
         }
     }
 }
@@ -1020,7 +1018,7 @@ static dim3 permute_grid(const pf_plan* p, const pf_batch& b) {
 int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2* ys, const float* weight,
                    hipStream_t s) {
     if ((p->mtot + PSB - 1) / PSB > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many samples");
-    hipLaunchKernelGGL(k_sort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->perm, y, weight, ys);
+    hipLaunchKernelGGL(k_sort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, y, weight, ys);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
@@ -1028,7 +1026,7 @@ int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2*
 int pf_launch_unsort(const pf_plan* p, const pf_batch& b, const float2* ys, float2* y, const float* weight,
                      float factor, hipStream_t s) {
     if ((p->mtot + PSB - 1) / PSB > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many samples");
-    hipLaunchKernelGGL(k_unsort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->perm, ys, weight, factor, y);
+    hipLaunchKernelGGL(k_unsort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, ys, weight, factor, y);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }

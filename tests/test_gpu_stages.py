@@ -30,6 +30,9 @@ CASES = [
     ((64, 128), 1.25),     # 80 x 160
     ((48, 40), 1.5),       # 72 x 60
     ((16, 20, 24), 2.0),   # 32 x 40 x 48
+    ((128, 100), 2.0),     # 256 x 200: y is a 256-point column pass
+    ((120, 96, 80), 2.0),  # 240 x 192 x 160 ... and 256-point z below
+    ((128, 24, 20), 2.0),  # 256 x 48 x 40: z is a 256-point column pass
 ]
 
 
