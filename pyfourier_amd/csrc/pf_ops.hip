@@ -584,9 +584,19 @@ constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 #ifndef PF_SPREAD_UNROLL
 #define PF_SPREAD_UNROLL 1
 #endif
+#ifndef PF_SPREAD_WREG
+#define PF_SPREAD_WREG 1  // an entry's z/y weights in registers (measured faster)
+#endif
+
+// 3-D tiles (few z/y rows) keep an entry's z/y weights in registers and are
+// compiled for >= 3 waves per SIMD (168 VGPRs); the 2-D tiles (long y
+// columns or x rows) would spill under that cap and read the weights from
+// LDS per row instead.
+template <int ZY>  // 100 * TZ + TY
+constexpr bool spread_wreg() { return PF_SPREAD_WREG && ZY / 100 > 1 && ZY / 100 + ZY % 100 <= 12; }
 
 template <int TZ, int TY, int TXL>
-__global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, const int4* __restrict__ items,
+__global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_spread_reg(GeoParams g, BatchParams B, const int4* __restrict__ items,
                                                    const int32_t* __restrict__ slist,
                                                    const float* __restrict__ taprec,
                                                    const int32_t* __restrict__ perm,
@@ -698,19 +708,29 @@ __global__ void __launch_bounds__(64) k_spread_reg(GeoParams g, BatchParams B, c
         for (int j = 0; j < n; ++j) {
             const unsigned mz = __builtin_amdgcn_readfirstlane(s_zmask[j]);
             const unsigned my = __builtin_amdgcn_readfirstlane(s_ymask[j]);
+            const float2 val = s_val[j][v];
             const float* w = s_w[j];
             float wxl[TXL];
 #pragma unroll
             for (int k = 0; k < TXL; ++k) wxl[k] = w[TZ + TY + half * TXL + k];
-            const float2 val = s_val[j][v];
+            // all z/y weights of the entry up front (vector LDS reads), so the
+            // row loop does not wait on LDS
+            constexpr bool WR = spread_wreg<100 * TZ + TY>();
+            float wzr[WR ? TZ : 1], wyr[WR ? TY : 1];
+            if constexpr (WR) {
+#pragma unroll
+                for (int a = 0; a < TZ; ++a) wzr[a] = w[a];
+#pragma unroll
+                for (int b = 0; b < TY; ++b) wyr[b] = w[TZ + b];
+            }
 #pragma unroll
             for (int a = 0; a < TZ; ++a) {
                 if (!((mz >> a) & 1u)) continue;
-                const float wz = w[a];
+                const float wz = WR ? wzr[WR ? a : 0] : w[a];
 #pragma unroll
                 for (int b = 0; b < TY; ++b) {
                     if (!((my >> b) & 1u)) continue;
-                    const float wzy = wz * w[TZ + b];
+                    const float wzy = wz * (WR ? wyr[WR ? b : 0] : w[TZ + b]);
                     const float vx = wzy * val.x, vy = wzy * val.y;
 #pragma unroll
                     for (int k = 0; k < TXL; ++k) {
