@@ -314,7 +314,7 @@ def bench_autograd(pf, dev, steps):
     x0 = torch.randn((C, N, N), dtype=torch.complex64, device=dev)
     target = pf.nufft(x0, nufft_plan=plan, device=dev)
     x = (x0 * 0.9).requires_grad_()
-    ct = torch.from_numpy(coord).requires_grad_()
+    ct = torch.from_numpy(coord).to(dev).requires_grad_()  # trajectory parameters live on the GPU
 
     def autograd():
         x.grad = None

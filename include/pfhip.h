@@ -222,15 +222,19 @@ int pf_unsort_samples(const pf_plan* plan, const pf_batch_desc* desc, const void
                       void* kspace, const float* weight, float factor, void* stream);
 
 /* Coordinate derivative of the forward gather (autograd of nufft/nufft_adj
- * with respect to the trajectory).  For tile-sorted h (F*npts, C) and the
- * forward grid (pf_pad_fft of the image), writes per sorted sample
- *   dcoord[s, a] = Re(sum_c conj(h[s, c]) * d(gather_c)/d(c_os[s, a]))
- * for the oversampled coordinate axes a = (z, y, x) (0 on unused axes), using
- * the analytic derivative of the Kaiser-Bessel tap weights; the caller applies
- * 1/scale, the norm factor and the coordinate unit scaling.  Replaces the
+ * with respect to the trajectory).  For tile-sorted h (Mtot, C) and the
+ * forward grid (pf_pad_fft of the image), computes per sample
+ *   g[a] = Re(sum_c conj(h[s, c]) * d(gather_c)/d(c_os[s, a]))
+ * for the oversampled coordinate axes a = (z, y, x), using the analytic
+ * derivative of the Kaiser-Bessel tap weights, and writes
+ *   dcoord[j, 2 - a] (=|+=) axis_scale[a] * g[a]
+ * at the sample's ORIGINAL index j, i.e. dcoord is (Mtot, ndim) float32 in
+ * the coordinate order (x, y, z); axis_scale (host, 3 floats, z y x order,
+ * NULL = 1) carries 1/scale, the norm factor and d c_os / d coord;
+ * accumulate != 0 adds to dcoord (several B0 segment chunks).  Replaces the
  * incomplete _nufft_backward, src/pyfourier/_nufft/_nufft.py:175-209. */
 int pf_interp_grad(const pf_plan* plan, const pf_batch_desc* desc, const void* grid, const void* h_sorted,
-                   float* dcoord_sorted, void* stream);
+                   const float* axis_scale, int accumulate, float* dcoord, void* stream);
 
 /* ---- Toeplitz-embedded Gram operator ------------------------------------ */
 

@@ -94,7 +94,7 @@ _SIG = {
         [_VP, ctypes.POINTER(BatchDesc)] + [ctypes.POINTER(ctypes.c_size_t)] * 4,
     ),
     "pf_pad_fft": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
-    "pf_interp_grad": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
+    "pf_interp_grad": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, ctypes.POINTER(ctypes.c_float), _I, _VP, _VP]),
     "pf_toeplitz_create":(_I, [_VP, _I, ctypes.POINTER(_I), _I, _I, _VP, ctypes.POINTER(_VP)]),
     "pf_toeplitz_destroy": (_I, [_VP]),
     "pf_toeplitz_workspace_size": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_size_t)]),

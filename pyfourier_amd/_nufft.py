@@ -108,7 +108,6 @@ class _Call:
             if self.weight_flat is None:
                 self.weight_full = w.to(torch.complex64 if torch.is_complex(w) else torch.float32)
         self.segments = self._segments()
-        self._cache = {}
 
     def _segments(self):
         p = self.plan
@@ -204,13 +203,20 @@ class _Call:
         img (C, nimg, *N) is the image whose forward grid is differentiated
         (or None with `kept`, the workspaces forward_keep returned for it);
         h (C, F, npts) the paired k-space (weight applied here); factor the
-        operator's 1/scale and norm factor.  Returns (F*npts, d) ordered (x, y, z).
+        operator's 1/scale and norm factor.  Returns (F*npts, d) ordered
+        (x, y, z), written by the native kernel at the original sample order.
         """
         it = self.interp
         C = h.shape[0]
         M = it.nframes * it.npts
+        dd = it.ndim
         L = _lib.lib()
-        acc = torch.zeros((M, 3), dtype=torch.float32, device=self.device)
+        # d c_os / d coord per stored axis (z, y, x): G/N times the unit factor
+        sc = [1.0, 1.0, 1.0]
+        for a in range(dd):
+            sc[3 - dd + a] = factor * it.os_shape[a] / it.shape[a] * it.unit_scale[dd - 1 - a]
+        scale = (ctypes.c_float * 3)(*sc)
+        out = torch.empty((M, dd), dtype=torch.float32, device=self.device)
         for i, seg in enumerate(self.segments):
             desc = self.desc(C, seg)
             if kept is not None:
@@ -220,27 +226,13 @@ class _Call:
                 _lib.check(L.pf_workspace_size(self.native.handle, ctypes.byref(desc), 1, ctypes.byref(nbytes)))
                 ws = torch.empty(max(1, nbytes.value), dtype=torch.uint8, device=self.device)
             grid, ys, fws, _ = self._split_ws(desc, ws)
-            ds = torch.empty((M, 3), dtype=torch.float32, device=self.device)
             h_, d, st = self.native.handle, ctypes.byref(desc), _lib.stream_of(self.device)
             with torch.cuda.device(self.device):
                 if kept is None:
                     _lib.check(L.pf_pad_fft(h_, d, _lib.ptr(img), _lib.ptr(grid), _lib.ptr(fws), st))
                 _lib.check(L.pf_sort_samples(h_, d, _lib.ptr(h), _lib.ptr(ys), _lib.ptr(self.weight_flat), st))
-                _lib.check(L.pf_interp_grad(h_, d, _lib.ptr(grid), _lib.ptr(ys), _lib.ptr(ds), st))
-            acc = ds if len(self.segments) == 1 else acc.add_(ds)
-        out = torch.empty_like(acc)
-        out[self.native.permutation()] = acc
-        # oversampled-grid axes (z, y, x) -> user axes (x, y, z): d c_os / d coord
-        dd = it.ndim
-        key = ("coord_scale", factor)
-        scale = self._cache.get(key)
-        if scale is None:
-            scale = torch.tensor(
-                [factor * it.os_shape[a] / it.shape[a] * it.unit_scale[dd - 1 - a] for a in range(dd)][::-1],
-                dtype=torch.float32,
-            ).to(self.device, non_blocking=True)
-            self._cache[key] = scale
-        return out[:, 3 - dd:].flip(1) * scale
+                _lib.check(L.pf_interp_grad(h_, d, _lib.ptr(grid), _lib.ptr(ys), scale, int(i > 0), _lib.ptr(out), st))
+        return out
 
     # ---- stage-by-stage execution (timing) ---------------------------------
     def _factor(self, norm_code: int, adjoint: bool) -> float:

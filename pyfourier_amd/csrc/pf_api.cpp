@@ -340,13 +340,16 @@ int pf_workspace_offsets(const pf_plan* p, const pf_batch_desc* d, size_t* grid_
 }
 
 int pf_interp_grad(const pf_plan* p, const pf_batch_desc* d, const void* grid, const void* h_sorted,
-                   float* dcoord_sorted, void* stream) {
+                   const float* axis_scale, int accumulate, float* dcoord, void* stream) {
     pf_batch b;
     int rc = make_batch(p, d, &b);
     if (rc != PF_OK) return rc;
-    if (!grid || !h_sorted || !dcoord_sorted) return pf_fail(PF_ERR_ARG, "null array");
-    return pf_launch_gather_dcoord(p, b, static_cast<const float2*>(grid), static_cast<const float2*>(h_sorted),
-                                   dcoord_sorted, static_cast<hipStream_t>(stream));
+    if (!grid || !h_sorted || !dcoord) return pf_fail(PF_ERR_ARG, "null array");
+    float sc[3] = {1.f, 1.f, 1.f};
+    if (axis_scale)
+        for (int a = 0; a < 3; ++a) sc[a] = axis_scale[a];
+    return pf_launch_gather_dcoord(p, b, static_cast<const float2*>(grid), static_cast<const float2*>(h_sorted), sc,
+                                   accumulate != 0, dcoord, static_cast<hipStream_t>(stream));
 }
 
 int pf_pad_fft(pf_plan* p, const pf_batch_desc* d, const void* image, void* grid, void* fft_workspace,

@@ -33,7 +33,21 @@ struct GradParams {
     int C, Cv, Cb, R, K, nfg, nt;
     const float2* basis;
     const float2* b0t;
+    int ndim;
+    int accumulate;
+    float scale[3];  // per stored axis (z, y, x)
 };
+
+// write one sample's three axis values at its original index, coordinate order
+__device__ __forceinline__ void write_dcoord(const GradParams& P, const int32_t* perm, int64_t s, const float* g,
+                                             float* dout) {
+    const int64_t j = perm[s];
+    for (int a = 3 - P.ndim; a < 3; ++a) {
+        float* o = dout + j * P.ndim + (2 - a);
+        const float v = P.scale[a] * g[a];
+        *o = P.accumulate ? *o + v : v;
+    }
+}
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -141,7 +155,7 @@ __global__ void __launch_bounds__(TB) k_gather_dcoord(GradParams P, const float*
         float sum[3] = {0.f, 0.f, 0.f};
         for (int q = 0; q < LS; ++q)
             for (int a = 0; a < 3; ++a) sum[a] += red[(threadIdx.x + q) * 3 + a];
-        for (int a = 0; a < 3; ++a) dout[s * 3 + a] = sum[a];
+        write_dcoord(P, perm, s, sum, dout);
     }
 }
 
@@ -283,7 +297,7 @@ __global__ void __launch_bounds__(TB) k_gather_dcoord_w(GradParams P, const floa
         float sum[3] = {0.f, 0.f, 0.f};
         for (int q = 0; q < LS; ++q)
             for (int a = 0; a < 3; ++a) sum[a] += red[(threadIdx.x + q) * 3 + a];
-        for (int a = 0; a < 3; ++a) dout[s * 3 + a] = sum[a];
+        write_dcoord(P, perm, s, sum, dout);
     }
 }
 
@@ -310,8 +324,8 @@ const GradLaunch DCOORD_W[2][2][PF_WMAX] = {
 
 }  // namespace
 
-int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* grid, const float2* h, float* out,
-                            hipStream_t s) {
+int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* grid, const float2* h, const float* scale,
+                            bool accumulate, float* out, hipStream_t s) {
     if (b.C > TB) return pf_fail(PF_ERR_UNSUPPORTED, "coordinate gradient supports at most 256 batch items");
     GradParams P;
     for (int a = 0; a < 3; ++a) {
@@ -333,6 +347,9 @@ int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* g
     P.nt = b.nt;
     P.basis = b.basis;
     P.b0t = b.b0t;
+    P.ndim = p->ndim;
+    P.accumulate = accumulate ? 1 : 0;
+    for (int a = 0; a < 3; ++a) P.scale[a] = scale[a];
     const int cpl = (b.Cv % 2 == 0 && b.C >= 2) ? 2 : 1;
     const int wy = p->width[1], wx = p->width[2];
     const bool idx32 = (int64_t)pf_gtot(p) * b.R * b.Cv < (1ll << 31);

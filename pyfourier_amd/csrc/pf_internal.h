@@ -166,5 +166,5 @@ void pf_plan_free(pf_plan* p);
 int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream);
 // register-spread tiles for ndim (3 ints each, z y x); returns how many exist
 int pf_spread_reg_tiles(int ndim, int* tiles, int cap);
-int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* grid, const float2* h, float* out,
-                            hipStream_t s);
+int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* grid, const float2* h, const float* scale,
+                            bool accumulate, float* out, hipStream_t s);

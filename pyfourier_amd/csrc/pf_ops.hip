@@ -520,15 +520,18 @@ __global__ void __launch_bounds__(TB) k_spread(GeoParams g, BatchParams B, int V
 }
 
 // Sum the partial tiles of a heavy tile in slot order and write it out.
-__global__ void __launch_bounds__(TB) k_spread_reduce(GeoParams g, BatchParams B, int VC,
+__global__ void __launch_bounds__(TB) k_spread_reduce(GeoParams g, BatchParams B, int VC, int nsub,
                                                       const int4* __restrict__ ritems,
                                                       const float2* __restrict__ partial,
                                                       float2* __restrict__ grid) {
+    // block = (split tile, value chunk, sub-range of TB tile values); each
+    // thread sums one value's partials in chunk order
     const int RC = B.R * B.Cv;
     const int V = B.Cb * RC;
     const int nvch = (V + VC - 1) / VC;
-    const int4 it = ritems[blockIdx.x / nvch];
-    const int vch = blockIdx.x % nvch;
+    const int iv = blockIdx.x / nsub;
+    const int4 it = ritems[iv / nvch];
+    const int vch = iv % nvch;
     const int vbase = vch * VC;
     const int vcnt = min(VC, V - vbase);
     const int64_t key = it.x;
@@ -541,12 +544,15 @@ __global__ void __launch_bounds__(TB) k_spread_reduce(GeoParams g, BatchParams B
     const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
     const int ex = min(Tx, g.os[2] - ox), ey = min(Ty, g.os[1] - oy), ez = min(Tz, g.os[0] - oz);
     const int tpts = Tx * Ty * Tz;
-    for (int i = threadIdx.x; i < tpts * vcnt; i += TB) {
+    {
+        const int i = (blockIdx.x % nsub) * TB + threadIdx.x;
+        if (i >= tpts * vcnt) return;
         const int v = i % vcnt;
         const int pt = i / vcnt;
         const int lx = pt % Tx, ly = (pt / Tx) % Ty, lz = pt / (Tx * Ty);
-        if (lx >= ex || ly >= ey || lz >= ez) continue;
+        if (lx >= ex || ly >= ey || lz >= ez) return;
         float2 a = make_float2(0.f, 0.f);
+#pragma unroll 4
         for (int sl = 0; sl < it.z; ++sl) {
             const float2* src = partial + ((int64_t)(it.y + sl) * nvch + vch) * (int64_t)tpts * VC;
             const float2 x = src[(int64_t)pt * VC + v];
@@ -1004,12 +1010,14 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     if (L.nslots > 0 && !partial) return pf_fail(PF_ERR_ARG, "spread needs a partial-tile workspace");
     const int64_t nb = (int64_t)L.nsitems * nvch;
     if (nb > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many spread blocks");
+    const int nsub = (int)(((int64_t)p->tile[0] * p->tile[1] * p->tile[2] * VC + TB - 1) / TB);
+    if ((int64_t)L.nritems * nvch * nsub > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many reduce blocks");
     if (const RegTile* rt = spread_reg_for(p, b)) {
         rt->fn(dim3((unsigned)nb), s, g, B, L, p, ys, grid, partial);
         PF_HIP(hipGetLastError());
         if (L.nritems > 0) {
-            hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch)), dim3(TB), 0, s, g, B,
-                               VC, L.ritems, partial, grid);
+            hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch * nsub)), dim3(TB), 0, s, g,
+                               B, VC, nsub, L.ritems, partial, grid);
             PF_HIP(hipGetLastError());
         }
         return PF_OK;
@@ -1024,8 +1032,8 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
                        p->perm, ys, grid, partial);
     PF_HIP(hipGetLastError());
     if (L.nritems > 0) {
-        hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch)), dim3(TB), 0, s, g, B, VC,
-                           L.ritems, partial, grid);
+        hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch * nsub)), dim3(TB), 0, s, g, B,
+                           VC, nsub, L.ritems, partial, grid);
         PF_HIP(hipGetLastError());
     }
     return PF_OK;
