@@ -508,6 +508,117 @@ __global__ void __launch_bounds__(256, PF_FFT_WPE) k_fftcol256(int N, int64_t I,
     }
 }
 
+// ---------------------------------------------------------------------------
+// X pass for G = 256 and one B0 segment (any subspace K and batch): the same
+// register four-step FFT, 16 columns = (image row, k*Cv + cv) per block.  The
+// image side is x-contiguous, the grid side has the columns innermost, so the
+// lane map differs between the stages: forward loads with the 16 threads of a
+// column on consecutive x and stores with consecutive columns; the adjoint
+// the other way round.  The transpose buffer uses a 17-entry row stride so
+// both maps are free of LDS bank conflicts.
+// ---------------------------------------------------------------------------
+constexpr int X256_ROW = 17;
+constexpr int X256_PLANE = 16 * X256_ROW + 4;
+
+template <bool FWD>
+__global__ void __launch_bounds__(256, PF_FFT_WPE) k_fftx256(Geo g, Batch bt, const float2* __restrict__ twg,
+                                                           const float* __restrict__ apod,
+                                                           const float2* __restrict__ in, float2* __restrict__ out,
+                                                           int KC, int YG, int nkc, float factor) {
+    constexpr int G = 256;
+    __shared__ float2 sh[16 * X256_PLANE];
+    __shared__ float2 tw[G];
+    const int N = g.N[2];
+    const int RC = bt.R * bt.Cv;
+    const int KCv = bt.K * bt.Cv;
+    const int64_t nrows = (int64_t)g.N[0] * g.N[1];
+    const int64_t rows_tot = (int64_t)bt.nfg * bt.Cb * nrows;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t row0 = (blk / nkc) * YG;
+    const int kc0 = (int)(blk % nkc) * KC;
+    const float* apz = apod;
+    const float* apy = apod + g.N[0];
+    const float* apx = apod + g.N[0] + g.N[1];
+    tw[threadIdx.x] = twg[threadIdx.x];
+    const int lo = N - N / 2, hi = G - N / 2;
+    // column b -> image row and (k, cv) pair
+    auto column = [&](int b, int64_t& row, int& kc) {
+        const int yy = b / KC;
+        kc = kc0 + b % KC;
+        row = row0 + yy;
+        return yy < YG && row < rows_tot && kc < KCv;
+    };
+    // image plane of (row, kc) and the row's deapodisation
+    auto image_row = [&](int64_t row, int kc, float& ayz) -> int64_t {
+        int fg, cb;
+        int64_t zy;
+        x_row_decode(bt, row, nrows, fg, cb, zy);
+        const int k = kc / bt.Cv, cv = kc % bt.Cv;
+        const int z = (int)(zy / g.N[1]), y = (int)(zy % g.N[1]);
+        ayz = apz[z] * apy[y];
+        const int ii = bt.nfg > 1 ? fg : (bt.nimg > 1 ? k : 0);
+        const int c = cb * bt.Cv + cv;
+        return ((int64_t)c * bt.nimg + ii) * g.ntot + zy * N;
+    };
+    float2 v[16];
+    {
+        const int t = FWD ? (threadIdx.x & 15) : (threadIdx.x >> 4);
+        const int b = FWD ? (threadIdx.x >> 4) : (threadIdx.x & 15);
+        int64_t row;
+        int kc;
+        const bool ok = column(b, row, kc);
+        if (FWD) {
+            float ayz = 0.f;
+            const int64_t base = ok ? image_row(row, kc, ayz) : 0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int n = t + 16 * k;
+                const int p = n < lo ? n + N / 2 : n - G + N / 2;
+                v[k] = (ok && (n < lo || n >= hi)) ? c_scale(in[base + p], ayz * apx[p]) : make_float2(0.f, 0.f);
+            }
+        } else {
+            const float2* src = in + (ok ? row : 0) * G * RC + kc;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const float2 x = ok ? src[(int64_t)(t + 16 * k) * RC] : make_float2(0.f, 0.f);
+                v[k] = make_float2(x.x, -x.y);
+            }
+        }
+        dft16(v);
+        __syncthreads();  // twiddle table
+#pragma unroll
+        for (int m1 = 1; m1 < 16; ++m1) v[m1] = c_mul(v[m1], tw[t * m1]);
+#pragma unroll
+        for (int m1 = 0; m1 < 16; ++m1) sh[m1 * X256_PLANE + t * X256_ROW + b] = v[m1];
+    }
+    __syncthreads();
+    const int m1 = FWD ? (threadIdx.x >> 4) : (threadIdx.x & 15);
+    const int b = FWD ? (threadIdx.x & 15) : (threadIdx.x >> 4);
+#pragma unroll
+    for (int tp = 0; tp < 16; ++tp) v[tp] = sh[m1 * X256_PLANE + tp * X256_ROW + b];
+    dft16(v);  // v[m2] = X[m1 + 16 m2] of column b
+    int64_t row;
+    int kc;
+    if (!column(b, row, kc)) return;
+    if (FWD) {
+        float2* dst = out + row * G * RC + kc;
+#pragma unroll
+        for (int m2 = 0; m2 < 16; ++m2) dst[(int64_t)(m1 + 16 * m2) * RC] = v[m2];
+    } else {
+        float ayz = 0.f;
+        float2* dst = out + image_row(row, kc, ayz);
+        const float sc = factor * ayz;
+#pragma unroll
+        for (int m2 = 0; m2 < 16; ++m2) {
+            const int n = m1 + 16 * m2;
+            if (n < lo || n >= hi) {
+                const int p = n < lo ? n + N / 2 : n - G + N / 2;
+                dst[p] = c_scale(make_float2(v[m2].x, -v[m2].y), sc * apx[p]);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -635,8 +746,26 @@ size_t pf_fftk_extra_bytes(const pf_plan* p, const pf_batch& b) {
 static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2* src, float2* dst, float factor,
                     hipStream_t s) {
     const pf_fftk_axis& X = p->fftk->ax[2];
-    AxisFFT A = axis_desc(X);
     const int KCv = b.K * b.Cv;
+    if (X.G == 256 && b.L == 1 && !b.b0s) {
+        const int KC = std::min(KCv, 16);
+        const int YG = 16 / KC;
+        const int nkc = (KCv + KC - 1) / KC;
+        const int64_t rows = (int64_t)b.nfg * b.Cb * p->shape[0] * p->shape[1];
+        const int64_t nblk = ((rows + YG - 1) / YG) * nkc;
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+        Geo g = geo_of(p);
+        Batch t = batch_of(b);
+        if (fwd)
+            hipLaunchKernelGGL(k_fftx256<true>, dim3((unsigned)nblk), dim3(256), 0, s, g, t, X.tw, p->apod, src, dst,
+                               KC, YG, nkc, 1.0f);
+        else
+            hipLaunchKernelGGL(k_fftx256<false>, dim3((unsigned)nblk), dim3(256), 0, s, g, t, X.tw, p->apod, src, dst,
+                               KC, YG, nkc, factor);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    AxisFFT A = axis_desc(X);
     if ((int64_t)b.L * X.G > GBMAX)
         return pf_fail(PF_ERR_UNSUPPORTED, "too many B0 segments for the fused FFT at this grid size");
     XBlk xb;

@@ -188,3 +188,27 @@ def test_small_grids_periodic_wrap(cuda_device, shape, oversamp):
     coord[0, 0] = -np.pi
     x = demo.complex_gaussian(rng, (3,) + shape)
     _check_pair(cuda_device, coord, shape, x, oversamp=oversamp)
+
+
+def test_subspace_and_frames_256_point_rows(cuda_device):
+    """x of 128 at sigma 2 runs the 256-point register row FFT, with a
+    subspace basis (image planes = coefficients) and with per-frame images."""
+    rng = np.random.default_rng(21)
+    shape = (12, 128)
+    F, K, C = 3, 2, 3
+    coord = (rng.uniform(-0.5, 0.5, (F, 2, 150, 2)) * np.array(shape[::-1])).astype(np.float32)
+    coord[0, 0, 0] = [-64, -6]
+    P = R.Plan(coord, shape, width=4, oversamp=2.0)
+    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=2.0)
+    basis = demo.t2_subspace_basis(rng, F, K)
+    x = demo.complex_gaussian(rng, (C, K) + shape)
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, basis=torch.from_numpy(basis),
+                 device=cuda_device)
+    assert rel(y.cpu().numpy().reshape(C, F, -1), R.nufft(x, P, basis=basis)) < PARITY
+    ya = demo.complex_gaussian(rng, (C, F) + coord.shape[1:-1])
+    xa = pf.nufft_adj(torch.from_numpy(ya).to(cuda_device), nufft_plan=plan, basis=torch.from_numpy(basis),
+                      device=cuda_device)
+    assert rel(xa.cpu().numpy(), R.nufft_adj(ya.reshape(C, F, -1), P, basis=basis)) < PARITY
+    xf = demo.complex_gaussian(rng, (C, F) + shape)
+    yf = pf.nufft(torch.from_numpy(xf).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    assert rel(yf.cpu().numpy().reshape(C, F, -1), R.nufft(xf, P)) < PARITY

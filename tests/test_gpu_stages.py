@@ -33,6 +33,8 @@ CASES = [
     ((128, 100), 2.0),     # 256 x 200: y is a 256-point column pass
     ((120, 96, 80), 2.0),  # 240 x 192 x 160 ... and 256-point z below
     ((128, 24, 20), 2.0),  # 256 x 48 x 40: z is a 256-point column pass
+    ((12, 20, 128), 2.0),  # 24 x 40 x 256: x is a 256-point row pass
+    ((10, 128), 2.0),      # 20 x 256
 ]
 
 
