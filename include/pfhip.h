@@ -246,7 +246,8 @@ typedef struct pf_toep pf_toep;
  *         uncentred FFT of the wrapped point-spread functions on the 2N grid
  *         (index order of an uncentred FFT), already scaled as the operator
  *         needs; copied into the object.
- * shape:  host (ndim,) image matrix N (z, y, x); 2N must factor into 2, 3, 5. */
+ * shape:  host (ndim,) image matrix N (z, y, x).  Embedding sizes 2^a 3^b 5^c
+ *         (<= 4096 per axis) run the pruned fused FFT passes, others rocFFT. */
 int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg, int ncoeff, void* stream,
                        pf_toep** out);
 int pf_toeplitz_destroy(pf_toep* op);
@@ -259,6 +260,59 @@ int pf_toeplitz_workspace_size(const pf_toep* op, int batch, size_t* bytes);
  * image, out: (batch, ncoeff, N...) complex64; b0_spatial: (nseg, N...) or NULL. */
 int pf_toeplitz_apply(pf_toep* op, int batch, const void* image, void* out, const void* b0_spatial,
                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* Dense Cartesian Gram operator: the same object on the N grid itself (no
+ * embedding), applied with pf_toeplitz_apply / _workspace_size / _destroy:
+ *   y = IFFT(T_f (x) FFT(Ct_l * x)) / prod(N), summed with conj(Ct_l).
+ * Replaces plan_gram / GramMatrix and _gram, src/pyfourier/_gram/_plan.py:14-177,
+ * src/pyfourier/_gram/_gram.py:11-133.
+ * kernel: (nframes, prod(N), R, R) complex64 device array in uncentred FFT
+ *         index order, R = nseg * ncoeff: per voxel the matrix
+ *         sum_t mask_t * conj(b_l'(x)) * b_l(x) * basis[t, k'] * conj(basis[t, k])
+ *         (the mask alone without basis and field map); copied.
+ * nframes: kernels per image frame (1 with a basis); the image is then
+ *         (batch, nframes, N...), or (batch, ncoeff, N...) with a basis.
+ * shape:  host (ndim,) N (z, y, x); sizes with prime factors other than 2, 3
+ *         and 5 use rocFFT. */
+int pf_gram_create(const void* kernel, int ndim, const int* shape, int nframes, int nseg, int ncoeff,
+                   void* stream, pf_toep** out);
+
+/* ---- Cartesian (masked) FFT ---------------------------------------------- */
+
+typedef struct pf_cart pf_cart;
+
+/* Build a Cartesian FFT plan on the current device.  Replaces plan_fft /
+ * FFTPlan, src/pyfourier/_fft/_plan.py:14-108 and
+ * src/pyfourier/_subroutines/_mask/_plan.py:14-89 (dense branch).
+ * mask:  (mframes, Nz*Ny*Nx) float32 device array of sampling weights in
+ *        centred k-space order (the reference's binary mask), or NULL for no
+ *        mask; copied.  mframes is 1 (shared by every frame) or the frame
+ *        count of the calls.
+ * shape: host (ndim,) matrix (z, y, x).  Sizes that factor into 2, 3, 5 use
+ *        the fused FFT passes, other sizes rocFFT. */
+int pf_cart_create(const float* mask, int mframes, int ndim, const int* shape, void* stream, pf_cart** out);
+int pf_cart_destroy(pf_cart* plan);
+
+/* Workspace bytes of pf_cart_fft / pf_cart_ifft for `desc` and nframes. */
+int pf_cart_workspace_size(const pf_cart* plan, const pf_batch_desc* desc, int nframes, size_t* bytes);
+
+/* kspace_t = factor * mask_t * sum_l b_l(x) sum_k conj(basis[t, k]) FFTc(Ct_l * X_k)
+ * Replaces _fft_fwd / _do_fft_fwd / _basis_adj, src/pyfourier/_fft/_fft.py:10-55,101-113,141-151
+ * (B0 segments added once, defect A.13 fixed) behind fftn(),
+ * src/pyfourier/_fft/_fwd.py:18-196.
+ * image (C, nimg, N...) with nimg = nframes (no basis) or ncoeff;
+ * kspace (C, nframes, N...) in centred order; desc->nt must equal Nx (the
+ * readout axis of b0_temporal); norm: PF_NORM_NONE (factor 1), _ORTHO
+ * (1/sqrt(prod N)), _ADJOINT (1/prod N). */
+int pf_cart_fft(pf_cart* plan, const pf_batch_desc* desc, int nframes, const void* image, void* kspace,
+                int norm, void* workspace, size_t workspace_bytes, void* stream);
+
+/* X_k = factor * sum_l conj(Ct_l) IFFTc(sum_t basis[t, k] conj(b_l(x)) mask_t kspace_t)
+ * Replaces _fft_adj / _do_fft_adj / _basis_fwd, src/pyfourier/_fft/_fft.py:58-98,116-138
+ * behind ifftn(), src/pyfourier/_fft/_adj.py:18-199.  norm: PF_NORM_NONE
+ * (1/prod N), _ORTHO (1/sqrt(prod N)), _ADJOINT (1). */
+int pf_cart_ifft(pf_cart* plan, const pf_batch_desc* desc, int nframes, const void* kspace, void* image,
+                 int norm, void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

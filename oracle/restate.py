@@ -28,6 +28,10 @@ reference repository root):
 * Pipe-Menon DCF       src/pyfourier/_dcomp/pipemenon.py:83-115
 * Toeplitz Gram        src/pyfourier/_sparsegram/_plan.py:108-249,
                        src/pyfourier/_sparsegram/_sparsegram.py:11-81
+* Cartesian FFT        src/pyfourier/_fft/_fft.py:10-151 (masked FFT with
+                       subspace basis and B0 segments along the readout)
+* dense Gram           src/pyfourier/_gram/_plan.py:124-160 (kernel),
+                       src/pyfourier/_gram/_gram.py:11-133 (apply), intent
 
 Arrays follow the reference conventions: coordinates ``(F, M, d)`` with the
 last axis ordered (x, y, z); grid/width/oversampling ordered (z, y, x).
@@ -564,3 +568,115 @@ def toeplitz_apply(x, T, shape):
     xp = resize(np.asarray(x, dtype=np.complex128), big)
     yp = fftc(T * fftc(xp, d, centered=False), d, inverse=True, centered=False)
     return resize(yp, list(shape))
+
+
+# --------------------------------------------------------------------------
+# Cartesian masked FFT: src/pyfourier/_fft/_fft.py:101-151 with the B0 loops
+# of :10-98 (each segment once, defect A.13 fixed).  x (B, F|K, *N) ->
+# y (B, F, *N), all in centred order.  Forward: FFTc, then _basis_adj
+# (y_t = sum_k conj(basis[t, k]) X_k, :141-151), then the mask; with
+# b0 = (Bt (L, Nx), Ct (L, *N)): y = sum_l Bt[l, kx] * fwd(Ct_l * x), the
+# temporal weight running along the readout (last) axis (_fft/_plan.py:83-96).
+# --------------------------------------------------------------------------
+def _mask_b(mask, ndim, F):
+    if mask is None:
+        return None
+    m = np.asarray(mask, dtype=np.float64)
+    m = m.reshape((-1,) + m.shape[-ndim:])
+    if m.shape[0] not in (1, F):
+        raise ValueError("mask frames must be 1 or F")
+    return m[None]
+
+
+def cart_fft(x, ndim, mask=None, basis=None, norm=None, b0=None):
+    """Masked centred FFT.  x (B, F|K, *N) -> y (B, F, *N)."""
+    x = np.asarray(x, dtype=np.complex128)
+    basis = _basis_c(basis)
+    F = basis.shape[0] if basis is not None else x.shape[1]
+    segs = [(None, None)] if b0 is None else list(zip(b0[0], b0[1]))
+    y = 0.0
+    for bt, ct in segs:
+        xs = x if ct is None else x * ct
+        k = fftc(xs, ndim, norm=norm)
+        if basis is not None:
+            k = np.einsum("tk,bk...->bt...", np.conj(basis), k)
+        if bt is not None:
+            k = k * np.asarray(bt)
+        y = y + k
+    m = _mask_b(mask, ndim, F)
+    return y if m is None else y * m
+
+
+def cart_ifft(y, ndim, mask=None, basis=None, norm=None, b0=None):
+    """Inverse of the above direction (_do_fft_adj, :116-138).  y (B, F, *N) -> x (B, F|K, *N)."""
+    y = np.asarray(y, dtype=np.complex128)
+    basis = _basis_c(basis)
+    m = _mask_b(mask, ndim, y.shape[1])
+    if m is not None:
+        y = y * m
+    segs = [(None, None)] if b0 is None else list(zip(b0[0], b0[1]))
+    x = 0.0
+    for bt, ct in segs:
+        ys = y if bt is None else y * np.conj(np.asarray(bt))
+        if basis is not None:
+            ys = np.einsum("tk,bt...->bk...", basis, ys)
+        xs = fftc(ys, ndim, norm=norm, inverse=True)
+        if ct is not None:
+            xs = xs * np.conj(ct)
+        x = x + xs
+    return x
+
+
+def cart_gram_kernel(shape, nframes=1, mask=None, basis=None, bt=None):
+    """Per-voxel Gram matrices in centred order, (nfr, *N, R, R) with
+    R = L*K: S[(l',k'), (l,k)] = sum_t m_t conj(b_l') b_l basis[t,k'] conj(basis[t,k])
+    (the intent of _gram/_plan.py:124-160: the mask once, outer products of
+    the basis over contrasts; b_l = Bt[l, x] along the readout)."""
+    d = len(shape)
+    basis = _basis_c(basis)
+    L = 1 if bt is None else np.asarray(bt).shape[0]
+    K = 1 if basis is None else basis.shape[1]
+    F = nframes if basis is None else basis.shape[0]
+    m = np.ones((1,) + tuple(shape)) if mask is None else _mask_b(mask, d, F)[0]
+    b = np.ones((L,) + (1,) * (d - 1) + (shape[-1],), dtype=np.complex128) if bt is None else \
+        np.asarray(bt, dtype=np.complex128).reshape((L,) + (1,) * (d - 1) + (shape[-1],))
+    bb = np.conj(b)[:, None] * b[None, :]  # (L', L, 1.., Nx)
+    if basis is None:
+        nfr = m.shape[0]
+        S = m[:, None, None] * bb[None]  # (nfr, L', L, *N)
+        S = np.moveaxis(S, (1, 2), (-2, -1))
+        return S
+    mt = np.broadcast_to(m, (F,) + tuple(shape))
+    BB = basis[:, :, None] * np.conj(basis)[:, None, :]  # (t, k', k)
+    Sk = np.einsum("t...,tpq->...pq", mt, BB)  # (*N, K', K)
+    S = np.einsum("ab...,...pq->...apbq", bb, Sk).reshape(tuple(shape) + (L * K, L * K))
+    return S[None]
+
+
+def cart_gram(x, shape, nframes=1, mask=None, basis=None, b0=None):
+    """Dense Gram (_gram/_gram.py:11-133, intent): per segment FFTc(Ct_l x_k),
+    the per-voxel matrix S, IFFTc, sum conj(Ct_l'); both FFTs orthonormal, so
+    the result equals ifftn(fftn(x)) for a binary mask under either norm."""
+    x = np.asarray(x, dtype=np.complex128)
+    d = len(shape)
+    bt, ct = (None, None) if b0 is None else b0
+    S = cart_gram_kernel(shape, nframes, mask, basis, bt)
+    L = 1 if ct is None else len(ct)
+    B, P = x.shape[:2]
+    K = 1 if basis is None else np.asarray(basis).shape[1]
+    u = np.stack([fftc(x if ct is None else x * ct[l], d, norm="ortho") for l in range(L)], axis=2)  # (B, P, L, *N)
+    if basis is None:
+        # (B, F, L, *N): per-frame L x L
+        Sf = np.broadcast_to(S, (P,) + S.shape[1:]) if S.shape[0] == 1 else S
+        v = np.einsum("f...pq,bfq...->bfp...", Sf, u)
+    else:
+        uk = u.transpose((0, 2, 1) + tuple(range(3, 3 + d))).reshape((B, L * K) + tuple(shape))
+        v = np.einsum("...pq,bq...->bp...", S[0], uk).reshape((B, L, K) + tuple(shape)).transpose(
+            (0, 2, 1) + tuple(range(3, 3 + d)))
+    out = 0.0
+    for l in range(L):
+        xs = fftc(v[:, :, l], d, norm="ortho", inverse=True)
+        if ct is not None:
+            xs = xs * np.conj(ct[l])
+        out = out + xs
+    return out

@@ -6,6 +6,9 @@ INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
     plan_nufft, nufft, nufft_adj      forward / adjoint NUFFT (+ subspace, B0)
     plan_spgram, sparse_gram          Toeplitz-embedded Gram operator
     pipe_menon_dcf                    iterative density compensation
+    plan_fft, fftn, ifftn             masked Cartesian FFT (+ subspace, B0)
+    plan_gram, gram                   dense Cartesian Gram operator
+    fft, ifft                         centred FFT utilities
 
 Beyond the reference: plan_nufft_batch / NUFFTBatch run an uneven list of
 problems (own matrix sizes and trajectories) as one HIP-graph replay;
@@ -24,6 +27,8 @@ from ._sparsegram import GramMatrix, plan_spgram, sparse_gram
 from ._batch import NUFFTBatch, plan_nufft_batch
 from ._dcomp import pipe_menon_dcf
 from ._tune import autotune_plan, spread_tiles
+from ._fft import FFTPlan, fft, fftn, ifft, ifftn, plan_fft
+from ._gram import gram, plan_gram
 
 __all__ = [
     "PFError",
@@ -41,6 +46,14 @@ __all__ = [
     "pipe_menon_dcf",
     "autotune_plan",
     "spread_tiles",
+    "FFTPlan",
+    "plan_fft",
+    "fftn",
+    "ifftn",
+    "fft",
+    "ifft",
+    "plan_gram",
+    "gram",
 ]
 
 __version__ = "0.1.0"

@@ -99,6 +99,12 @@ _SIG = {
     "pf_toeplitz_destroy": (_I, [_VP]),
     "pf_toeplitz_workspace_size": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_size_t)]),
     "pf_toeplitz_apply": (_I, [_VP, _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
+    "pf_gram_create": (_I, [_VP, _I, ctypes.POINTER(_I), _I, _I, _I, _VP, ctypes.POINTER(_VP)]),
+    "pf_cart_create": (_I, [_VP, _I, _I, ctypes.POINTER(_I), _VP, ctypes.POINTER(_VP)]),
+    "pf_cart_destroy": (_I, [_VP]),
+    "pf_cart_workspace_size": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, ctypes.POINTER(ctypes.c_size_t)]),
+    "pf_cart_fft": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, _VP, _VP, _I, _VP, ctypes.c_size_t, _VP]),
+    "pf_cart_ifft": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, _VP, _VP, _I, _VP, ctypes.c_size_t, _VP]),
     "pf_ifft_crop": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, ctypes.c_float, _VP, _VP]),
     "pf_sort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
     "pf_unsort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, ctypes.c_float, _VP]),

@@ -11,35 +11,42 @@
 // them one kernel applies the per-voxel R x R kernel (R = L segments x K
 // subspace coefficients; R = 1 is a pointwise multiply).  With B0 segments the
 // X pass multiplies by Ct_l on the way in and sums conj(Ct_l) on the way out.
+//
+// The same object, built on the N grid itself (oversampling 1, no embedding)
+// and with one kernel per frame, is the dense Cartesian Gram of
+// src/pyfourier/_gram/_gram.py:11-133 (pf_gram_create): there the kernel is
+// the sampling mask (or its per-voxel subspace/B0 matrix) in FFT order.
 #include <algorithm>
 #include <new>
 
 #include "pf_internal.h"
 
 struct pf_toep {
-    pf_plan* geo = nullptr;   // 2N embedding geometry (no samples)
-    float2* T = nullptr;      // (Gtot, R, R) kernel in stored FFT order
+    pf_plan* geo = nullptr;   // embedding geometry (no samples): 2N, or N for the dense Gram
+    float2* T = nullptr;      // (nfr, Gtot, R, R) kernel in stored FFT order
     int R = 1;
     int L = 1;
     int K = 1;
+    int nfr = 1;              // kernel frames: each image frame has its own kernel
 };
 
 namespace {
 
 constexpr int TB = 256;
 
-// grid (Cb, Gtot, R, Cv) in place: u[r] <- sum_r' T[pt, r, r'] u[r']
+// grid (nfr, Cb, Gtot, R, Cv) in place: u[r] <- sum_r' T[f, pt, r, r'] u[r']
 template <int RMAX>
 __global__ void __launch_bounds__(TB) k_toep_mul(const float2* __restrict__ T, float2* __restrict__ grid,
-                                                 int64_t gtot, int R, int Cv, int Cb) {
-    const int64_t total = (int64_t)Cb * gtot * Cv;
+                                                 int64_t gtot, int R, int Cv, int Cb, int nfr) {
+    const int64_t total = (int64_t)nfr * Cb * gtot * Cv;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int cv = (int)(e % Cv);
         const int64_t pt = (e / Cv) % gtot;
-        const int64_t cb = e / (Cv * gtot);
-        float2* g = grid + (cb * gtot + pt) * (int64_t)R * Cv + cv;
-        const float2* t = T + pt * (int64_t)R * R;
+        const int64_t fcb = e / (Cv * gtot);
+        const int64_t f = fcb / Cb;
+        float2* g = grid + (fcb * gtot + pt) * (int64_t)R * Cv + cv;
+        const float2* t = T + (f * gtot + pt) * (int64_t)R * R;
         if (RMAX == 1) {
             const float2 a = t[0], u = g[0];
             g[0] = make_float2(a.x * u.x - a.y * u.y, a.x * u.y + a.y * u.x);
@@ -78,8 +85,8 @@ pf_batch toep_batch(const pf_toep* t, int C, const float2* b0s) {
     b.L = t->L;
     b.K = t->K;
     b.R = t->R;
-    b.nfg = 1;
-    b.nimg = t->K;
+    b.nfg = t->nfr;
+    b.nimg = t->nfr > 1 ? t->nfr : t->K;
     b.basis = nullptr;
     b.b0t = nullptr;
     b.b0s = b0s;
@@ -89,16 +96,17 @@ pf_batch toep_batch(const pf_toep* t, int C, const float2* b0s) {
 
 size_t align_up(size_t n) { return (n + 255) / 256 * 256; }
 
-}  // namespace
+size_t extra_bytes(const pf_toep* t, const pf_batch& b) {
+    return pf_fftk_supported(t->geo) ? pf_fftk_extra_bytes(t->geo, b) : 0;
+}
 
-extern "C" {
-
-int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg, int ncoeff, void* stream,
-                       pf_toep** out) {
+int toep_create(const void* kernel, int ndim, const int* shape, int nfr, int nseg, int ncoeff, float sigma,
+                void* stream, pf_toep** out) {
     if (!out) return pf_fail(PF_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (!kernel || !shape) return pf_fail(PF_ERR_ARG, "null argument");
-    if (nseg < 1 || ncoeff < 1) return pf_fail(PF_ERR_ARG, "nseg and ncoeff must be >= 1");
+    if (nseg < 1 || ncoeff < 1 || nfr < 1) return pf_fail(PF_ERR_ARG, "nframes, nseg and ncoeff must be >= 1");
+    if (nfr > 1 && ncoeff > 1) return pf_fail(PF_ERR_ARG, "per-frame kernels and a subspace basis exclude each other");
     const int R = nseg * ncoeff;
     if (R > 16) return pf_fail(PF_ERR_UNSUPPORTED, "Toeplitz kernel blocks larger than 16x16");
     pf_toep* t = new (std::nothrow) pf_toep();
@@ -106,16 +114,15 @@ int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg,
     t->R = R;
     t->L = nseg;
     t->K = ncoeff;
+    t->nfr = nfr;
     t->geo = new (std::nothrow) pf_plan();
     int w[3] = {1, 1, 1};
-    float s[3] = {2.0f, 2.0f, 2.0f};
+    float s[3] = {sigma, sigma, sigma};
     int rc = t->geo ? pf_plan_params(t->geo, ndim, shape, w, s, 1, 1, nullptr, 0) : pf_fail(PF_ERR_ALLOC, "host allocation");
-    if (rc == PF_OK && !pf_fftk_supported(t->geo))
-        rc = pf_fail(PF_ERR_UNSUPPORTED, "Toeplitz embedding grid must factor into 2, 3 and 5");
     if (rc == PF_OK) rc = pf_plan_upload_apod(t->geo);
-    if (rc == PF_OK) rc = pf_fftk_init(t->geo);
+    if (rc == PF_OK && pf_fftk_supported(t->geo)) rc = pf_fftk_init(t->geo);
     if (rc == PF_OK) {
-        const size_t n = (size_t)pf_gtot(t->geo) * R * R * sizeof(float2);
+        const size_t n = (size_t)nfr * pf_gtot(t->geo) * R * R * sizeof(float2);
         hipError_t e = hipMalloc(&t->T, n);
         if (e == hipSuccess)
             e = hipMemcpyAsync(t->T, kernel, n, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream));
@@ -132,6 +139,20 @@ int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg,
     return PF_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg, int ncoeff, void* stream,
+                       pf_toep** out) {
+    return toep_create(kernel, ndim, shape, 1, nseg, ncoeff, 2.0f, stream, out);
+}
+
+int pf_gram_create(const void* kernel, int ndim, const int* shape, int nframes, int nseg, int ncoeff, void* stream,
+                   pf_toep** out) {
+    return toep_create(kernel, ndim, shape, nframes, nseg, ncoeff, 1.0f, stream, out);
+}
+
 int pf_toeplitz_destroy(pf_toep* t) {
     if (!t) return PF_OK;
     if (t->T) (void)hipFree(t->T);
@@ -144,7 +165,7 @@ int pf_toeplitz_workspace_size(const pf_toep* t, int batch, size_t* bytes) {
     if (!t || !bytes) return pf_fail(PF_ERR_ARG, "null argument");
     if (batch < 1) return pf_fail(PF_ERR_ARG, "batch must be >= 1");
     pf_batch b = toep_batch(t, batch, nullptr);
-    *bytes = align_up((size_t)b.C * b.R * pf_gtot(t->geo) * sizeof(float2)) + align_up(pf_fftk_extra_bytes(t->geo, b));
+    *bytes = align_up((size_t)b.nfg * b.C * b.R * pf_gtot(t->geo) * sizeof(float2)) + align_up(extra_bytes(t, b));
     return PF_OK;
 }
 
@@ -154,25 +175,36 @@ int pf_toeplitz_apply(pf_toep* t, int batch, const void* image, void* out, const
     if (batch < 1) return pf_fail(PF_ERR_ARG, "batch must be >= 1");
     if (t->L > 1 && !b0_spatial) return pf_fail(PF_ERR_ARG, "B0 segments need the spatial maps");
     pf_batch b = toep_batch(t, batch, static_cast<const float2*>(b0_spatial));
-    const size_t gb = align_up((size_t)b.C * b.R * pf_gtot(t->geo) * sizeof(float2));
-    const size_t need = gb + align_up(pf_fftk_extra_bytes(t->geo, b));
+    const size_t gb = align_up((size_t)b.nfg * b.C * b.R * pf_gtot(t->geo) * sizeof(float2));
+    const size_t need = gb + align_up(extra_bytes(t, b));
     if (!workspace || workspace_bytes < need) return pf_fail(PF_ERR_ARG, "workspace too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
     float2* grid = static_cast<float2*>(workspace);
     float2* extra = reinterpret_cast<float2*>(static_cast<char*>(workspace) + gb);
-    int rc = pf_fftk_forward(t->geo, b, static_cast<const float2*>(image), grid, extra, s);
+    const bool fused = pf_fftk_supported(t->geo);
+    int rc;
+    if (fused) {
+        rc = pf_fftk_forward(t->geo, b, static_cast<const float2*>(image), grid, extra, s);
+    } else {
+        // grid sizes with other prime factors: generic pad, rocFFT, crop
+        rc = pf_launch_deapod_pad(t->geo, b, static_cast<const float2*>(image), grid, s);
+        if (rc == PF_OK) rc = pf_fft_exec(t->geo, b, grid, 0, s);
+    }
     if (rc != PF_OK) return rc;
     const int64_t gtot = pf_gtot(t->geo);
-    const int64_t total = (int64_t)b.Cb * gtot * b.Cv;
+    const int64_t total = (int64_t)b.nfg * b.Cb * gtot * b.Cv;
     const unsigned nb = (unsigned)std::min<int64_t>((total + TB - 1) / TB, 1 << 20);
     if (t->R == 1)
-        hipLaunchKernelGGL(k_toep_mul<1>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, 1, b.Cv, b.Cb);
+        hipLaunchKernelGGL(k_toep_mul<1>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, 1, b.Cv, b.Cb, b.nfg);
     else if (t->R <= 4)
-        hipLaunchKernelGGL(k_toep_mul<4>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, t->R, b.Cv, b.Cb);
+        hipLaunchKernelGGL(k_toep_mul<4>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, t->R, b.Cv, b.Cb, b.nfg);
     else
-        hipLaunchKernelGGL(k_toep_mul<16>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, t->R, b.Cv, b.Cb);
+        hipLaunchKernelGGL(k_toep_mul<16>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, t->R, b.Cv, b.Cb, b.nfg);
     PF_HIP(hipGetLastError());
-    return pf_fftk_adjoint(t->geo, b, grid, static_cast<float2*>(out), extra, (float)(1.0 / (double)gtot), s);
+    const float factor = (float)(1.0 / (double)gtot);
+    if (fused) return pf_fftk_adjoint(t->geo, b, grid, static_cast<float2*>(out), extra, factor, s);
+    if ((rc = pf_fft_exec(t->geo, b, grid, 1, s)) != PF_OK) return rc;
+    return pf_launch_crop_deapod(t->geo, b, grid, static_cast<float2*>(out), factor, s);
 }
 
 }  // extern "C"

@@ -373,3 +373,106 @@ def test_pipe_menon_vs_exact_operator(niter):
         w = w / np.abs(y)
     got = R.pipe_menon_dcf(coord, N, niter=niter)
     assert rel(got.ravel(), w.ravel()) < 2e-3
+
+
+# ---------------------------------------------------------------------------
+# Cartesian masked FFT and dense Gram
+# ---------------------------------------------------------------------------
+def _cart_grid(shape):
+    """Centred Cartesian k-space points, (prod N, d) ordered (x, y[, z]) with x fastest."""
+    g = np.meshgrid(*[np.arange(n) - n // 2 for n in shape], indexing="ij")
+    return np.stack([a.ravel() for a in g[::-1]], axis=1).astype(np.float64)
+
+
+@pytest.mark.parametrize("shape", [(9,), (6, 7), (4, 5, 6)])
+def test_cart_fft_is_centred_dft_and_inverse(shape):
+    rng = np.random.default_rng(40)
+    x = demo.complex_gaussian(rng, (2, 1) + shape).astype(np.complex128)
+    y = R.cart_fft(x, len(shape))
+    direct = U.nudft(x[:, 0], _cart_grid(shape), list(shape))
+    np.testing.assert_allclose(y.reshape(2, -1), direct, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(R.cart_ifft(y, len(shape)), x, rtol=1e-12, atol=1e-12)
+
+
+def test_cart_fft_basis_is_conjugated_in_forward():
+    """_basis_adj multiplies by basis.conj().T (src/pyfourier/_fft/_fft.py:141-151):
+    with basis = i * I the forward returns -i * FFTc(X_t), the inverse +i."""
+    rng = np.random.default_rng(41)
+    x = demo.complex_gaussian(rng, (1, 3, 8, 6)).astype(np.complex128)
+    basis = 1j * np.eye(3)
+    y = R.cart_fft(x, 2, basis=basis)
+    np.testing.assert_allclose(y, -1j * R.fftc(x, 2), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(R.cart_ifft(y, 2, basis=basis), x, rtol=1e-12, atol=1e-12)
+
+
+def test_cart_fft_mask_frames_and_adjointness():
+    rng = np.random.default_rng(42)
+    T, K, shape = 5, 2, (6, 8)
+    mask = (rng.random((T,) + shape) < 0.4).astype(np.float32)
+    basis = demo.t2_subspace_basis(rng, T, K) * np.exp(1j * rng.random((T, K)))
+    x = demo.complex_gaussian(rng, (2, K) + shape)
+    y = R.cart_fft(x, 2, mask=mask, basis=basis, norm="ortho")
+    assert np.all(y[:, mask == 0] == 0)
+    w = demo.complex_gaussian(rng, (2, T) + shape)
+    lhs = np.vdot(y, w)
+    rhs = np.vdot(x, R.cart_ifft(w, 2, mask=mask, basis=basis, norm="ortho"))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    # plain frames with one shared mask: each frame is its own masked FFT
+    xf = demo.complex_gaussian(rng, (1, T) + shape).astype(np.complex128)
+    yf = R.cart_fft(xf, 2, mask=mask[:1])
+    for t in range(T):
+        np.testing.assert_allclose(yf[0, t], mask[0] * R.fftc(xf[0, t], 2), rtol=1e-12, atol=1e-12)
+
+
+def test_cart_fft_b0_exact_for_piecewise_constant_field_map():
+    """With a field map of L distinct values, Ct_l = indicator of value l and
+    Bt[l, t] = exp(-2 pi i f_l t) make the segmented model exact; it must then
+    equal the direct field-map signal with t running along the readout (x)."""
+    rng = np.random.default_rng(43)
+    shape = (6, 10)
+    freqs = np.array([-60.0, 0.0, 35.0, 90.0])
+    labels = rng.integers(0, len(freqs), size=shape)
+    zmap = freqs[labels]
+    t = np.linspace(0, 4e-3, shape[-1])
+    ct = np.stack([(labels == l).astype(np.complex128) for l in range(len(freqs))])
+    bt = np.exp(-2j * math.pi * freqs[:, None] * t[None, :])
+    x = demo.complex_gaussian(rng, (2, 1) + shape)
+    y = R.cart_fft(x, 2, b0=(bt, ct))
+    ye = U.nudft_b0(x[:, 0], _cart_grid(shape), shape, zmap, t)
+    np.testing.assert_allclose(y.reshape(2, -1), ye, rtol=1e-10, atol=1e-10)
+    # the inverse direction is the adjoint under ortho
+    w = demo.complex_gaussian(rng, (2, 1) + shape)
+    lhs = np.vdot(R.cart_fft(x, 2, b0=(bt, ct), norm="ortho"), w)
+    rhs = np.vdot(x, R.cart_ifft(w, 2, b0=(bt, ct), norm="ortho"))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("case", ["plain", "frames", "basis", "b0", "basis_b0"])
+def test_cart_gram_equals_inverse_of_forward(case):
+    """The kernel form (per-voxel matrices between two FFTs) equals
+    ifftn(fftn(x)) for a binary mask, whatever the norm."""
+    rng = np.random.default_rng(44)
+    shape = (8, 6)
+    T, K, L = 4, 3, 2
+    mask = (rng.random((T,) + shape) < 0.5).astype(np.float32)
+    basis = b0 = None
+    nimg = 1
+    if case == "plain":
+        mask = mask[:1]
+    if case == "frames":
+        nimg = T
+    if case in ("basis", "basis_b0"):
+        basis = demo.t2_subspace_basis(rng, T, K) * np.exp(1j * rng.random((T, K)))
+        nimg = K
+    if case in ("b0", "basis_b0"):
+        if case == "b0":
+            mask = mask[:1]
+        bt = np.exp(-2j * math.pi * rng.random((L, shape[-1])))
+        ct = np.exp(-2j * math.pi * rng.random((L,) + shape)) * rng.random((L,) + shape)
+        b0 = (bt, ct)
+    x = demo.complex_gaussian(rng, (2, nimg) + shape)
+    g = R.cart_gram(x, shape, nframes=nimg if basis is None else 1, mask=mask, basis=basis, b0=b0)
+    for norm in (None, "ortho"):
+        ref = R.cart_ifft(R.cart_fft(x, 2, mask=mask, basis=basis, norm=norm, b0=b0), 2, mask=mask, basis=basis,
+                          norm=norm, b0=b0)
+        assert rel(g, ref) < 1e-12
