@@ -34,7 +34,7 @@ struct pf_cart {
 namespace {
 
 constexpr int CT = 256;      // threads per block
-constexpr int CX = 64;       // x points per block
+constexpr int CXMAX = 256;   // x points per block (a whole row up to 256)
 constexpr int CLDS = 8192;   // LDS budget per block in float2 (64 KB)
 constexpr int CART_RMAX = 32;
 
@@ -47,7 +47,10 @@ struct CartBatch {
     int Cv, Cb, R, L, K, nfg, T, mframes;
     int cvb;   // coils per block (a divisor of Cv)
     int ncvb;  // Cv / cvb
+    int cx;    // x points per block
     int nxc;   // x runs per row
+    int tc;    // frames per block (sample kernel with one frame grid)
+    int ntc;   // frame chunks
 };
 
 __device__ __forceinline__ int centred_of(int s, int N) {
@@ -64,13 +67,15 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
 }
 
 struct CartBlock {
-    int fg, cb, x0, nx, cv0, ncv, zc, yc;
+    int fg, cb, x0, nx, cv0, ncv, zc, yc, t0, t1;
     int64_t zy;
 };
 
 __device__ __forceinline__ CartBlock cart_block(const CartGeo& g, const CartBatch& b) {
     CartBlock k;
     int64_t blk = blockIdx.x;
+    const int tq = (int)(blk % b.ntc);
+    blk /= b.ntc;
     const int xq = (int)(blk % b.nxc);
     blk /= b.nxc;
     const int cq = (int)(blk % b.ncvb);
@@ -80,12 +85,15 @@ __device__ __forceinline__ CartBlock cart_block(const CartGeo& g, const CartBatc
     blk /= nzy;
     k.cb = (int)(blk % b.Cb);
     k.fg = (int)(blk / b.Cb);
-    k.x0 = xq * CX;
-    k.nx = min(CX, g.N[2] - k.x0);
+    k.x0 = xq * b.cx;
+    k.nx = min(b.cx, g.N[2] - k.x0);
     k.cv0 = cq * b.cvb;
     k.ncv = b.cvb;
     k.zc = centred_of((int)(k.zy / g.N[1]), g.N[0]);
     k.yc = centred_of((int)(k.zy % g.N[1]), g.N[1]);
+    // frames of this block: the frame grid's own frame, or a chunk of all
+    k.t0 = b.nfg > 1 ? k.fg : tq * b.tc;
+    k.t1 = b.nfg > 1 ? k.fg + 1 : min(b.T, k.t0 + b.tc);
     return k;
 }
 
@@ -110,8 +118,7 @@ __global__ void __launch_bounds__(CT) k_cart_sample(CartGeo g, CartBatch b, cons
         u[xl * LD + r * b.cvb + cvl] = src[(int64_t)xl * RC + r * b.Cv + k.cv0 + cvl];
     }
     __syncthreads();
-    const int t0 = b.nfg > 1 ? k.fg : 0;
-    const int t1 = b.nfg > 1 ? k.fg + 1 : b.T;
+    const int t0 = k.t0, t1 = k.t1;
     for (int q = threadIdx.x; q < k.nx * k.ncv; q += CT) {
         const int xl = q % k.nx;
         const int cvl = q / k.nx;
@@ -127,6 +134,7 @@ __global__ void __launch_bounds__(CT) k_cart_sample(CartGeo g, CartBatch b, cons
                 if (b0t) v[r] = cmul(v[r], b0t[(int64_t)(r / b.K) * Nx + xc]);
             }
         }
+#pragma unroll 2
         for (int t = t0; t < t1; ++t) {
             float2 acc = make_float2(0.f, 0.f);
             if (basis) {
@@ -159,8 +167,7 @@ __global__ void __launch_bounds__(CT) k_cart_scatter(CartGeo g, CartBatch b, con
     const int LD = b.R * b.cvb + 1;
     const int RC = b.R * b.Cv;
     const int Nx = g.N[2];
-    const int t0 = b.nfg > 1 ? k.fg : 0;
-    const int t1 = b.nfg > 1 ? k.fg + 1 : b.T;
+    const int t0 = k.t0, t1 = k.t1;
     for (int q = threadIdx.x; q < k.nx * k.ncv; q += CT) {
         const int xl = q % k.nx;
         const int cvl = q / k.nx;
@@ -170,6 +177,7 @@ __global__ void __launch_bounds__(CT) k_cart_scatter(CartGeo g, CartBatch b, con
         float2 acc[RMAX];
 #pragma unroll
         for (int r = 0; r < RMAX; ++r) acc[r] = make_float2(0.f, 0.f);
+#pragma unroll 4
         for (int t = t0; t < t1; ++t) {
             float2 y = in[(c * b.T + t) * g.ntot + vox];
             if (mask) {
@@ -255,7 +263,7 @@ size_t cart_ws_bytes(const pf_cart* c, const pf_batch& b) {
     return align_up(cart_grid_bytes(c, b)) + align_up(extra);
 }
 
-CartBatch cart_kbatch(const pf_cart* c, const pf_batch& b, int nframes) {
+CartBatch cart_kbatch(const pf_cart* c, const pf_batch& b, int nframes, bool fwd) {
     CartBatch k;
     k.Cv = b.Cv;
     k.Cb = b.Cb;
@@ -265,9 +273,12 @@ CartBatch cart_kbatch(const pf_cart* c, const pf_batch& b, int nframes) {
     k.nfg = b.nfg;
     k.T = nframes;
     k.mframes = c->mframes;
-    // coils per block: the largest divisor of Cv whose (x run, R, coil) slab
-    // fits the LDS budget
-    const int lim = std::max(1, (CLDS / CX - 1) / b.R);
+    // x run: a whole row up to 256 points; coils per block: the largest
+    // divisor of Cv whose (x run, R, coil) slab fits the LDS budget
+    const int Nx = c->geo->shape[2];
+    k.cx = std::min(Nx, CXMAX);
+    while (k.cx > 1 && k.cx * (b.R + 1) > CLDS) k.cx /= 2;
+    const int lim = std::max(1, (CLDS / k.cx - 1) / b.R);
     k.cvb = 1;
     for (int v = std::min(b.Cv, lim); v >= 1; --v)
         if (b.Cv % v == 0) {
@@ -275,7 +286,18 @@ CartBatch cart_kbatch(const pf_cart* c, const pf_batch& b, int nframes) {
             break;
         }
     k.ncvb = b.Cv / k.cvb;
-    k.nxc = (c->geo->shape[2] + CX - 1) / CX;
+    k.nxc = (Nx + k.cx - 1) / k.cx;
+    // forward with one frame grid (basis) and many frames: split the frames
+    // over blocks until about 4096 blocks are in flight (each re-reads its
+    // small grid slab); the adjoint sums over frames inside one block
+    k.tc = nframes;
+    k.ntc = 1;
+    if (fwd && b.nfg == 1 && nframes > 1) {
+        const int64_t base = (int64_t)b.Cb * c->geo->shape[0] * c->geo->shape[1] * k.ncvb * k.nxc;
+        const int64_t want = std::min<int64_t>(nframes, (4096 + base - 1) / base);
+        k.tc = (int)((nframes + want - 1) / want);
+        k.ntc = (nframes + k.tc - 1) / k.tc;
+    }
     return k;
 }
 
@@ -297,11 +319,11 @@ int launch_cart(const pf_cart* c, const pf_batch& b, int nframes, bool fwd, cons
     CartGeo g;
     for (int a = 0; a < 3; ++a) g.N[a] = p->shape[a];
     g.ntot = pf_ntot(p);
-    CartBatch k = cart_kbatch(c, b, nframes);
-    const int64_t nblk = (int64_t)b.nfg * b.Cb * p->shape[0] * p->shape[1] * k.ncvb * k.nxc;
+    CartBatch k = cart_kbatch(c, b, nframes, fwd);
+    const int64_t nblk = (int64_t)b.nfg * b.Cb * p->shape[0] * p->shape[1] * k.ncvb * k.nxc * k.ntc;
     if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many Cartesian blocks");
     if (nblk < 1) return PF_OK;
-    const size_t lds = (size_t)std::min(CX, p->shape[2]) * (b.R * k.cvb + 1) * sizeof(float2);
+    const size_t lds = (size_t)k.cx * (b.R * k.cvb + 1) * sizeof(float2);
     const unsigned nb = (unsigned)nblk;
     if (b.R == 1)
         launch_pair<1>(fwd, nb, lds, s, g, k, grid_in, grid_out, ks_in, ks_out, c->mask, b.basis, b.b0t, factor);
