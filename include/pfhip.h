@@ -314,6 +314,34 @@ int pf_cart_fft(pf_cart* plan, const pf_batch_desc* desc, int nframes, const voi
 int pf_cart_ifft(pf_cart* plan, const pf_batch_desc* desc, int nframes, const void* kspace, void* image,
                  int norm, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- GRAPPA operator gridding (GROG) ---------------------------------- */
+
+/* Per-sample GROG bookkeeping.  Replaces the index / shift tables of
+ * grog_interp, src/pyfourier/_preinterp/_grog.py:176-218.
+ * coord: (npts, ndim) float32 grid units, last axis (x, y[, z]);
+ * shape: host (ndim,) matrix (z, y, x); nsteps: odd quantisation levels.
+ * lut:   (npts) int32: packed steps sx + sy*nsteps [+ sz*nsteps^2] with
+ *        s_a = floor((nsteps-1) * (round(c_a) - c_a)) + nsteps/2, or -1 for a
+ *        sample whose nearest grid point lies outside the matrix;
+ * index: (npts, ndim) int32 round(c) + N/2, clamped into [0, N);
+ * flat:  (npts) int64 sum_a index_a * prod_{b<a} N_b before clamping (the
+ *        reference's key for counting repeated grid points). */
+int pf_grog_index(const float* coord, int ndim, const int* shape, long long npts, int nsteps, int* lut, int* index,
+                  long long* flat, void* stream);
+
+/* Move every sample's coil vector onto its grid point:
+ *   out[b, :, s] = in[b, :, s] @ T[0, s_x] @ T[1, s_y] [@ T[2, s_z]]
+ * (zero where lut[s] = -1).  Replaces do_interpolation / _interp,
+ * src/pyfourier/_preinterp/_grog.py:354-425, with the product operator of
+ * :140-174 applied factor by factor.
+ * in, out: (batch, ncoil, npts) complex64, not aliased; ncoil <= 64.
+ * tables:  (ndim, nsteps, table_batch, ncoil, ncoil) complex64: per axis
+ *          (x, y, z) and step the fractional operator power, row-vector
+ *          convention (target = source @ G); table_batch is 1 (shared) or
+ *          batch (one operator set per batch entry, e.g. per slice). */
+int pf_grog_apply(const void* in, void* out, int batch, int ncoil, long long npts, int ndim, int nsteps,
+                  const void* tables, int table_batch, const int* lut, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,10 @@ reference repository root):
                        subspace basis and B0 segments along the readout)
 * dense Gram           src/pyfourier/_gram/_plan.py:124-160 (kernel),
                        src/pyfourier/_gram/_gram.py:11-133 (apply), intent
+* sparse FFT           src/pyfourier/_sparsefft/_sparsefft.py:11-144 with the
+                       pick / zero-fill of src/pyfourier/_subroutines/_mask/
+* GROG                 src/pyfourier/_preinterp/_grog.py:22-425 (intent: the
+                       row-vector operator product, one shift per sample)
 
 Arrays follow the reference conventions: coordinates ``(F, M, d)`` with the
 last axis ordered (x, y, z); grid/width/oversampling ordered (z, y, x).
@@ -680,3 +684,161 @@ def cart_gram(x, shape, nframes=1, mask=None, basis=None, b0=None):
             xs = xs * np.conj(ct[l])
         out = out + xs
     return out
+
+
+# --------------------------------------------------------------------------
+# Sparse FFT: centred FFT, then the sampled points picked
+# (_dense2sparse/_2D.py:13-46: y[f, b, j] = X[f, b, idx_f[j]]; with a basis
+# y[f] = sum_k basis[f, k] X_k[idx_f], _2D_subspace.py:13-54); the inverse
+# zero-fills with repeated points summed and conj(basis), then inverse FFT.
+# indexes (F, M, d) integer grid positions in [0, N), last axis (x, y, z).
+# b0 = (Bt (L, nsamples), Ct (L, *N)) with the readout index j % nsamples.
+# --------------------------------------------------------------------------
+def _flat_index(indexes, shape):
+    idx = np.asarray(indexes, dtype=np.int64)
+    d = len(shape)
+    flat = np.zeros(idx.shape[:-1], dtype=np.int64)
+    for a in range(d):  # coordinate axis a (x, y, z) is grid axis d-1-a
+        flat = flat * shape[a] + idx[..., d - 1 - a]
+    return flat
+
+
+def sparse_fft(x, shape, indexes, basis=None, weight=None, norm=None, b0=None):
+    """x (B, F|K, *N) -> y (B, F, M)."""
+    x = np.asarray(x, dtype=np.complex128)
+    d = len(shape)
+    flat = _flat_index(indexes, shape)  # (F, M)
+    F, M = flat.shape
+    basis = _basis_c(basis)
+    B = x.shape[0]
+    segs = [(None, None)] if b0 is None else list(zip(b0[0], b0[1]))
+    y = np.zeros((B, F, M), dtype=np.complex128)
+    for bt, ct in segs:
+        xs = x if ct is None else x * ct
+        k = fftc(xs, d, norm=norm).reshape(B, xs.shape[1], -1)
+        if basis is None:
+            ys = np.stack([k[:, f if k.shape[1] > 1 else 0, flat[f]] for f in range(F)], axis=1)
+        else:
+            ys = np.stack([sum(basis[f, q] * k[:, q, flat[f]] for q in range(basis.shape[1])) for f in range(F)],
+                          axis=1)
+        if bt is not None:
+            t = np.asarray(bt).reshape(-1)
+            ys = ys * np.tile(t, M // t.size)[None, None]
+        y += ys
+    if weight is not None:
+        y = y * np.asarray(weight).reshape(1, F, -1)
+    return y
+
+
+def sparse_ifft(y, shape, indexes, basis=None, weight=None, norm=None, b0=None):
+    """y (B, F, M) -> x (B, F|K, *N)."""
+    y = np.asarray(y, dtype=np.complex128)
+    d = len(shape)
+    flat = _flat_index(indexes, shape)
+    F, M = flat.shape
+    B = y.shape[0]
+    if weight is not None:
+        y = y * np.asarray(weight).reshape(1, F, -1)
+    basis = _basis_c(basis)
+    P = F if basis is None else basis.shape[1]
+    ntot = int(np.prod(shape))
+    segs = [(None, None)] if b0 is None else list(zip(b0[0], b0[1]))
+    x = 0.0
+    for bt, ct in segs:
+        ys = y if bt is None else y * np.conj(np.tile(np.asarray(bt).reshape(-1), M // np.asarray(bt).size))[None, None]
+        g = np.zeros((B, P, ntot), dtype=np.complex128)
+        for f in range(F):
+            if basis is None:
+                np.add.at(g[:, f], (slice(None), flat[f]), ys[:, f])
+            else:
+                for q in range(P):
+                    np.add.at(g[:, q], (slice(None), flat[f]), np.conj(basis[f, q]) * ys[:, f])
+        xs = fftc(g.reshape((B, P) + tuple(shape)), d, norm=norm, inverse=True)
+        if ct is not None:
+            xs = xs * np.conj(ct)
+        x = x + xs
+    return x
+
+
+# --------------------------------------------------------------------------
+# GROG: operators trained by regularised least squares on shifted
+# calibration patches (:273-340), fractional powers by eigendecomposition
+# (:344-367), the product table G[(sz, sy, sx)] = Gx^dx Gy^dy Gz^dz
+# (:140-174) indexed by the quantised shift of each sample (:176-184), and
+# every sample's coil row vector times its operator (the intent of
+# :354-425).  Indexes round(c) + N//2 (x, y, z), weights 1/count of the
+# flattened index (:188-218), samples off the matrix zeroed and their
+# indexes clamped (:227-236).
+# --------------------------------------------------------------------------
+def grog_train(calib, ndim, lamda):
+    c = np.moveaxis(np.asarray(calib, dtype=np.complex128), 0, -1)
+    if c.ndim == 3:
+        c = c[None]
+    nc = c.shape[-1]
+
+    def op(src, dst):
+        S = src.reshape(src.shape[0], -1, nc) if ndim == 2 else src.reshape(1, -1, nc)
+        T = dst.reshape(S.shape)
+        out = []
+        for s_, t_ in zip(S, T):
+            Sh = s_.conj().T
+            lam0 = lamda * np.sqrt((np.abs(Sh) ** 2).sum()) / nc
+            out.append(np.linalg.solve(Sh @ s_ + lam0 * np.eye(nc), Sh @ t_))
+        return np.stack(out)
+
+    gx = op(c[:, :, :-1], c[:, :, 1:])
+    gy = op(c[:, :-1], c[:, 1:])
+    if ndim == 2:
+        return [gx, gy]
+    return [gx, gy, op(c[:-1], c[1:])]
+
+
+def grog_power(G, delta):
+    w, V = np.linalg.eig(G)
+    return V @ np.diag(w.astype(np.complex128) ** delta) @ np.linalg.inv(V)
+
+
+def grog_lut(coord, nsteps):
+    """Per-axis step indexes (float32 arithmetic like the reference)."""
+    c = np.asarray(coord, dtype=np.float32)
+    d = np.round(c).astype(np.float32) - c  # numpy rounds half to even
+    return (np.floor(np.float32(nsteps - 1) * d).astype(np.int64) + nsteps // 2)
+
+
+def grog_interp(x, calib, coord, shape, lamda=0.01, nsteps=11):
+    """x (B, nc, M), coord (M, d) grid units (x, y, z), shape (z, y, x)."""
+    x = np.asarray(x, dtype=np.complex128)
+    coord = np.asarray(coord, dtype=np.float32).reshape(-1, np.asarray(coord).shape[-1])
+    d = coord.shape[-1]
+    cal = np.asarray(calib, dtype=np.complex128)
+    cal = cal / np.linalg.norm(cal)
+    nsteps = 2 * (nsteps // 2) + 1
+    ops = grog_train(cal, d, lamda)
+    deltas = (np.arange(nsteps) - (nsteps - 1) // 2) / (nsteps - 1)
+    nob = ops[0].shape[0]
+    pw = [[[grog_power(g[o], dl) for dl in deltas] for o in range(nob)] for g in ops]  # [axis][slice][step]
+    steps = grog_lut(coord, nsteps)  # (M, d)
+    B, nc, M = x.shape
+    out = np.zeros_like(x)
+    N = list(shape)[::-1]  # (x, y, z)
+    idx = (np.round(coord).astype(np.int64) + np.asarray(N) // 2)
+    for b in range(B):
+        o = b if nob > 1 else 0
+        for s_ in range(M):
+            G = np.eye(nc, dtype=np.complex128)
+            for a in range(d):
+                G = G @ pw[a][o][steps[s_, a]]
+            out[b, :, s_] = x[b, :, s_] @ G
+    flat = np.zeros(M, dtype=np.int64)
+    unfold = 1
+    for a in range(d):
+        flat += unfold * idx[:, a]
+        unfold *= N[a]
+    _, inv, cnt = np.unique(flat, return_inverse=True, return_counts=True)
+    weights = 1.0 / cnt[inv].astype(np.float32)
+    outside = np.zeros(M, dtype=bool)
+    for a in range(d):
+        outside |= (idx[:, a] < 0) | (idx[:, a] >= N[a])
+        idx[:, a] = np.clip(idx[:, a], 0, N[a] - 1)
+    out[:, :, outside] = 0
+    return out, idx, weights

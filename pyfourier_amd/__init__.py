@@ -9,6 +9,8 @@ INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
     plan_fft, fftn, ifftn             masked Cartesian FFT (+ subspace, B0)
     plan_gram, gram                   dense Cartesian Gram operator
     fft, ifft                         centred FFT utilities
+    plan_spfft, sparse_fftn, sparse_ifftn  sparse (Cartesian-sampled) FFT
+    grog_interp                       GRAPPA operator gridding
 
 Beyond the reference: plan_nufft_batch / NUFFTBatch run an uneven list of
 problems (own matrix sizes and trajectories) as one HIP-graph replay;
@@ -29,6 +31,8 @@ from ._dcomp import pipe_menon_dcf
 from ._tune import autotune_plan, spread_tiles
 from ._fft import FFTPlan, fft, fftn, ifft, ifftn, plan_fft
 from ._gram import gram, plan_gram
+from ._sparsefft import plan_spfft, sparse_fftn, sparse_ifftn
+from ._grog import grog_interp
 
 __all__ = [
     "PFError",
@@ -54,6 +58,10 @@ __all__ = [
     "ifft",
     "plan_gram",
     "gram",
+    "plan_spfft",
+    "sparse_fftn",
+    "sparse_ifftn",
+    "grog_interp",
 ]
 
 __version__ = "0.1.0"

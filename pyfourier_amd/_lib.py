@@ -105,6 +105,8 @@ _SIG = {
     "pf_cart_workspace_size": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, ctypes.POINTER(ctypes.c_size_t)]),
     "pf_cart_fft": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, _VP, _VP, _I, _VP, ctypes.c_size_t, _VP]),
     "pf_cart_ifft": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, _VP, _VP, _I, _VP, ctypes.c_size_t, _VP]),
+    "pf_grog_index": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.c_longlong, _I, _VP, _VP, _VP, _VP]),
+    "pf_grog_apply": (_I, [_VP, _VP, _I, _I, ctypes.c_longlong, _I, _I, _VP, _I, _VP, _VP]),
     "pf_ifft_crop": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, ctypes.c_float, _VP, _VP]),
     "pf_sort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
     "pf_unsort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, ctypes.c_float, _VP]),

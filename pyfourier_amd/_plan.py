@@ -224,11 +224,13 @@ def plan_nufft(
     L_batch_size=None,
     tile=None,
     chunk=0,
+    grid_units=False,
 ):
     """Precompute a NUFFT plan (same signature as the reference plan_nufft).
 
     coord: (ncontrasts, nviews, nsamples, ndim) in any unit, last axis (x, y[, z]);
     shape: image matrix (scalar or (z, y, x)); width/oversamp: scalar or per axis (z, y, x).
+    grid_units (extension): skip the unit detection, coord is in grid units.
     tile, chunk (extensions): the sort/spread tile (z, y, x) and the spread
     work-item size, e.g. from pyfourier_amd.autotune_plan; None/0 = defaults.
     """
@@ -243,8 +245,13 @@ def plan_nufft(
     width = _per_axis(width, ndim, np.int16)
     oversamp = _per_axis(oversamp, ndim, np.float32)
     shape = _per_axis(shape, ndim, np.int64)
-    units = unit_factors(coord, shape[::-1].tolist())
-    coord = normalize_coordinates(coord, shape[::-1].tolist())
+    if grid_units:
+        # caller guarantees grid units (e.g. the sparse FFT's integer indexes)
+        units = (1.0,) * ndim
+        coord = coord.to(torch.float32)
+    else:
+        units = unit_factors(coord, shape[::-1].tolist())
+        coord = normalize_coordinates(coord, shape[::-1].tolist())
     # Cartesian axes get oversamp 1 and width 1 (src/pyfourier/_nufft/_plan.py:139-146)
     is_cart = np.asarray(
         [bool(torch.allclose(coord[..., a], torch.round(coord[..., a]))) for a in range(ndim)][::-1]

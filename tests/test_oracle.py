@@ -476,3 +476,129 @@ def test_cart_gram_equals_inverse_of_forward(case):
         ref = R.cart_ifft(R.cart_fft(x, 2, mask=mask, basis=basis, norm=norm, b0=b0), 2, mask=mask, basis=basis,
                           norm=norm, b0=b0)
         assert rel(g, ref) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# sparse FFT and GROG
+# ---------------------------------------------------------------------------
+def test_sparse_fft_full_grid_is_centred_dft_and_inverse():
+    rng = np.random.default_rng(50)
+    shape = (6, 8)
+    g = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    idx = np.stack([g[1].ravel(), g[0].ravel()], -1)[None]  # (1, M, 2) x, y; every grid point once
+    x = demo.complex_gaussian(rng, (2, 1) + shape).astype(np.complex128)
+    y = R.sparse_fft(x, shape, idx)
+    np.testing.assert_allclose(y.reshape((2, 1) + shape), R.fftc(x, 2), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(R.sparse_ifft(y, shape, idx), x, rtol=1e-12, atol=1e-12)
+
+
+def test_sparse_fft_duplicates_sum_and_adjoint():
+    rng = np.random.default_rng(51)
+    shape = (5, 7)
+    F, K = 3, 2
+    idx = np.stack([rng.integers(0, 7, (F, 40)), rng.integers(0, 5, (F, 40))], -1)
+    idx[:, 1] = idx[:, 0]  # a repeated point in every frame
+    basis = demo.t2_subspace_basis(rng, F, K) * np.exp(1j * rng.random((F, K)))
+    x = demo.complex_gaussian(rng, (2, K) + shape)
+    w = demo.complex_gaussian(rng, (2, F, 40))
+    y = R.sparse_fft(x, shape, idx, basis=basis, norm="ortho")
+    lhs = np.vdot(y, w)
+    rhs = np.vdot(x, R.sparse_ifft(w, shape, idx, basis=basis, norm="ortho"))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    # the repeated point is gathered twice and zero-filled as the sum of both
+    np.testing.assert_allclose(y[:, :, 0], y[:, :, 1], rtol=1e-12, atol=1e-12)
+    e = np.zeros((1, 1, 40), dtype=np.complex128)
+    e[0, 0, 0] = e[0, 0, 1] = 1.0
+    g = R.sparse_ifft(e, shape, idx[:1], norm="ortho")
+    k = R.fftc(g, 2, norm="ortho")
+    iy, ix = idx[0, 0, 1], idx[0, 0, 0]
+    assert abs(k[0, 0, iy, ix] - 2.0) < 1e-12
+
+
+def _shift_model(rng, nc, ndim):
+    """Commuting shift operators A_a = P diag(exp(i th_a)) P^-1 and calibration
+    data generated exactly by them: c[..., y, x] = v @ Ay^y @ Ax^x."""
+    P = np.eye(nc) + 0.3 * (rng.standard_normal((nc, nc)) + 1j * rng.standard_normal((nc, nc)))
+    Pi = np.linalg.inv(P)
+    th = [rng.uniform(-0.6, 0.6, nc) for _ in range(ndim)]
+    A = [P @ np.diag(np.exp(1j * t)) @ Pi for t in th]
+
+    def pw(a, e):
+        return P @ np.diag(np.exp(1j * th[a] * e)) @ Pi
+
+    return A, pw
+
+
+def test_grog_training_recovers_exact_shift_operators():
+    rng = np.random.default_rng(52)
+    nc, n = 6, 10
+    A, pw = _shift_model(rng, nc, 2)
+    v = rng.standard_normal((3, nc)) + 1j * rng.standard_normal((3, nc))  # several sources for full rank
+    cal = np.zeros((nc, 3, n, n), dtype=np.complex128)  # (nc, slices, y, x)
+    for z in range(3):
+        for yy in range(n):
+            for xx in range(n):
+                cal[:, z, yy, xx] = v[z] @ pw(1, yy) @ pw(0, xx)
+    gx, gy = R.grog_train(cal, 2, 0.0)
+    for z in range(3):
+        assert rel(gx[z], A[0]) < 1e-8
+        assert rel(gy[z], A[1]) < 1e-8
+
+
+def test_grog_fractional_power_vs_schur_pade():
+    from scipy.linalg import fractional_matrix_power
+
+    rng = np.random.default_rng(53)
+    A, _ = _shift_model(rng, 5, 1)
+    for dl in (-0.5, -0.2, 0.1, 0.4):
+        np.testing.assert_allclose(R.grog_power(A[0], dl), fractional_matrix_power(A[0], dl), rtol=1e-9,
+                                   atol=1e-9)
+
+
+def test_grog_lut_known_answers():
+    # shift d = round(c) - c, step = floor(10 d) + 5 (nsteps 11); round half to even
+    c = np.array([[0.26], [-1.24], [2.5], [3.5], [0.0], [-0.05]], dtype=np.float32)
+    np.testing.assert_array_equal(R.grog_lut(c, 11)[:, 0], [2, 7, 0, 10, 5, 5])
+
+
+def test_grog_moves_model_samples_by_the_quantised_shift():
+    """On data generated by the shift model, a sample at c comes out as the
+    model value at c + delta(step): the row-vector product with the x
+    operator for the x shift and the y operator for the y shift."""
+    from scipy.linalg import fractional_matrix_power
+
+    rng = np.random.default_rng(54)
+    nc, n = 4, 12
+    A, pw = _shift_model(rng, nc, 2)
+    v = rng.standard_normal((4, nc)) + 1j * rng.standard_normal((4, nc))
+    cal = np.zeros((nc, n, n), dtype=np.complex128)
+    for yy in range(n):
+        for xx in range(n):
+            cal[:, yy, xx] = sum(v[q] @ pw(1, yy + 0.37 * q) @ pw(0, xx - 0.21 * q) for q in range(4))
+    M = 30
+    coord = rng.uniform(-5.4, 5.4, (M, 2)).astype(np.float32)
+    u = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+    x = np.stack([u @ fractional_matrix_power(A[0], float(cx)) @ fractional_matrix_power(A[1], float(cy))
+                  for cx, cy in coord.astype(np.float64)], -1)[None]  # (1, nc, M)
+    out, idx, w = R.grog_interp(x, cal, coord, (n, n), lamda=0.0, nsteps=11)
+    steps = R.grog_lut(coord, 11)
+    delta = (steps - 5) / 10.0
+    want = np.stack([u @ fractional_matrix_power(A[0], float(coord[s, 0]) + delta[s, 0])
+                     @ fractional_matrix_power(A[1], float(coord[s, 1]) + delta[s, 1]) for s in range(M)], -1)
+    assert rel(out[0], want) < 1e-6
+    np.testing.assert_array_equal(idx, np.round(coord).astype(np.int64) + n // 2)
+
+
+def test_grog_indexes_weights_and_outside_samples():
+    rng = np.random.default_rng(55)
+    nc, n = 3, 8
+    cal = demo.complex_gaussian(rng, (nc, n, n))
+    coord = np.array([[0.1, 0.2], [-0.2, 0.1], [3.6, 0.0], [-4.4, 1.0], [1.0, -1.0], [0.9, -1.2]], np.float32)
+    x = demo.complex_gaussian(rng, (2, nc, 6))
+    out, idx, w = R.grog_interp(x, cal, coord, (n, n))
+    # samples 0, 1 share (4, 4); 4, 5 share (5, 3); 2 rounds to x = 8 (outside), 3 to x = 0 (inside).
+    # The reference counts repeated points by the flattened index x + 8 y computed before clamping,
+    # so the outside point (8, 4) and (0, 5) share the key 40 and count as a pair.
+    np.testing.assert_array_equal(idx, [[4, 4], [4, 4], [7, 4], [0, 5], [5, 3], [5, 3]])
+    np.testing.assert_allclose(w, [0.5, 0.5, 0.5, 0.5, 0.5, 0.5])
+    assert np.all(out[:, :, 2] == 0) and np.all(out[:, :, 3] != 0)
