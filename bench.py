@@ -371,6 +371,96 @@ def bench_cartesian(pf, dev, steps):
     return out
 
 
+def bench_grog_sparse(pf, dev, steps):
+    """GROG of a 16-slice 2-D radial acquisition (402 spokes x 512 samples,
+    32 coils, per-slice operators from 24^2 calibration patches, 11 shift
+    steps) onto a 256^2 grid, then the sparse FFT of the gridded points
+    (16 x 32 images, fwd+adj).  grog_ms: the whole call (host operator
+    training and fractional powers included); apply_ms: the native
+    coil-operator kernel alone on the same data (HIP events), priced against
+    HBM (data in + out once) and FP32 (8 flops per complex MAC, ndim nc^2
+    MACs per sample).  The sparse FFT is compared with torch.fft plus an
+    index gather / index_add zero-fill."""
+    import ctypes
+
+    import torch
+
+    from pyfourier_amd import _grog, _lib, demo
+
+    rng = np.random.default_rng(9)
+    S, nc, nv, ns, N = 16, 32, 402, 512, 256
+    coord = (demo.golden_angle_radial_2d(nv, ns)[None] * (N / (2 * math.pi))).astype(np.float32)  # grid units
+    calib = torch.randn((nc, S, 24, 24), dtype=torch.complex64)
+    x = torch.randn((S, nc, 1, nv, ns), dtype=torch.complex64, device=dev)
+    out = {}
+    t_grog = _time(lambda: pf.grog_interp(x, calib, coord, N), dev, max(2, steps // 2), warmup=1)
+    y, idx, w = pf.grog_interp(x, calib, coord, N)
+
+    # the apply kernel alone
+    npts = nv * ns
+    cal = calib.numpy().astype(np.complex128)
+    cal = cal / np.linalg.norm(cal)
+    ops = _grog.train_operators(cal, 2, 0.01)
+    deltas = (np.arange(11) - 5) / 10
+    tab = torch.from_numpy(np.stack([_grog.operator_powers(g, deltas) for g in ops]).astype(np.complex64)).to(dev)
+    L = _lib.lib()
+    cd = torch.from_numpy(coord.reshape(-1, 2)).to(dev)
+    lut = torch.empty(npts, dtype=torch.int32, device=dev)
+    ind = torch.empty((npts, 2), dtype=torch.int32, device=dev)
+    flat = torch.empty(npts, dtype=torch.int64, device=dev)
+    sh = (ctypes.c_int * 2)(N, N)
+    st = _lib.stream_of(dev)
+    _lib.check(L.pf_grog_index(_lib.ptr(cd), 2, sh, npts, 11, _lib.ptr(lut), _lib.ptr(ind), _lib.ptr(flat), st))
+    xin = x.reshape(S, nc, npts).contiguous()
+    yo = torch.empty_like(xin)
+
+    def apply():
+        _lib.check(L.pf_grog_apply(_lib.ptr(xin), _lib.ptr(yo), S, nc, npts, 2, 11, _lib.ptr(tab), S,
+                                   _lib.ptr(lut), st))
+
+    for _ in range(2):
+        apply()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(torch.cuda.current_stream(dev))
+    for _ in range(steps):
+        apply()
+    e1.record(torch.cuda.current_stream(dev))
+    torch.cuda.synchronize(dev)
+    t_apply = e0.elapsed_time(e1) / steps
+    err = float((yo.reshape(y.shape) - y).norm() / y.norm())
+    nbytes = 2 * S * nc * npts * 8
+    flops = 8.0 * 2 * nc * nc * S * npts
+    out["grog_2d_16slices_32coils_402x512"] = {
+        "grog_ms": round(t_grog, 3), "apply_ms": round(t_apply, 3),
+        "apply_GB_s": round(nbytes / (t_apply * 1e-3) / 1e9, 1),
+        "apply_TFLOP_s": round(flops / (t_apply * 1e-3) / 1e12, 2),
+        "apply_frac_fp32_peak": round(flops / (t_apply * 1e-3) / 157.3e12, 3),
+        "apply_vs_call_rel_err": float(f"{err:.1e}")}
+
+    # sparse FFT of the gridded samples: (16 slices x 32 coils) images of 256^2
+    plan = pf.plan_spfft(idx.cpu(), N)
+    img = torch.randn((S * nc, N, N), dtype=torch.complex64, device=dev)
+    flat_t = (idx[..., 1].reshape(-1).to(dev) * N + idx[..., 0].reshape(-1).to(dev))
+
+    def native():
+        k = pf.sparse_fftn(img, plan=plan, device=dev)
+        pf.sparse_ifftn(k, plan=plan, device=dev)
+
+    def plain():
+        ax = (-2, -1)
+        k = torch.fft.fftshift(torch.fft.fftn(torch.fft.ifftshift(img, dim=ax), dim=ax), dim=ax)
+        ks = k.reshape(S * nc, -1)[:, flat_t]
+        g = torch.zeros((S * nc, N * N), dtype=torch.complex64, device=dev).index_add_(1, flat_t, ks)
+        g = g.reshape(S * nc, N, N)
+        torch.fft.fftshift(torch.fft.ifftn(torch.fft.ifftshift(g, dim=ax), dim=ax), dim=ax)
+
+    t_n = _time(native, dev, steps)
+    t_p = _time(plain, dev, steps)
+    out["sparse_fft_2d_256_512images_206k_points"] = {"native_ms": round(t_n, 3), "torch_ms": round(t_p, 3),
+                                                      "speedup": round(t_p / t_n, 2)}
+    return out
+
+
 def bench_autograd(pf, dev, steps):
     """256^2 x 8 coils, 16 spiral interleaves x 4096 samples, 10 ms readout,
     8 B0 time segments (smooth +-100 Hz field map): one L2 data-consistency
@@ -481,7 +571,8 @@ def main():
     if not args.no_extras and not args.small:
         extras = {}
         for name, fn in (("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
-                         ("autograd_b0", bench_autograd), ("cartesian", bench_cartesian)):
+                         ("autograd_b0", bench_autograd), ("cartesian", bench_cartesian),
+                         ("grog_sparse", bench_grog_sparse)):
             extras[name] = fn(pf, dev, max(3, args.steps // 2))
             log(f"{name}: {extras[name]}")
 

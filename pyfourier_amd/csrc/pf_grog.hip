@@ -77,12 +77,17 @@ __global__ void __launch_bounds__(GT) k_grog_apply(const float2* __restrict__ in
                                                    int64_t npts, int nc, int ndim, int nsteps,
                                                    const float2* __restrict__ tables, int nob,
                                                    const int* __restrict__ lut) {
-    extern __shared__ float2 tab[];
+    extern __shared__ float4 tab4[];
+    float2* tab = reinterpret_cast<float2*>(tab4);
     const int64_t ntile = (npts + GT - 1) / GT;
     const int64_t b = blockIdx.x / ntile;
     const int64_t s = (blockIdx.x % ntile) * GT + threadIdx.x;
     const int ob = nob > 1 ? (int)b : 0;
-    const int mstride = nc * nc + 2;  // padded LDS matrix stride (float2)
+    // LDS rows padded with zeros to a multiple of GJC columns (no guards in
+    // the inner loop, 16-byte reads); +2 per matrix spreads the steps' equal
+    // entries over different banks
+    const int ncp = (nc + GJC - 1) / GJC * GJC;
+    const int mstride = nc * ncp + 2;
     const bool live = s < npts;
     const int code = live ? lut[s] : -1;
     const int64_t col = b * nc * npts + s;  // element (b, 0, s); coil i at + i * npts
@@ -93,9 +98,10 @@ __global__ void __launch_bounds__(GT) k_grog_apply(const float2* __restrict__ in
         const float2* T_g = tables + (((int64_t)a * nsteps) * nob + ob) * nc * nc;
         if (TABLE_LDS) {
             __syncthreads();  // previous axis done with the table
-            for (int q = threadIdx.x; q < nsteps * nc * nc; q += GT) {
-                const int st = q / (nc * nc), e = q % (nc * nc);
-                tab[st * mstride + e] = T_g[((int64_t)st * nob) * nc * nc + e];
+            for (int q = threadIdx.x; q < nsteps * nc * ncp; q += GT) {
+                const int st = q / (nc * ncp), e = q % (nc * ncp);
+                const int i = e / ncp, j = e % ncp;
+                tab[st * mstride + e] = j < nc ? T_g[((int64_t)st * nob) * nc * nc + i * nc + j] : make_float2(0.f, 0.f);
             }
             __syncthreads();
         }
@@ -112,10 +118,22 @@ __global__ void __launch_bounds__(GT) k_grog_apply(const float2* __restrict__ in
 #pragma unroll
             for (int i = 0; i < NCMAX; ++i) {
                 if (i < nc) {
-                    const float2* row = m + i * nc + j0;
+                    if (TABLE_LDS) {
+                        const float4* row = reinterpret_cast<const float4*>(m + i * ncp + j0);
+                        float4 r[GJC / 2];
 #pragma unroll
-                    for (int j = 0; j < GJC; ++j)
-                        if (j0 + j < nc) y[j] = gmac(y[j], x[i], row[j]);
+                        for (int q = 0; q < GJC / 2; ++q) r[q] = row[q];
+#pragma unroll
+                        for (int q = 0; q < GJC / 2; ++q) {
+                            y[2 * q] = gmac(y[2 * q], x[i], make_float2(r[q].x, r[q].y));
+                            y[2 * q + 1] = gmac(y[2 * q + 1], x[i], make_float2(r[q].z, r[q].w));
+                        }
+                    } else {
+                        const float2* row = m + i * nc + j0;
+#pragma unroll
+                        for (int j = 0; j < GJC; ++j)
+                            if (j0 + j < nc) y[j] = gmac(y[j], x[i], row[j]);
+                    }
                 }
             }
 #pragma unroll
@@ -131,7 +149,7 @@ int launch_apply(const float2* in, float2* out, int B, int nc, int64_t npts, int
     const int64_t ntile = (npts + GT - 1) / GT;
     const int64_t nblk = (int64_t)B * ntile;
     if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many GROG samples");
-    const int64_t tab = (int64_t)nsteps * (nc * nc + 2);
+    const int64_t tab = (int64_t)nsteps * (nc * ((nc + GJC - 1) / GJC * GJC) + 2);
     if (tab <= GLDS_MAX) {
         const size_t lds = (size_t)tab * sizeof(float2);
         if (lds > 64 * 1024) {
