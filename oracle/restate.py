@@ -424,6 +424,8 @@ def nufft(x, plan, basis=None, weight=None, norm=None, b0=None, daxis=None):
     also be numpy's 'forward' (1/G on the FFT, the scaling of the adjoint).
     """
     x = np.asarray(x, dtype=np.complex128)
+    if x.ndim != plan.ndim + 2:
+        raise ValueError(f"x must be (B, F|K, *N), got shape {x.shape}")
     B = x.shape[0]
     F = plan.nframes
     basis = _basis_c(basis)
@@ -451,6 +453,8 @@ def nufft(x, plan, basis=None, weight=None, norm=None, b0=None, daxis=None):
 def nufft_adj(y, plan, basis=None, weight=None, norm=None, b0=None):
     """Adjoint NUFFT.  y (B, F, M) -> x (B, F|K, *N)."""
     y = np.asarray(y, dtype=np.complex128)
+    if y.ndim != 3:
+        raise ValueError(f"y must be (B, F, M), got shape {y.shape}")
     F = plan.nframes
     if weight is not None:
         y = y * np.asarray(weight).reshape(1, F, -1)

@@ -602,3 +602,13 @@ def test_grog_indexes_weights_and_outside_samples():
     np.testing.assert_array_equal(idx, [[4, 4], [4, 4], [7, 4], [0, 5], [5, 3], [5, 3]])
     np.testing.assert_allclose(w, [0.5, 0.5, 0.5, 0.5, 0.5, 0.5])
     assert np.all(out[:, :, 2] == 0) and np.all(out[:, :, 3] != 0)
+
+
+def test_operator_restatement_rejects_wrong_rank():
+    """y must be (B, F, M): a (B, M) array would be read with the wrong axes."""
+    coord = demo.golden_angle_radial_2d(4, 16)
+    P = R.Plan(coord, 8, width=4, oversamp=2.0)
+    with pytest.raises(ValueError):
+        R.nufft_adj(np.zeros((2, P.npts), np.complex64), P)
+    with pytest.raises(ValueError):
+        R.nufft(np.zeros((2, 8, 8), np.complex64), P)
