@@ -64,10 +64,11 @@ struct BatchParams {
     const float2* basis;
     const float2* b0t;
     const float2* b0s;
+    int dma;  // spread values copied global -> LDS (set by the spread launcher)
 };
 
 BatchParams bparams(const pf_batch& b) {
-    BatchParams q{b.C, b.Cv, b.Cb, b.R, b.L, b.K, b.nfg, b.nimg, b.nt, b.basis, b.b0t, b.b0s};
+    BatchParams q{b.C, b.Cv, b.Cb, b.R, b.L, b.K, b.nfg, b.nimg, b.nt, b.basis, b.b0t, b.b0s, 0};
     return q;
 }
 
@@ -618,7 +619,7 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
     __shared__ unsigned s_zmask[EB];           // tile z rows inside the footprint
     __shared__ unsigned s_ymask[EB];           // tile y rows inside the footprint
     __shared__ float s_w[EB][NW];              // weights at the tile's z, y, x positions (0 outside)
-    __shared__ float2 s_val[EB][VL];           // values times conj(coefficient)
+    __shared__ __attribute__((aligned(16))) float2 s_val[EB][VL];  // values times conj(coefficient)
 
     const int RC = B.R * B.Cv;
     const int V = B.Cb * RC;
@@ -635,6 +636,7 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
     const int cb = uu / RC, rv = uu % RC;
     const int r = rv / B.Cv, cv = rv % B.Cv;
     const int ycol = cb * B.Cv + cv;
+    const bool dma = B.dma != 0;
 
     const int64_t key = it.x;
     const int fg_frame = (int)(key / g.ntile_tot);
@@ -697,15 +699,37 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
             s_ymask[lane] = my;
         }
         __syncthreads();
-        // phase 2: the batch's values, lane (value, half) takes entries half, half+2, ...
-        for (int j = half; j < n; j += 2) {
-            const int s = s_idx[j];
-            float2 val = vok ? ys[(int64_t)s * B.C + ycol] : make_float2(0.f, 0.f);
-            if (B.basis || B.b0t) {
-                const int64_t tread = B.b0t ? ((int64_t)perm[s] % g.npts) % B.nt : 0;
-                val = cmulc(val, sample_coef(B, r, s_frame[j], tread));
+        // phase 2: the batch's values
+        if (dma) {
+            // plain values, 32 whole coils per wave: copied global -> LDS
+            // without registers, 16 lanes x 16 bytes per entry, 4 entries per
+            // instruction, all 8 instructions in flight at once
+            const int sub = lane >> 4, p16 = lane & 15;
+            const char* ybase = reinterpret_cast<const char*>(ys);
+            // not unrolled: one address live at a time (the accumulators hold
+            // the register budget); the copies still overlap, nothing waits
+            // on their data until the barrier
+#pragma unroll 1
+            for (int q = 0; q < EB / 4; ++q) {
+                const int j = q * 4 + sub;
+                const unsigned sj = (unsigned)s_idx[j < n ? j : 0];  // slots past n are never read
+                // 32-bit byte offset (the launcher checks the samples fit in 4 GiB)
+                const unsigned off = (sj * (unsigned)B.C + (unsigned)(vch * VL)) * 8u + (unsigned)p16 * 16u;
+                __builtin_amdgcn_global_load_lds(ybase + off, (__attribute__((address_space(3))) void*)&s_val[q * 4][0],
+                                                 16, 0, 0);
             }
-            s_val[j][v] = val;
+            __builtin_amdgcn_s_waitcnt(0);
+        } else {
+            // lane (value, half) takes entries half, half+2, ...
+            for (int j = half; j < n; j += 2) {
+                const int s = s_idx[j];
+                float2 val = vok ? ys[(int64_t)s * B.C + ycol] : make_float2(0.f, 0.f);
+                if (B.basis || B.b0t) {
+                    const int64_t tread = B.b0t ? ((int64_t)perm[s] % g.npts) % B.nt : 0;
+                    val = cmulc(val, sample_coef(B, r, s_frame[j], tread));
+                }
+                s_val[j][v] = val;
+            }
         }
         __syncthreads();
         // phase 3: accumulate; rows outside the footprint are skipped by
@@ -716,9 +740,18 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
             const unsigned my = __builtin_amdgcn_readfirstlane(s_ymask[j]);
             const float2 val = s_val[j][v];
             const float* w = s_w[j];
+            // the value times this lane's x weights, once per entry: a row
+            // pair then costs one z*y product and two FMAs per x point (wide
+            // x runs keep the weights instead, for registers)
+            constexpr bool VX = TXL <= 4;
+            float2 vxk[TXL];
             float wxl[TXL];
 #pragma unroll
-            for (int k = 0; k < TXL; ++k) wxl[k] = w[TZ + TY + half * TXL + k];
+            for (int k = 0; k < TXL; ++k) {
+                const float wx = w[TZ + TY + half * TXL + k];
+                if constexpr (VX) vxk[k] = make_float2(wx * val.x, wx * val.y);
+                else wxl[k] = wx;
+            }
             // all z/y weights of the entry up front (vector LDS reads), so the
             // row loop does not wait on LDS
             constexpr bool WR = spread_wreg<100 * TZ + TY>();
@@ -737,11 +770,19 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
                 for (int b = 0; b < TY; ++b) {
                     if (!((my >> b) & 1u)) continue;
                     const float wzy = wz * (WR ? wyr[WR ? b : 0] : w[TZ + b]);
-                    const float vx = wzy * val.x, vy = wzy * val.y;
+                    if constexpr (VX) {
 #pragma unroll
-                    for (int k = 0; k < TXL; ++k) {
-                        acc[a][b][k].x += wxl[k] * vx;
-                        acc[a][b][k].y += wxl[k] * vy;
+                        for (int k = 0; k < TXL; ++k) {
+                            acc[a][b][k].x += wzy * vxk[k].x;
+                            acc[a][b][k].y += wzy * vxk[k].y;
+                        }
+                    } else {
+                        const float vx = wzy * val.x, vy = wzy * val.y;
+#pragma unroll
+                        for (int k = 0; k < TXL; ++k) {
+                            acc[a][b][k].x += wxl[k] * vx;
+                            acc[a][b][k].y += wxl[k] * vy;
+                        }
                     }
                 }
             }
@@ -1013,6 +1054,13 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     const int nsub = (int)(((int64_t)p->tile[0] * p->tile[1] * p->tile[2] * VC + TB - 1) / TB);
     if ((int64_t)L.nritems * nvch * nsub > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many reduce blocks");
     if (const RegTile* rt = spread_reg_for(p, b)) {
+        // one value per coil (no basis, no B0) in whole 32-coil chunks: a
+        // wave's values of a sample are one contiguous 256-byte run, copied
+        // global -> LDS with 32-bit offsets (sorted samples under 4 GiB)
+        B.dma = (!b.basis && !b.b0t && b.R == 1 && b.C % SPREAD_VL == 0 &&
+                 (uint64_t)p->mtot * b.C * sizeof(float2) < (1ull << 32))
+                    ? 1
+                    : 0;
         rt->fn(dim3((unsigned)nb), s, g, B, L, p, ys, grid, partial);
         PF_HIP(hipGetLastError());
         if (L.nritems > 0) {
