@@ -17,6 +17,10 @@
 namespace {
 
 constexpr int TB = 256;
+#ifndef PF_GATHER_TB
+#define PF_GATHER_TB 256
+#endif
+constexpr int GTB = PF_GATHER_TB;  // gather block: samples of a block share the CU's L1
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
@@ -247,7 +251,7 @@ __global__ void __launch_bounds__(TB) k_gather(GeoParams g, BatchParams B, int64
 // in registers by constant index), z stays a loop whose weight is read from
 // the record.  In-slab offsets are 32-bit (the host checks G * R * Cv < 2^31).
 template <int CPL, int WY, int WX>
-__global__ void __launch_bounds__(TB) k_gather_w(GeoParams g, BatchParams B, int64_t mtot,
+__global__ void __launch_bounds__(GTB) k_gather_w(GeoParams g, BatchParams B, int64_t mtot,
                                                  const float* __restrict__ taprec,
                                                  const int32_t* __restrict__ perm,
                                                  const float2* __restrict__ grid, float2* __restrict__ ys) {
@@ -915,7 +919,7 @@ using GatherLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchPa
 template <int CPL, int WY, int WX>
 void launch_gather_w(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_plan* p,
                      const float2* in, float2* ys) {
-    hipLaunchKernelGGL((k_gather_w<CPL, WY, WX>), grid, dim3(TB), 0, s, g, B, p->mtot, p->taprec, p->perm, in, ys);
+    hipLaunchKernelGGL((k_gather_w<CPL, WY, WX>), grid, dim3(GTB), 0, s, g, B, p->mtot, p->taprec, p->perm, in, ys);
 }
 
 template <int CPL, int W>
@@ -942,7 +946,7 @@ int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, fl
     if (idx32 && (wy == 1 || wy == wx)) {
         if (GatherLaunch fn = GATHER_W[cpl - 1][wy == 1 ? 0 : 1][wx - 1]) {
             const int64_t threads = p->mtot * (b.C / cpl);
-            fn(dim3(nblk(threads, TB)), s, g, B, p, grid, ys);
+            fn(dim3(nblk(threads, GTB)), s, g, B, p, grid, ys);
             PF_HIP(hipGetLastError());
             return PF_OK;
         }

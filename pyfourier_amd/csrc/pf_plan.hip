@@ -54,6 +54,7 @@ struct TapParams {
     int rec;
     int woff[3];
     int mbits[3];  // Morton bits per axis (0 on all axes: row-major tile order)
+    int ibits[3];  // Morton bits of the position inside the tile (0: tile order only)
     const int64_t* foff;  // ragged frame offsets (nframes + 1) or null
     int nframes;
 };
@@ -145,8 +146,14 @@ __global__ void k_taps(const float* __restrict__ coord, TapParams P, uint32_t* _
     }
     ri[3] = (int)f;
     const int msum = P.mbits[0] + P.mbits[1] + P.mbits[2];
+    const int isum = P.ibits[0] + P.ibits[1] + P.ibits[2];
     if (msum > 0)
-        keys[i] = ((uint32_t)f << msum) | morton3((uint32_t)tcoord[0], (uint32_t)tcoord[1], (uint32_t)tcoord[2], P.mbits);
+        // Z order of the tiles, then Z order inside the tile: consecutive
+        // samples have overlapping footprints (gather cache reuse)
+        keys[i] = ((uint32_t)f << (msum + isum)) |
+                  (morton3((uint32_t)tcoord[0], (uint32_t)tcoord[1], (uint32_t)tcoord[2], P.mbits) << isum) |
+                  morton3((uint32_t)(ri[0] - tcoord[0] * P.tile[0]), (uint32_t)(ri[1] - tcoord[1] * P.tile[1]),
+                          (uint32_t)(ri[2] - tcoord[2] * P.tile[2]), P.ibits);
     else
         keys[i] = (uint32_t)(f * P.ntile_tot +
                              (tcoord[0] * P.ntiles[1] + tcoord[1]) * P.ntiles[2] + tcoord[2]);
@@ -461,15 +468,20 @@ int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
     P.rec = p->rec;
     P.foff = p->foff;
     P.nframes = p->nframes;
-    int msum = 0;
+    int msum = 0, isum = 0;
     for (int a = 0; a < 3; ++a) {
         P.mbits[a] = 0;
         while ((1 << P.mbits[a]) < p->ntiles[a]) ++P.mbits[a];
         msum += P.mbits[a];
+        P.ibits[a] = 0;
+        while ((1 << P.ibits[a]) < p->tile[a]) ++P.ibits[a];
+        isum += P.ibits[a];
     }
-    // Z-order keys need frame << msum to fit 32 bits; else row-major order
+    // Z-order keys need frame << (msum + isum) to fit 32 bits: drop the
+    // in-tile order first, then fall back to row-major tile order
+    if ((uint64_t)p->nframes << (msum + isum) >= (1ull << 32)) P.ibits[0] = P.ibits[1] = P.ibits[2] = 0, isum = 0;
     if (msum == 0 || (uint64_t)p->nframes << msum >= (1ull << 32))
-        P.mbits[0] = P.mbits[1] = P.mbits[2] = 0, msum = 0;
+        P.mbits[0] = P.mbits[1] = P.mbits[2] = 0, msum = 0, P.ibits[0] = P.ibits[1] = P.ibits[2] = 0, isum = 0;
     const int64_t M = p->mtot;
     const int TB = 256;
 
@@ -489,7 +501,7 @@ int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
                        ids.as<int32_t>(), rec0.as<float>(), cos0.as<float>());
     PF_HIP(hipGetLastError());
 
-    const uint64_t nkeys = msum > 0 ? (uint64_t)p->nframes << msum : (uint64_t)p->nframes * p->ntile_tot;
+    const uint64_t nkeys = msum > 0 ? (uint64_t)p->nframes << (msum + isum) : (uint64_t)p->nframes * p->ntile_tot;
     unsigned nbits = bits_for(nkeys + 1);
     size_t tbytes = 0;
     PF_HIP(rocprim::radix_sort_pairs(nullptr, tbytes, keys.as<uint32_t>(), keys2.as<uint32_t>(),
