@@ -9,10 +9,13 @@
 // fixed).  b_l is the temporal B0 interpolator along the readout (x) axis of
 // centred k-space, src/pyfourier/_fft/_plan.py:83-96.
 //
-// The object owns a sample-free geometry plan (width 1, oversampling 1, unit
-// apodisation), so the fused FFT passes of the NUFFT (or rocFFT for sizes with
-// other prime factors) compute FFTc on the (nfg, Cb, Nz, Ny, Nx, R, Cv) grid in
-// stored FFT order.  One more kernel per direction moves between that grid and
+// Mask-only calls (no basis, no field map) on sizes of 2, 3 and 5 run the
+// FFT passes directly on the caller's layout, centred on both sides, with the
+// mask and norm factor fused into the last forward / first inverse pass and
+// no workspace (pf_fftk_cart).  Otherwise the object's sample-free geometry
+// plan (width 1, oversampling 1, unit apodisation) lets the fused FFT passes
+// of the NUFFT (or rocFFT for sizes with other prime factors) compute FFTc on
+// the (nfg, Cb, Nz, Ny, Nx, R, Cv) grid in stored FFT order.  One more kernel per direction moves between that grid and
 // the caller's (C, T, Nz, Ny, Nx) centred k-space: it applies the shift, the
 // basis contraction over K, the B0 sum over L, the mask and the norm factor in
 // the same pass that transposes coils out of the innermost position.  Both are
@@ -258,7 +261,14 @@ size_t cart_grid_bytes(const pf_cart* c, const pf_batch& b) {
     return (size_t)b.nfg * b.C * b.R * (size_t)pf_gtot(c->geo) * sizeof(float2);
 }
 
+// mask-only calls on sizes the fused FFT supports run the FFT passes directly
+// on the caller's layout (no grid, no transposes, no workspace)
+bool cart_direct(const pf_cart* c, const pf_batch& b) {
+    return b.R == 1 && !b.basis && !b.b0t && pf_fftk_supported(c->geo);
+}
+
 size_t cart_ws_bytes(const pf_cart* c, const pf_batch& b) {
+    if (cart_direct(c, b)) return 0;
     size_t extra = pf_fftk_supported(c->geo) ? pf_fftk_extra_bytes(c->geo, b) : 0;
     return align_up(cart_grid_bytes(c, b)) + align_up(extra);
 }
@@ -411,13 +421,17 @@ int pf_cart_fft(pf_cart* c, const pf_batch_desc* d, int nframes, const void* ima
     if (rc != PF_OK) return rc;
     if (!image || !kspace) return pf_fail(PF_ERR_ARG, "null array");
     if (norm < PF_NORM_NONE || norm > PF_NORM_ADJOINT) return pf_fail(PF_ERR_ARG, "norm must be 0, 1 or 2");
-    if (!workspace || workspace_bytes < cart_ws_bytes(c, b)) return pf_fail(PF_ERR_ARG, "workspace too small");
+    if (workspace_bytes < cart_ws_bytes(c, b) || (!workspace && cart_ws_bytes(c, b) > 0))
+        return pf_fail(PF_ERR_ARG, "workspace too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    float2* grid = static_cast<float2*>(workspace);
-    float2* extra = reinterpret_cast<float2*>(static_cast<char*>(workspace) + align_up(cart_grid_bytes(c, b)));
     double fwd, inv;
     cart_factors(c, norm, &fwd, &inv);
     pf_plan* p = c->geo;
+    if (cart_direct(c, b))
+        return pf_fftk_cart(p, (int64_t)b.C * nframes, nframes, c->mask, c->mframes, true,
+                            static_cast<const float2*>(image), static_cast<float2*>(kspace), (float)fwd, s);
+    float2* grid = static_cast<float2*>(workspace);
+    float2* extra = reinterpret_cast<float2*>(static_cast<char*>(workspace) + align_up(cart_grid_bytes(c, b)));
     if (pf_fftk_supported(p)) {
         if ((rc = pf_fftk_forward(p, b, static_cast<const float2*>(image), grid, extra, s)) != PF_OK) return rc;
     } else {
@@ -434,13 +448,17 @@ int pf_cart_ifft(pf_cart* c, const pf_batch_desc* d, int nframes, const void* ks
     if (rc != PF_OK) return rc;
     if (!image || !kspace) return pf_fail(PF_ERR_ARG, "null array");
     if (norm < PF_NORM_NONE || norm > PF_NORM_ADJOINT) return pf_fail(PF_ERR_ARG, "norm must be 0, 1 or 2");
-    if (!workspace || workspace_bytes < cart_ws_bytes(c, b)) return pf_fail(PF_ERR_ARG, "workspace too small");
+    if (workspace_bytes < cart_ws_bytes(c, b) || (!workspace && cart_ws_bytes(c, b) > 0))
+        return pf_fail(PF_ERR_ARG, "workspace too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    float2* grid = static_cast<float2*>(workspace);
-    float2* extra = reinterpret_cast<float2*>(static_cast<char*>(workspace) + align_up(cart_grid_bytes(c, b)));
     double fwd, inv;
     cart_factors(c, norm, &fwd, &inv);
     pf_plan* p = c->geo;
+    if (cart_direct(c, b))
+        return pf_fftk_cart(p, (int64_t)b.C * nframes, nframes, c->mask, c->mframes, false,
+                            static_cast<const float2*>(kspace), static_cast<float2*>(image), (float)inv, s);
+    float2* grid = static_cast<float2*>(workspace);
+    float2* extra = reinterpret_cast<float2*>(static_cast<char*>(workspace) + align_up(cart_grid_bytes(c, b)));
     if ((rc = launch_cart(c, b, nframes, false, nullptr, grid, static_cast<const float2*>(kspace), nullptr, 1.0f, s)) !=
         PF_OK)
         return rc;

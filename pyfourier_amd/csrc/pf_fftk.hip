@@ -619,6 +619,206 @@ __global__ void __launch_bounds__(256, PF_FFT_WPE) k_fftx256(Geo g, Batch bt, co
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Cartesian passes (masked centred FFT without basis or B0): the same FFTs
+// directly on the caller's (planes, Nz, Ny, Nx) layout, oversampling 1.
+// Both sides are centred: stored position n of a line holds centred index
+// (n + N/2) mod N, in the image and in k-space alike, so a pass reads and
+// writes line element (n + N/2) mod N.  The last forward pass multiplies by
+// the mask (frame t = plane mod T) and the norm factor on the way out; the
+// first inverse pass applies the mask on the way in and the last one the
+// factor.  A block reads all of its lines before it writes, so passes run in
+// place.
+// ---------------------------------------------------------------------------
+struct CartEpi {
+    const float* mask;  // (mframes, rows, I) centred, or null
+    int T;              // frames per plane group (plane = c * T + t)
+    int mframes;
+    float factor;
+};
+
+__device__ __forceinline__ int crot(int n, int N) {
+    const int j = n + N / 2;
+    return j >= N ? j - N : j;
+}
+
+__device__ __forceinline__ float cmask(const CartEpi& e, int64_t o, int rows, int j, int64_t I, int64_t i) {
+    if (!e.mask) return 1.f;
+    const int64_t tm = e.mframes > 1 ? o % e.T : 0;
+    return e.mask[(tm * rows + j) * I + i];
+}
+
+// column pass, G = 256: view (O, 256, I); four-step register FFT as k_fftcol256
+template <bool FWD>
+__global__ void __launch_bounds__(256, PF_FFT_WPE) k_cartcol256(int64_t I, const float2* __restrict__ twg,
+                                                              const float2* in, float2* out, CartEpi e) {
+    constexpr int G = 256, B = 16;
+    __shared__ float2 sh[16 * C256_PLANE];
+    __shared__ float2 tw[G];
+    const int b = threadIdx.x & 15, t = threadIdx.x >> 4;
+    const int64_t nchunk = (I + B - 1) / B;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t o = blk / nchunk;
+    const int64_t i = (blk % nchunk) * B + b;
+    const bool bok = i < I;
+    tw[threadIdx.x] = twg[threadIdx.x];
+    const float2* src = in + o * G * I + i;
+    float2 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int j = crot(t + 16 * k, G);
+        float2 x = bok ? src[(int64_t)j * I] : make_float2(0.f, 0.f);
+        if (!FWD) {
+            if (bok) x = c_scale(x, cmask(e, o, G, j, I, i));
+            x = make_float2(x.x, -x.y);
+        }
+        v[k] = x;
+    }
+    dft16(v);
+    __syncthreads();
+#pragma unroll
+    for (int m1 = 1; m1 < 16; ++m1) v[m1] = c_mul(v[m1], tw[t * m1]);
+#pragma unroll
+    for (int m1 = 0; m1 < 16; ++m1) sh[m1 * C256_PLANE + t * 16 + b] = v[m1];
+    __syncthreads();
+#pragma unroll
+    for (int tp = 0; tp < 16; ++tp) v[tp] = sh[t * C256_PLANE + tp * 16 + b];
+    dft16(v);  // v[m2] = X[t + 16 m2]
+    if (!bok) return;
+    float2* dst = out + o * G * I + i;
+#pragma unroll
+    for (int m2 = 0; m2 < 16; ++m2) {
+        const int j = crot(t + 16 * m2, G);
+        float2 y = FWD ? v[m2] : make_float2(v[m2].x, -v[m2].y);
+        const float sc = FWD ? e.factor * cmask(e, o, G, j, I, i) : e.factor;
+        dst[(int64_t)j * I] = c_scale(y, sc);
+    }
+}
+
+// column pass, any supported G: Stockham in LDS over B columns
+template <bool FWD>
+__global__ void __launch_bounds__(FT, PF_FFT_WPE) k_cartcol(AxisFFT A, int64_t I, int B, int bshift, const float2* in,
+                                                            float2* out, CartEpi e) {
+    extern __shared__ float2 sh[];
+    const int G = A.G;
+    const int LD = B + 1;
+    float2* buf = sh;
+    float2* tw = sh + G * LD;
+    const int64_t nchunk = (I + B - 1) / B;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t o = blk / nchunk;
+    const int64_t i0 = (blk % nchunk) * B;
+    const int bw = (int)std::min<int64_t>(B, I - i0);
+    for (int q = threadIdx.x; q < G; q += FT) tw[q] = A.tw[q];
+    const float2* src = in + o * (int64_t)G * I + i0;
+    for (int q = threadIdx.x; q < (G << bshift); q += FT) {
+        const int b = q & (B - 1), n = q >> bshift;
+        const int j = crot(n, G);
+        float2 x = b < bw ? src[(int64_t)j * I + b] : make_float2(0.f, 0.f);
+        if (!FWD) {
+            if (b < bw) x = c_scale(x, cmask(e, o, G, j, I, i0 + b));
+            x = make_float2(x.x, -x.y);
+        }
+        buf[n * LD + b] = x;
+    }
+    __syncthreads();
+    lds_fft(buf, LD, B, A, tw);
+    float2* dst = out + o * (int64_t)G * I + i0;
+    for (int q = threadIdx.x; q < (G << bshift); q += FT) {
+        const int b = q & (B - 1), n = q >> bshift;
+        if (b >= bw) continue;
+        const int j = crot(n, G);
+        const float2 x = buf[n * LD + b];
+        const float2 y = FWD ? x : make_float2(x.x, -x.y);
+        const float sc = FWD ? e.factor * cmask(e, o, G, j, I, i0 + b) : e.factor;
+        dst[(int64_t)j * I + b] = c_scale(y, sc);
+    }
+}
+
+// row pass, G = 256: 16 contiguous rows per block; load lanes run along the
+// row, the transpose hands every thread one output phase m1, and the store
+// lanes run along the row again
+template <bool FWD>
+__global__ void __launch_bounds__(256, PF_FFT_WPE) k_cartrow256(int64_t nrows, const float2* __restrict__ twg,
+                                                              const float2* in, float2* out, CartEpi e) {
+    constexpr int G = 256;
+    __shared__ float2 sh[16 * X256_PLANE];
+    __shared__ float2 tw[G];
+    const int64_t r0 = (int64_t)pf_xcd_remap(blockIdx.x, gridDim.x) * 16;
+    tw[threadIdx.x] = twg[threadIdx.x];
+    float2 v[16];
+    {
+        const int t = threadIdx.x & 15, b = threadIdx.x >> 4;
+        const int64_t row = r0 + b;
+        const bool ok = row < nrows;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int j = crot(t + 16 * k, G);
+            float2 x = ok ? in[row * G + j] : make_float2(0.f, 0.f);
+            if (!FWD) {
+                if (ok) x = c_scale(x, cmask(e, row, 1, 0, G, j));
+                x = make_float2(x.x, -x.y);
+            }
+            v[k] = x;
+        }
+        dft16(v);
+        __syncthreads();
+#pragma unroll
+        for (int m1 = 1; m1 < 16; ++m1) v[m1] = c_mul(v[m1], tw[t * m1]);
+#pragma unroll
+        for (int m1 = 0; m1 < 16; ++m1) sh[m1 * X256_PLANE + t * X256_ROW + b] = v[m1];
+    }
+    __syncthreads();
+    const int m1 = threadIdx.x & 15, b = threadIdx.x >> 4;
+#pragma unroll
+    for (int tp = 0; tp < 16; ++tp) v[tp] = sh[m1 * X256_PLANE + tp * X256_ROW + b];
+    dft16(v);  // v[m2] = X[m1 + 16 m2] of row b
+    const int64_t row = r0 + b;
+    if (row >= nrows) return;
+#pragma unroll
+    for (int m2 = 0; m2 < 16; ++m2) {
+        const int j = crot(m1 + 16 * m2, G);
+        const float2 y = FWD ? v[m2] : make_float2(v[m2].x, -v[m2].y);
+        const float sc = FWD ? e.factor * cmask(e, row, 1, 0, G, j) : e.factor;
+        out[row * G + j] = c_scale(y, sc);
+    }
+}
+
+// row pass, any supported G: B rows per block transposed into LDS columns
+template <bool FWD>
+__global__ void __launch_bounds__(FT, PF_FFT_WPE) k_cartrow(AxisFFT A, int64_t nrows, int B, const float2* in,
+                                                            float2* out, CartEpi e) {
+    extern __shared__ float2 sh[];
+    const int G = A.G;
+    const int LD = B + 1;
+    float2* buf = sh;
+    float2* tw = sh + G * LD;
+    const int64_t r0 = (int64_t)pf_xcd_remap(blockIdx.x, gridDim.x) * B;
+    for (int q = threadIdx.x; q < G; q += FT) tw[q] = A.tw[q];
+    for (int q = threadIdx.x; q < B * G; q += FT) {
+        const int j = q % G, r = q / G;  // centred element j of row r
+        const int64_t row = r0 + r;
+        float2 x = row < nrows ? in[row * G + j] : make_float2(0.f, 0.f);
+        if (!FWD) {
+            if (row < nrows) x = c_scale(x, cmask(e, row, 1, 0, G, j));
+            x = make_float2(x.x, -x.y);
+        }
+        buf[stored_pos(j, G, G) * LD + r] = x;
+    }
+    __syncthreads();
+    lds_fft(buf, LD, B, A, tw);
+    for (int q = threadIdx.x; q < B * G; q += FT) {
+        const int j = q % G, r = q / G;
+        const int64_t row = r0 + r;
+        if (row >= nrows) continue;
+        const float2 x = buf[stored_pos(j, G, G) * LD + r];
+        const float2 y = FWD ? x : make_float2(x.x, -x.y);
+        const float sc = FWD ? e.factor * cmask(e, row, 1, 0, G, j) : e.factor;
+        out[row * G + j] = c_scale(y, sc);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -857,4 +1057,114 @@ int pf_fftk_adjoint(pf_plan* p, const pf_batch& b, float2* grid, float2* img, fl
     if ((rc = launch_col(p, 0, false, FC, (int64_t)p->os[1] * p->os[2] * RC, grid, t2, s)) != PF_OK) return rc;
     if ((rc = launch_col(p, 1, false, FC * p->shape[0], (int64_t)p->os[2] * RC, t2, t1, s)) != PF_OK) return rc;
     return launch_x(p, b, false, t1, img, factor, s);
+}
+
+// Masked centred FFT on the caller's layout (planes, Nz, Ny, Nx), in place
+// after the first pass; see the Cartesian pass kernels above.
+static int cart_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, const float2* src, float2* dst,
+                    const CartEpi& e, hipStream_t s) {
+    const pf_fftk_axis& X = p->fftk->ax[axis];
+    if (X.G == 256) {
+        const int64_t nblk = O * ((I + 15) / 16);
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+        if (nblk < 1) return PF_OK;
+        if (fwd)
+            hipLaunchKernelGGL(k_cartcol256<true>, dim3((unsigned)nblk), dim3(256), 0, s, I, X.tw, src, dst, e);
+        else
+            hipLaunchKernelGGL(k_cartcol256<false>, dim3((unsigned)nblk), dim3(256), 0, s, I, X.tw, src, dst, e);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    AxisFFT A = axis_desc(X);
+    const int B = cols_for(X.G);
+    int bshift = 0;
+    while ((1 << bshift) < B) ++bshift;
+    const int64_t nblk = O * ((I + B - 1) / B);
+    if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+    if (nblk < 1) return PF_OK;
+    const size_t lds = ((size_t)X.G * (B + 1) + X.G) * sizeof(float2);
+    if (lds > 64 * 1024) {
+        static bool raised = false;
+        if (!raised) {
+            const void* k[] = {reinterpret_cast<const void*>(k_cartcol<true>),
+                               reinterpret_cast<const void*>(k_cartcol<false>)};
+            for (const void* f : k)
+                if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+                    return pf_fail(PF_ERR_HIP, "could not raise the Cartesian FFT kernels' LDS limit");
+            raised = true;
+        }
+    }
+    if (fwd)
+        hipLaunchKernelGGL(k_cartcol<true>, dim3((unsigned)nblk), dim3(FT), lds, s, A, I, B, bshift, src, dst, e);
+    else
+        hipLaunchKernelGGL(k_cartcol<false>, dim3((unsigned)nblk), dim3(FT), lds, s, A, I, B, bshift, src, dst, e);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src, float2* dst, const CartEpi& e,
+                    hipStream_t s) {
+    const pf_fftk_axis& X = p->fftk->ax[2];
+    if (X.G == 256) {
+        const int64_t nblk = (nrows + 15) / 16;
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+        if (nblk < 1) return PF_OK;
+        if (fwd)
+            hipLaunchKernelGGL(k_cartrow256<true>, dim3((unsigned)nblk), dim3(256), 0, s, nrows, X.tw, src, dst, e);
+        else
+            hipLaunchKernelGGL(k_cartrow256<false>, dim3((unsigned)nblk), dim3(256), 0, s, nrows, X.tw, src, dst, e);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    AxisFFT A = axis_desc(X);
+    const int B = cols_for(X.G);
+    const int64_t nblk = (nrows + B - 1) / B;
+    if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+    if (nblk < 1) return PF_OK;
+    const size_t lds = ((size_t)X.G * (B + 1) + X.G) * sizeof(float2);
+    if (lds > 64 * 1024) {
+        static bool raised = false;
+        if (!raised) {
+            const void* k[] = {reinterpret_cast<const void*>(k_cartrow<true>),
+                               reinterpret_cast<const void*>(k_cartrow<false>)};
+            for (const void* f : k)
+                if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+                    return pf_fail(PF_ERR_HIP, "could not raise the Cartesian FFT kernels' LDS limit");
+            raised = true;
+        }
+    }
+    if (fwd)
+        hipLaunchKernelGGL(k_cartrow<true>, dim3((unsigned)nblk), dim3(FT), lds, s, A, nrows, B, src, dst, e);
+    else
+        hipLaunchKernelGGL(k_cartrow<false>, dim3((unsigned)nblk), dim3(FT), lds, s, A, nrows, B, src, dst, e);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+int pf_fftk_cart(pf_plan* p, int64_t planes, int T, const float* mask, int mframes, bool fwd, const float2* src,
+                 float2* dst, float factor, hipStream_t s) {
+    int rc = pf_fftk_init(p);
+    if (rc != PF_OK) return rc;
+    const int64_t Nz = p->shape[0], Ny = p->shape[1], Nx = p->shape[2];
+    const CartEpi none{nullptr, 1, 0, 1.0f};
+    const CartEpi last{mask, T, mframes, factor};              // forward: last pass
+    const CartEpi first{mask, T, mframes, 1.0f};               // inverse: first pass
+    const CartEpi fin{nullptr, 1, 0, factor};                   // inverse: last pass
+    if (p->ndim == 1) {
+        const CartEpi one{mask, T, mframes, factor};
+        return cart_row(p, fwd, planes, src, dst, one, s);
+    }
+    if (fwd) {
+        if ((rc = cart_row(p, true, planes * Nz * Ny, src, dst, none, s)) != PF_OK) return rc;
+        if (p->ndim == 2) return cart_col(p, 1, true, planes, Nx, dst, dst, last, s);
+        if ((rc = cart_col(p, 1, true, planes * Nz, Nx, dst, dst, none, s)) != PF_OK) return rc;
+        return cart_col(p, 0, true, planes, Ny * Nx, dst, dst, last, s);
+    }
+    if (p->ndim == 2) {
+        if ((rc = cart_col(p, 1, false, planes, Nx, src, dst, first, s)) != PF_OK) return rc;
+        return cart_row(p, false, planes * Ny, dst, dst, fin, s);
+    }
+    if ((rc = cart_col(p, 0, false, planes, Ny * Nx, src, dst, first, s)) != PF_OK) return rc;
+    if ((rc = cart_col(p, 1, false, planes * Nz, Nx, dst, dst, none, s)) != PF_OK) return rc;
+    return cart_row(p, false, planes * Nz * Ny, dst, dst, fin, s);
 }

@@ -135,6 +135,10 @@ size_t pf_fftk_extra_bytes(const pf_plan* p, const pf_batch& b);
 int pf_fftk_forward(pf_plan* p, const pf_batch& b, const float2* img, float2* grid, float2* extra, hipStream_t s);
 int pf_fftk_adjoint(pf_plan* p, const pf_batch& b, float2* grid, float2* img, float2* extra, float factor,
                     hipStream_t s);
+// masked centred FFT (oversampling 1) on the layout (planes, Nz, Ny, Nx):
+// src -> dst, in place after the first pass; mask (mframes, Nz*Ny*Nx) or null
+int pf_fftk_cart(pf_plan* p, int64_t planes, int T, const float* mask, int mframes, bool fwd, const float2* src,
+                 float2* dst, float factor, hipStream_t s);
 
 // ---- small helpers --------------------------------------------------------
 static inline int64_t pf_gtot(const pf_plan* p) { return (int64_t)p->os[0] * p->os[1] * p->os[2]; }
