@@ -885,6 +885,107 @@ __global__ void __launch_bounds__(TB) k_unsort(int64_t mtot, int C, const int32_
     }
 }
 
+// 16-byte forms of the two permutations (even C, even Mtot, 16-byte aligned
+// arrays): a block moves QSB samples x QSC values, every thread issues all of
+// its QN loads before it stores anything, the sorted side as 16-byte pieces of
+// whole coil rows, the user side as pairs of consecutive samples.
+constexpr int QSB = 128;
+constexpr int QSC = 32;
+constexpr int QN = QSB * QSC / 2 / TB;  // float4 per thread and phase
+
+// user -> sorted, times the density weight
+__global__ void __launch_bounds__(TB) k_sort4(int64_t mtot, int C, const int32_t* __restrict__ iperm,
+                                              const float2* __restrict__ y, const float* __restrict__ w,
+                                              float2* __restrict__ ys) {
+    __shared__ float2 t[QSB][QSC + 1];
+    const int64_t j0 = (int64_t)blockIdx.x * QSB;
+    const int c0 = blockIdx.y * QSC;
+    const int cc = min(QSC, C - c0);
+    const int nj = (int)min<int64_t>(QSB, mtot - j0);  // even
+    float4 v[QN];
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        const int i = threadIdx.x + k * TB;
+        const int c = i / (QSB / 2), jl = 2 * (i % (QSB / 2));
+        if (c < cc && jl < nj) v[k] = *reinterpret_cast<const float4*>(y + (int64_t)(c0 + c) * mtot + j0 + jl);
+    }
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        const int i = threadIdx.x + k * TB;
+        const int c = i / (QSB / 2), jl = 2 * (i % (QSB / 2));
+        if (c < cc && jl < nj) {
+            const float a0 = w ? w[j0 + jl] : 1.0f, a1 = w ? w[j0 + jl + 1] : 1.0f;
+            t[jl][c] = make_float2(v[k].x * a0, v[k].y * a0);
+            t[jl + 1][c] = make_float2(v[k].z * a1, v[k].w * a1);
+        }
+    }
+    __syncthreads();
+    const int h = cc >> 1;  // float4 per sorted row
+    int row[QN];
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        const int i = threadIdx.x + k * TB;
+        const int jl = i / h;
+        row[k] = jl < nj ? iperm[j0 + jl] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        const int i = threadIdx.x + k * TB;
+        const int jl = i / h, q = i - jl * h;
+        if (jl < nj) {
+            const float2 a = t[jl][2 * q], b = t[jl][2 * q + 1];
+            *reinterpret_cast<float4*>(ys + (int64_t)row[k] * C + c0 + 2 * q) = make_float4(a.x, a.y, b.x, b.y);
+        }
+    }
+}
+
+// sorted -> user, times factor and the density weight
+__global__ void __launch_bounds__(TB) k_unsort4(int64_t mtot, int C, const int32_t* __restrict__ iperm,
+                                                const float2* __restrict__ ys, const float* __restrict__ w,
+                                                float factor, float2* __restrict__ y) {
+    __shared__ float2 t[QSB][QSC + 1];
+    const int64_t j0 = (int64_t)blockIdx.x * QSB;
+    const int c0 = blockIdx.y * QSC;
+    const int cc = min(QSC, C - c0);
+    const int h = cc >> 1;
+    const int nj = (int)min<int64_t>(QSB, mtot - j0);  // even
+    int row[QN];
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        const int i = threadIdx.x + k * TB;
+        const int jl = i / h;
+        row[k] = jl < nj ? iperm[j0 + jl] : 0;
+    }
+    float4 v[QN];
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        const int i = threadIdx.x + k * TB;
+        const int jl = i / h, q = i - jl * h;
+        if (jl < nj) v[k] = *reinterpret_cast<const float4*>(ys + (int64_t)row[k] * C + c0 + 2 * q);
+    }
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        const int i = threadIdx.x + k * TB;
+        const int jl = i / h, q = i - jl * h;
+        if (jl < nj) {
+            t[jl][2 * q] = make_float2(v[k].x, v[k].y);
+            t[jl][2 * q + 1] = make_float2(v[k].z, v[k].w);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        const int i = threadIdx.x + k * TB;
+        const int c = i / (QSB / 2), jl = 2 * (i % (QSB / 2));
+        if (c < cc && jl < nj) {
+            const float a0 = factor * (w ? w[j0 + jl] : 1.0f), a1 = factor * (w ? w[j0 + jl + 1] : 1.0f);
+            const float2 u0 = t[jl][c], u1 = t[jl + 1][c];
+            *reinterpret_cast<float4*>(y + (int64_t)(c0 + c) * mtot + j0 + jl) =
+                make_float4(u0.x * a0, u0.y * a0, u1.x * a1, u1.y * a1);
+        }
+    }
+}
+
 unsigned grid_stride_blocks(int64_t n) {
     int64_t b = (n + TB - 1) / TB;
     return (unsigned)std::min<int64_t>(b, 1 << 20);
@@ -1095,10 +1196,22 @@ static dim3 permute_grid(const pf_plan* p, const pf_batch& b) {
     return dim3((unsigned)((p->mtot + PSB - 1) / PSB), (unsigned)((b.C + PSC - 1) / PSC));
 }
 
+static dim3 permute_grid4(const pf_plan* p, const pf_batch& b) {
+    return dim3((unsigned)((p->mtot + QSB - 1) / QSB), (unsigned)((b.C + QSC - 1) / QSC));
+}
+
+// the 16-byte permutation kernels need pairs of values on both sides
+static bool permute_vec(const pf_plan* p, const pf_batch& b, const void* a, const void* c) {
+    return b.C % 2 == 0 && p->mtot % 2 == 0 && ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(c)) & 15) == 0;
+}
+
 int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2* ys, const float* weight,
                    hipStream_t s) {
     if ((p->mtot + PSB - 1) / PSB > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many samples");
-    hipLaunchKernelGGL(k_sort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, y, weight, ys);
+    if (permute_vec(p, b, y, ys))
+        hipLaunchKernelGGL(k_sort4, permute_grid4(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, y, weight, ys);
+    else
+        hipLaunchKernelGGL(k_sort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, y, weight, ys);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
@@ -1106,7 +1219,12 @@ int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2*
 int pf_launch_unsort(const pf_plan* p, const pf_batch& b, const float2* ys, float2* y, const float* weight,
                      float factor, hipStream_t s) {
     if ((p->mtot + PSB - 1) / PSB > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many samples");
-    hipLaunchKernelGGL(k_unsort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, ys, weight, factor, y);
+    if (permute_vec(p, b, ys, y))
+        hipLaunchKernelGGL(k_unsort4, permute_grid4(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, ys, weight, factor,
+                           y);
+    else
+        hipLaunchKernelGGL(k_unsort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, ys, weight, factor,
+                           y);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }
