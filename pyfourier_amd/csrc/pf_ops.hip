@@ -595,6 +595,9 @@ constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 #ifndef PF_SPREAD_UNROLL
 #define PF_SPREAD_UNROLL 1
 #endif
+#ifndef PF_SPREAD_PIPE
+#define PF_SPREAD_PIPE 1  // plain 32-coil batches: software-pipelined copies
+#endif
 #ifndef PF_SPREAD_WREG
 #define PF_SPREAD_WREG 1  // an entry's z/y weights in registers (measured faster)
 #endif
@@ -808,6 +811,274 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
                     if (a < ez && b < ey && lx < ex) {
                         const int64_t gpt = ((int64_t)(oz + a) * g.os[1] + (oy + b)) * g.os[2] + (ox + lx);
                         base[gpt * RC] = acc[a][b][k];
+                    }
+                }
+    } else {
+        const int Tx = g.tile[2], Ty = g.tile[1], Tz = g.tile[0];
+        float2* dst = partial + ((int64_t)it.w * nvch + vch) * (int64_t)(Tx * Ty * Tz) * VL + v;
+#pragma unroll
+        for (int a = 0; a < TZ; ++a)
+#pragma unroll
+            for (int b = 0; b < TY; ++b)
+#pragma unroll
+                for (int k = 0; k < TXL; ++k) {
+                    const int lx = half * TXL + k;
+                    if (a < Tz && b < Ty && lx < Tx) dst[(int64_t)((a * Ty + b) * Tx + lx) * VL] = acc[a][b][k];
+                }
+    }
+}
+
+// -------------------------------------------------------------------------
+// Software-pipelined register spread for plain values in whole 32-coil
+// chunks (no basis, no B0).  Same tile ownership, lane roles and summation
+// order as k_spread_reg; the difference is where the wave waits.  Batches of
+// SPIPE_EB entries alternate between two LDS buffer pairs: while the wave
+// accumulates batch b from one pair, the tap records and the values of
+// batch b + 1 are already being copied global -> LDS into the other pair
+// (global_load_lds, no registers held), and the sample ids of batch b + 2 are
+// in flight.  The wave then waits once per batch, for copies issued a whole
+// batch of arithmetic earlier.
+// -------------------------------------------------------------------------
+constexpr int SPIPE_EB = 16;
+constexpr int SPIPE_RP = 8;  // 16-byte record pieces per entry slot (a record has at most 4 + 3 * 8 dwords)
+static_assert(4 + 3 * PF_WMAX <= 4 * SPIPE_RP, "tap record exceeds its LDS slot");
+
+// an entry's row: its TZ + TY + TX weights at the tile's positions, then the
+// bit masks of the tile z and y rows its footprint covers; padded to 16 bytes
+template <int TZ, int TY, int TXL>
+constexpr int spipe_row() { return (TZ + TY + 2 * TXL + 2 + 3) / 4 * 4; }
+
+// copies of one batch: each entry's tap record (rec4 pieces of 16 bytes; the
+// slot's unused pieces take a harmless copy of piece 0) and its 32 values;
+// sid holds lane j's sample id for entries j < n
+__device__ __forceinline__ void spipe_issue(float4 (&s_rec)[SPIPE_EB][SPIPE_RP], float2 (&val)[SPIPE_EB][SPREAD_VL],
+                                            int sid, int n, int rec4, const float* __restrict__ taprec,
+                                            const char* __restrict__ ybase, int C, int vch) {
+    const int lane = threadIdx.x;
+    const int c = lane & (SPIPE_RP - 1);
+    const int cs = c < rec4 ? c : 0;
+#pragma unroll
+    for (int k = 0; k < SPIPE_EB * SPIPE_RP / 64; ++k) {
+        const int e = k * (64 / SPIPE_RP) + (lane >> 3);
+        const int se = __shfl(sid, e < n ? e : 0);
+        const float* src = taprec + (int64_t)se * (rec4 * 4) + 4 * cs;
+        if (k * (64 / SPIPE_RP) < n)
+            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_rec[k * (64 / SPIPE_RP)][0],
+                                             16, 0, 0);
+    }
+    // values: 16 lanes x 16 bytes per entry, 4 entries per copy
+    const int sub = lane >> 4, p16 = lane & 15;
+#pragma unroll
+    for (int q = 0; q < SPIPE_EB / 4; ++q) {
+        const int j = q * 4 + sub;
+        const unsigned sj = (unsigned)__shfl(sid, j < n ? j : 0);
+        const unsigned off = (sj * (unsigned)C + (unsigned)(vch * SPREAD_VL)) * 8u + (unsigned)p16 * 16u;
+        if (q * 4 < n)
+            __builtin_amdgcn_global_load_lds(ybase + off, (__attribute__((address_space(3))) void*)&val[q * 4][0], 16,
+                                             0, 0);
+    }
+}
+
+// one entry per lane: its record -> its weights at this tile's positions
+// (0 outside the footprint) and the tile rows its footprint covers
+template <int TZ, int TY, int TXL>
+__device__ __forceinline__ void spipe_weights(const float4 (&s_rec)[SPIPE_EB][SPIPE_RP], int n, const GeoParams& g,
+                                              int oz, int oy, int ox,
+                                              float (&s_w)[SPIPE_EB][spipe_row<TZ, TY, TXL>()]) {
+    constexpr int TX = 2 * TXL;
+    const int lane = threadIdx.x;
+    if (lane >= n) return;
+    const float* rec_l = reinterpret_cast<const float*>(&s_rec[lane][0]);
+    const int* ri = reinterpret_cast<const int*>(rec_l);
+    int bz = oz - ri[0];
+    bz += bz < 0 ? g.os[0] : 0;
+    int by = oy - ri[1];
+    by += by < 0 ? g.os[1] : 0;
+    int bx = ox - ri[2];
+    bx += bx < 0 ? g.os[2] : 0;
+    unsigned mz = 0, my = 0;
+#pragma unroll
+    for (int a = 0; a < TZ; ++a) {
+        int i = bz + a;
+        i -= i >= g.os[0] ? g.os[0] : 0;
+        const bool in = i < g.width[0];
+        s_w[lane][a] = in ? rec_l[g.woff[0] + i] : 0.f;
+        mz |= in ? 1u << a : 0u;
+    }
+#pragma unroll
+    for (int b = 0; b < TY; ++b) {
+        int i = by + b;
+        i -= i >= g.os[1] ? g.os[1] : 0;
+        const bool in = i < g.width[1];
+        s_w[lane][TZ + b] = in ? rec_l[g.woff[1] + i] : 0.f;
+        my |= in ? 1u << b : 0u;
+    }
+#pragma unroll
+    for (int x = 0; x < TX; ++x) {
+        int i = bx + x;
+        i -= i >= g.os[2] ? g.os[2] : 0;
+        s_w[lane][TZ + TY + x] = i < g.width[2] ? rec_l[g.woff[2] + i] : 0.f;
+    }
+    constexpr int NW = TZ + TY + TX;
+    s_w[lane][NW] = __uint_as_float(mz);
+    s_w[lane][NW + 1] = __uint_as_float(my);
+}
+
+// the batch's n entries into the lane's accumulators, in entry order
+template <int TZ, int TY, int TXL>
+__device__ __forceinline__ void spipe_accumulate(const float2 (&val_b)[SPIPE_EB][SPREAD_VL], int n,
+                                                 const float (&s_w)[SPIPE_EB][spipe_row<TZ, TY, TXL>()],
+                                                 float2 (&acc)[TZ][TY][TXL]) {
+    const int lane = threadIdx.x;
+    const int v = lane & (SPREAD_VL - 1);
+    const int half = lane / SPREAD_VL;
+    constexpr bool WR = spread_wreg<100 * TZ + TY>();
+    constexpr bool VX = TXL <= 4;
+    for (int j = 0; j < n; ++j) {
+        constexpr int NW = TZ + TY + 2 * TXL;
+        const unsigned mz = __builtin_amdgcn_readfirstlane(__float_as_uint(s_w[j][NW]));
+        const unsigned my = __builtin_amdgcn_readfirstlane(__float_as_uint(s_w[j][NW + 1]));
+        const float2 val = val_b[j][v];
+        const float* w = s_w[j];
+        float2 vxk[TXL];
+        float wxl[TXL];
+#pragma unroll
+        for (int k = 0; k < TXL; ++k) {
+            const float wx = w[TZ + TY + half * TXL + k];
+            if constexpr (VX) vxk[k] = make_float2(wx * val.x, wx * val.y);
+            else wxl[k] = wx;
+        }
+        float wzr[WR ? TZ : 1], wyr[WR ? TY : 1];
+        if constexpr (WR) {
+#pragma unroll
+            for (int a = 0; a < TZ; ++a) wzr[a] = w[a];
+#pragma unroll
+            for (int b = 0; b < TY; ++b) wyr[b] = w[TZ + b];
+        }
+#pragma unroll
+        for (int a = 0; a < TZ; ++a) {
+            if (!((mz >> a) & 1u)) continue;
+            const float wz = WR ? wzr[WR ? a : 0] : w[a];
+#pragma unroll
+            for (int b = 0; b < TY; ++b) {
+                if (!((my >> b) & 1u)) continue;
+                const float wzy = wz * (WR ? wyr[WR ? b : 0] : w[TZ + b]);
+                if constexpr (VX) {
+#pragma unroll
+                    for (int k = 0; k < TXL; ++k) {
+                        acc[a][b][k].x += wzy * vxk[k].x;
+                        acc[a][b][k].y += wzy * vxk[k].y;
+                    }
+                } else {
+                    const float vx = wzy * val.x, vy = wzy * val.y;
+#pragma unroll
+                    for (int k = 0; k < TXL; ++k) {
+                        acc[a][b][k].x += wxl[k] * vx;
+                        acc[a][b][k].y += wxl[k] * vy;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int TZ, int TY, int TXL>
+__global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1)
+    k_spread_pipe(GeoParams g, BatchParams B, const int4* __restrict__ items, const int32_t* __restrict__ slist,
+                  const float* __restrict__ taprec, const float2* __restrict__ ys, float2* __restrict__ grid,
+                  float2* __restrict__ partial) {
+    constexpr int VL = SPREAD_VL;
+    constexpr int EB = SPIPE_EB;
+    __shared__ __attribute__((aligned(16))) float4 s_rec[EB][SPIPE_RP];
+    __shared__ __attribute__((aligned(16))) float2 s_val0[EB][VL];
+    __shared__ __attribute__((aligned(16))) float2 s_val1[EB][VL];
+    __shared__ __attribute__((aligned(16))) float s_w[EB][spipe_row<TZ, TY, TXL>()];
+
+    const int nvch = B.C / VL;  // plain values: one per coil
+    const unsigned bid = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int4 it = items[bid / nvch];
+    const int vch = bid % nvch;
+    const int lane = threadIdx.x;
+    const int v = lane & (VL - 1);
+    const int half = lane / VL;
+    const int u = vch * VL + v;
+
+    const int64_t key = it.x;
+    const int fg_frame = (int)(key / g.ntile_tot);
+    const int64_t tile = key % g.ntile_tot;
+    const int tix = (int)(tile % g.ntiles[2]);
+    const int tiy = (int)((tile / g.ntiles[2]) % g.ntiles[1]);
+    const int tiz = (int)(tile / ((int64_t)g.ntiles[2] * g.ntiles[1]));
+    const int oz = tiz * g.tile[0], oy = tiy * g.tile[1], ox = tix * g.tile[2];
+    const int ez = min(g.tile[0], g.os[0] - oz), ey = min(g.tile[1], g.os[1] - oy);
+    const int ex = min(g.tile[2], g.os[2] - ox);
+    const int rec4 = g.rec / 4;
+    const char* ybase = reinterpret_cast<const char*>(ys);
+
+    float2 acc[TZ][TY][TXL];
+#pragma unroll
+    for (int a = 0; a < TZ; ++a)
+#pragma unroll
+        for (int b = 0; b < TY; ++b)
+#pragma unroll
+            for (int k = 0; k < TXL; ++k) acc[a][b][k] = make_float2(0.f, 0.f);
+
+    // batch i covers entries [it.y + i*EB, ...): its records land in s_rec,
+    // its values in s_val0 (even i) or s_val1 (odd i).  Per batch: wait for
+    // its copies, turn records into weights, start the next batch's copies
+    // (records overwrite s_rec, which is no longer read), accumulate.
+    const int cnt = it.z - it.y;
+    const int32_t* sl = slist + it.y;  // the item's entries
+    int sidA = lane < cnt ? sl[lane] : 0;
+    int sidB = lane < cnt - EB ? sl[EB + lane] : 0;
+    spipe_issue(s_rec, s_val0, sidA, min(EB, cnt), rec4, taprec, ybase, B.C, vch);
+    for (int e0 = 0; e0 < cnt; e0 += 2 * EB) {
+        {
+            const int n = min(EB, cnt - e0);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            spipe_weights<TZ, TY, TXL>(s_rec, n, g, oz, oy, ox, s_w);
+            __syncthreads();
+            const int n1 = min(EB, cnt - e0 - EB);
+            if (n1 > 0) {
+                // ids first: a reload of a spilled register before them waits on
+                // nothing but itself
+                const int sidC = lane < cnt - e0 - 2 * EB ? sl[e0 + 2 * EB + lane] : 0;
+                spipe_issue(s_rec, s_val1, sidB, n1, rec4, taprec, ybase, B.C, vch);
+                sidA = sidC;
+            }
+            spipe_accumulate<TZ, TY, TXL>(s_val0, n, s_w, acc);
+        }
+        if (e0 + EB >= cnt) break;
+        {
+            const int n = min(EB, cnt - e0 - EB);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            spipe_weights<TZ, TY, TXL>(s_rec, n, g, oz, oy, ox, s_w);
+            __syncthreads();
+            const int n2 = min(EB, cnt - e0 - 2 * EB);
+            if (n2 > 0) {
+                const int sidC = lane < cnt - e0 - 3 * EB ? sl[e0 + 3 * EB + lane] : 0;
+                spipe_issue(s_rec, s_val0, sidA, n2, rec4, taprec, ybase, B.C, vch);
+                sidB = sidC;
+            }
+            spipe_accumulate<TZ, TY, TXL>(s_val1, n, s_w, acc);
+        }
+    }
+
+    if (it.w < 0) {
+        const int64_t slab = (int64_t)(B.nfg > 1 ? fg_frame : 0) * B.Cb + u / B.Cv;
+        float2* base = grid + slab * g.gtot * B.Cv + u % B.Cv;
+#pragma unroll
+        for (int a = 0; a < TZ; ++a)
+#pragma unroll
+            for (int b = 0; b < TY; ++b)
+#pragma unroll
+                for (int k = 0; k < TXL; ++k) {
+                    const int lx = half * TXL + k;
+                    if (a < ez && b < ey && lx < ex) {
+                        const int64_t gpt = ((int64_t)(oz + a) * g.os[1] + (oy + b)) * g.os[2] + (ox + lx);
+                        base[gpt * B.Cv] = acc[a][b][k];
                     }
                 }
     } else {
@@ -1073,8 +1344,12 @@ using RegLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchParam
 template <int TZ, int TY, int TXL>
 void launch_reg(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_slist& L,
                 const pf_plan* p, const float2* ys, float2* out, float2* partial) {
-    hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist, p->taprec,
-                       p->perm, ys, out, partial);
+    if (B.dma && PF_SPREAD_PIPE)
+        hipLaunchKernelGGL((k_spread_pipe<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist, p->taprec,
+                           ys, out, partial);
+    else
+        hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist, p->taprec,
+                           p->perm, ys, out, partial);
 }
 
 struct RegTile {

@@ -28,10 +28,12 @@ def _case(ndim):
 
 @pytest.mark.parametrize("ndim", [2, 3])
 @pytest.mark.parametrize("chunk", [0, 16])
-def test_all_spread_tiles_parity(cuda_device, ndim, chunk):
+@pytest.mark.parametrize("C", [20, 64])
+def test_all_spread_tiles_parity(cuda_device, ndim, chunk, C):
+    # C >= 16 values per point: the register spread runs; C = 64 (whole
+    # 32-coil chunks, two per tile) takes its software-pipelined form
     rng = np.random.default_rng(ndim)
     coord, shape = _case(ndim)
-    C = 20  # >= 16 values per point: the register spread runs
     P = R.Plan(coord, shape, width=5, oversamp=1.5)
     x = demo.complex_gaussian(rng, (C,) + shape)
     y = demo.complex_gaussian(rng, (C,) + coord.shape[:-1])
