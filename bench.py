@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--cpu-coils", type=int, default=32, help="coils in the CPU baseline sample")
     ap.add_argument("--small", action="store_true", help="64^3 x 8 coils quick run")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched / autograd comparisons")
+    ap.add_argument("--only", default=None, help="run one comparison of the extras by name and print its JSON")
     return ap.parse_args()
 
 
@@ -294,6 +295,44 @@ def bench_subspace(pf, dev, steps):
             "plain_ms": round(t_plain, 3), "fused_ms": round(t_fused, 3), "speedup": round(t_plain / t_fused, 2)}
 
 
+def bench_toeplitz(pf, dev, steps):
+    """2-D Toeplitz Gram: 48 variable-density spiral interleaves x 4096
+    samples, 256^2 x 16 coils, |k| density weights, kernel on the 512^2
+    grid.  sparse_gram (pad -> FFT -> per-voxel kernel multiply -> IFFT ->
+    crop, one native call) vs the explicit normal operator
+    nufft_adj(w * nufft(x)) it stands for, at the nufft defaults (W=4,
+    oversampling 1.25), with their relative difference (the two kernel
+    approximations) and the multiply's own byte count
+    2*C*(2N)^2*8 + (2N)^2*8."""
+    import torch
+
+    from pyfourier_amd import demo
+
+    N, C = 256, 16
+    coord = demo.variable_density_spiral_2d(48, 4096, N)
+    kr = np.sqrt((coord.astype(np.float64) ** 2).sum(-1))
+    w = torch.from_numpy((kr / kr.max()).astype(np.float32)).to(dev)
+    ct = torch.from_numpy(coord)
+    gm = pf.plan_spgram(ct, N, weight=w, oversamp=1.25, device=dev)
+    plan = pf.plan_nufft(ct, N, width=4, oversamp=1.25).to(dev)
+    x = torch.randn((C, N, N), dtype=torch.complex64, device=dev)
+
+    def gram():
+        return pf.sparse_gram(x, gm, device=dev)
+
+    def explicit():
+        return pf.nufft_adj(pf.nufft(x, nufft_plan=plan, device=dev) * w, nufft_plan=plan, device=dev)
+
+    a, b = gram(), explicit()
+    diff = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    t_g = _time(gram, dev, steps)
+    t_e = _time(explicit, dev, steps)
+    return {"workload": "2-D Toeplitz Gram, 256^2 x 16 coils, 48 spirals x 4096, kernel on 512^2",
+            "gram_ms": round(t_g, 3), "explicit_ms": round(t_e, 3), "speedup": round(t_e / t_g, 2),
+            "rel_diff_vs_explicit": round(diff, 5),
+            "multiply_bytes": 2 * C * (2 * N) ** 2 * 8 + (2 * N) ** 2 * 8}
+
+
 def bench_cartesian(pf, dev, steps):
     """Cartesian operators: masked centred FFT fwd+adj (fftn + ifftn) of
     256^2 x 32 coils with an R=4 mask, a 100-contrast K=4 subspace version
@@ -507,6 +546,11 @@ def bench_autograd(pf, dev, steps):
             "plain_ms": round(t_plain, 3), "autograd_ms": round(t_auto, 3), "speedup": round(t_plain / t_auto, 2)}
 
 
+EXTRAS = (("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
+          ("autograd_b0", bench_autograd), ("toeplitz", bench_toeplitz), ("cartesian", bench_cartesian),
+          ("grog_sparse", bench_grog_sparse))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -521,6 +565,13 @@ def main():
     from pyfourier_amd._nufft import stage_timer
 
     dev = torch.device("cuda", 0)
+    if args.only is not None:
+        fns = dict(EXTRAS)
+        if args.only not in fns:
+            print(f"--only takes one of {sorted(fns)}", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps({args.only: fns[args.only](pf, dev, args.steps)}), flush=True)
+        return
     N, C, NSP, NS = (64, 8, 2048, 128) if args.small else (128, 32, 8192, 256)
     x, xc, coord, w = make_inputs(N, C, NSP, NS)
     log(f"inputs ready: {N}^3 x {C} coils, {NSP}x{NS} samples")
@@ -570,9 +621,7 @@ def main():
     extras = None
     if not args.no_extras and not args.small:
         extras = {}
-        for name, fn in (("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
-                         ("autograd_b0", bench_autograd), ("cartesian", bench_cartesian),
-                         ("grog_sparse", bench_grog_sparse)):
+        for name, fn in EXTRAS:
             extras[name] = fn(pf, dev, max(3, args.steps // 2))
             log(f"{name}: {extras[name]}")
 

@@ -106,8 +106,9 @@ int pf_plan_create(const float* coord, int ndim, const int* shape, const int* wi
  *        the register-resident spread runs for the tiles listed in
  *        pf_spread_tiles, any other tile uses the LDS-staged spread.
  * chunk: spread-list entries per work item before a tile is split into
- *        chunks that a reduce pass sums in order (0: automatic, 128..1024
- *        from the size of the spread list). */
+ *        chunks that a reduce pass sums in order (0: automatic, 64..1024:
+ *        4x the mean tile load, or less so that the list makes ~4096 work
+ *        items). */
 int pf_plan_create_tiled(const float* coord, int ndim, const int* shape, const int* width,
                          const float* oversamp, int nframes, long long npts, const int* tile,
                          int chunk, void* stream, pf_plan** out);

@@ -609,16 +609,19 @@ constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 template <int ZY>  // 100 * TZ + TY
 constexpr bool spread_wreg() { return PF_SPREAD_WREG && ZY / 100 > 1 && ZY / 100 + ZY % 100 <= 12; }
 
-template <int TZ, int TY, int TXL>
+// VL values per wave (32, 16 or 8 for small batches): lane = (value, x part),
+// 64 / VL x parts of TXL points each across the tile's x extent.
+template <int TZ, int TY, int TXL, int VL = SPREAD_VL>
 __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_spread_reg(GeoParams g, BatchParams B, const int4* __restrict__ items,
                                                    const int32_t* __restrict__ slist,
                                                    const float* __restrict__ taprec,
                                                    const int32_t* __restrict__ perm,
                                                    const float2* __restrict__ ys, float2* __restrict__ grid,
                                                    float2* __restrict__ partial) {
-    constexpr int VL = SPREAD_VL;
+    static_assert(VL == 8 || VL == 16 || VL == 32, "values per wave");
+    constexpr int XP = 64 / VL;  // x parts
     constexpr int EB = SPREAD_EB;
-    constexpr int TX = 2 * TXL;
+    constexpr int TX = XP * TXL;
     constexpr int NW = TZ + TY + TX;  // tile-local tap weights of one entry
     static_assert(TZ <= 32 && TY <= 32, "row masks hold 32 bits per axis");
     __shared__ int s_idx[EB];
@@ -643,7 +646,7 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
     const int cb = uu / RC, rv = uu % RC;
     const int r = rv / B.Cv, cv = rv % B.Cv;
     const int ycol = cb * B.Cv + cv;
-    const bool dma = B.dma != 0;
+    const bool dma = VL == SPREAD_VL && B.dma != 0;  // whole 32-coil rows only
 
     const int64_t key = it.x;
     const int fg_frame = (int)(key / g.ntile_tot);
@@ -707,7 +710,7 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
         }
         __syncthreads();
         // phase 2: the batch's values
-        if (dma) {
+        if (VL == SPREAD_VL && dma) {
             // plain values, 32 whole coils per wave: copied global -> LDS
             // without registers, 16 lanes x 16 bytes per entry, 4 entries per
             // instruction, all 8 instructions in flight at once
@@ -727,8 +730,8 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
             }
             __builtin_amdgcn_s_waitcnt(0);
         } else {
-            // lane (value, half) takes entries half, half+2, ...
-            for (int j = half; j < n; j += 2) {
+            // lane (value, x part) takes entries xp, xp + XP, ...
+            for (int j = half; j < n; j += XP) {
                 const int s = s_idx[j];
                 float2 val = vok ? ys[(int64_t)s * B.C + ycol] : make_float2(0.f, 0.f);
                 if (B.basis || B.b0t) {
@@ -1336,50 +1339,87 @@ int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, fl
     return PF_OK;
 }
 
-// Register-spread instances: tile (z, y, x) held by one wave, x split in
-// two halves of TXL = x / 2 points.  64 complex accumulators per lane each.
+// Register-spread instances: tile (z, y, x) held by one wave of VL values
+// x (64 / VL) x parts of TXL points; VL = 32 holds 64 complex accumulators
+// per lane, the small-batch instances (VL = 16, 8) proportionally fewer.
 using RegLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchParams&, const pf_slist&, const pf_plan*,
                            const float2*, float2*, float2*);
 
-template <int TZ, int TY, int TXL>
+template <int TZ, int TY, int TXL, int VL>
 void launch_reg(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_slist& L,
                 const pf_plan* p, const float2* ys, float2* out, float2* partial) {
-    if (B.dma && PF_SPREAD_PIPE)
-        hipLaunchKernelGGL((k_spread_pipe<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist, p->taprec,
-                           ys, out, partial);
+    if constexpr (VL == SPREAD_VL) {
+        if (B.dma && PF_SPREAD_PIPE) {
+            hipLaunchKernelGGL((k_spread_pipe<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist,
+                               p->taprec, ys, out, partial);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL, VL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist, p->taprec,
+                       p->perm, ys, out, partial);
+}
+
+// the instance of tile (TZ, TY, TX) with VL values per wave, or null when the
+// tile's x extent has fewer points than the wave has x parts
+template <int TZ, int TY, int TX, int VL>
+constexpr RegLaunch reg_fn() {
+    if constexpr (TX * VL >= 64)
+        return &launch_reg<TZ, TY, TX * VL / 64, VL>;
     else
-        hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist, p->taprec,
-                           p->perm, ys, out, partial);
+        return nullptr;
 }
 
 struct RegTile {
     int ndim, z, y, x;
-    RegLaunch fn;
+    RegLaunch fn[3];  // VL = 32, 16, 8
 };
 
-const RegTile REG_TILES[] = {
-    {3, 8, 4, 4, &launch_reg<8, 4, 2>},   {3, 4, 4, 8, &launch_reg<4, 4, 4>},
-    {3, 4, 8, 4, &launch_reg<4, 8, 2>},   {3, 2, 4, 16, &launch_reg<2, 4, 8>},
-    {2, 1, 8, 16, &launch_reg<1, 8, 8>},  {2, 1, 16, 8, &launch_reg<1, 16, 4>},
-    {2, 1, 4, 32, &launch_reg<1, 4, 16>}, {2, 1, 32, 4, &launch_reg<1, 32, 2>},
-    {1, 1, 1, 64, &launch_reg<1, 1, 32>},
-};
-
-// The smallest register instance that holds the plan's tile, or null.
-static const RegTile* spread_reg_for(const pf_plan* p, const pf_batch& b) {
-    if (b.Cb * b.R * b.Cv < 16) return nullptr;
-    const RegTile* best = nullptr;
-    for (const RegTile& t : REG_TILES) {
-        if (t.ndim != p->ndim || p->tile[0] > t.z || p->tile[1] > t.y || p->tile[2] > t.x) continue;
-        if (!best || t.z * t.y * t.x < best->z * best->y * best->x) best = &t;
-    }
-    return best;
+template <int TZ, int TY, int TX>
+constexpr RegTile reg_tile(int ndim) {
+    return RegTile{ndim, TZ, TY, TX, {reg_fn<TZ, TY, TX, 32>(), reg_fn<TZ, TY, TX, 16>(), reg_fn<TZ, TY, TX, 8>()}};
 }
 
-static bool spread_reg_ok(const pf_plan* p, const pf_batch& b) { return spread_reg_for(p, b) != nullptr; }
+const RegTile REG_TILES[] = {
+    reg_tile<8, 4, 4>(3),  reg_tile<4, 4, 8>(3),  reg_tile<4, 8, 4>(3),  reg_tile<2, 4, 16>(3),
+    reg_tile<1, 8, 16>(2), reg_tile<1, 16, 8>(2), reg_tile<1, 4, 32>(2), reg_tile<1, 32, 4>(2),
+    reg_tile<1, 1, 64>(1),
+};
+
+// values per wave for V batch values: the smallest of 32, 16, 8 that holds
+// them (32 beyond), among those the tile has an instance for
+static int spread_vl_for(const RegTile& t, int V) {
+    int want = V > 16 ? 0 : (V > 8 ? 1 : 2);
+    while (want > 0 && !t.fn[want]) --want;
+    return want;
+}
+
+struct RegChoice {
+    const RegTile* tile = nullptr;
+    int vl = 0;  // values per wave
+    RegLaunch fn = nullptr;
+};
+
+// The smallest register instance that holds the plan's tile (batches of at
+// least 8 values per point), or an empty choice.
+static RegChoice spread_reg_choice(const pf_plan* p, const pf_batch& b) {
+    RegChoice c;
+    const int V = b.Cb * b.R * b.Cv;
+    if (V < 8) return c;
+    for (const RegTile& t : REG_TILES) {
+        if (t.ndim != p->ndim || p->tile[0] > t.z || p->tile[1] > t.y || p->tile[2] > t.x) continue;
+        if (!c.tile || t.z * t.y * t.x < c.tile->z * c.tile->y * c.tile->x) c.tile = &t;
+    }
+    if (c.tile) {
+        const int k = spread_vl_for(*c.tile, V);
+        c.vl = 32 >> k;
+        c.fn = c.tile->fn[k];
+    }
+    return c;
+}
 
 int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b) {
-    if (spread_reg_ok(p, b)) return SPREAD_VL;
+    const RegChoice rc = spread_reg_choice(p, b);
+    if (rc.fn) return rc.vl;
     const int tpts = p->tile[0] * p->tile[1] * p->tile[2];
     const int V = b.Cb * b.R * b.Cv;
     // accumulator (2*VC+1 words per point) within the LDS budget, multiple of 4
@@ -1433,7 +1473,8 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     if (nb > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many spread blocks");
     const int nsub = (int)(((int64_t)p->tile[0] * p->tile[1] * p->tile[2] * VC + TB - 1) / TB);
     if ((int64_t)L.nritems * nvch * nsub > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many reduce blocks");
-    if (const RegTile* rt = spread_reg_for(p, b)) {
+    const RegChoice rc = spread_reg_choice(p, b);
+    if (rc.fn) {
         // one value per coil (no basis, no B0) in whole 32-coil chunks: a
         // wave's values of a sample are one contiguous 256-byte run, copied
         // global -> LDS with 32-bit offsets (sorted samples under 4 GiB)
@@ -1441,7 +1482,7 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
                  (uint64_t)p->mtot * b.C * sizeof(float2) < (1ull << 32))
                     ? 1
                     : 0;
-        rt->fn(dim3((unsigned)nb), s, g, B, L, p, ys, grid, partial);
+        rc.fn(dim3((unsigned)nb), s, g, B, L, p, ys, grid, partial);
         PF_HIP(hipGetLastError());
         if (L.nritems > 0) {
             hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch * nsub)), dim3(TB), 0, s, g,

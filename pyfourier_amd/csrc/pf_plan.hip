@@ -396,13 +396,16 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
     // tile far heavier than the rest (the k-space centre of radial and
     // spiral trajectories) bounds the kernel unless it is split; splitting
     // tiles of ordinary weight only adds partial-tile traffic.  Split above
-    // four times the mean entries of a non-empty tile, within 128..1024.
+    // four times the mean entries of a non-empty tile, and small enough that
+    // the list makes at least ~4096 work items (a 2-D grid has few tiles: one
+    // wave per tile would leave most of the GPU idle), within 64..1024.
     int CH = p->max_chunk;
     if (CH <= 0) {
         int64_t nonempty = 0;
         for (uint64_t k = 0; k < nkeys; ++k) nonempty += he[k] > hs[k];
-        const int64_t c = 4 * NE / std::max<int64_t>(1, nonempty);
-        CH = (int)std::min<int64_t>(1024, std::max<int64_t>(128, c));
+        const int64_t heavy = 4 * NE / std::max<int64_t>(1, nonempty);
+        const int64_t fill = (NE + 4095) / 4096;
+        CH = (int)std::min<int64_t>(1024, std::max<int64_t>(64, std::min(heavy, fill)));
     }
     std::vector<int4> items, ritems;
     int slot = 0;

@@ -28,10 +28,12 @@ def _case(ndim):
 
 @pytest.mark.parametrize("ndim", [2, 3])
 @pytest.mark.parametrize("chunk", [0, 16])
-@pytest.mark.parametrize("C", [20, 64])
+@pytest.mark.parametrize("C", [8, 12, 20, 64])
 def test_all_spread_tiles_parity(cuda_device, ndim, chunk, C):
-    # C >= 16 values per point: the register spread runs; C = 64 (whole
-    # 32-coil chunks, two per tile) takes its software-pipelined form
+    # C >= 8 values per point: the register spread runs, with 8, 16 or 32
+    # values per wave (8 and 12 take the small-batch instances, or 16 where a
+    # tile is too narrow for 8 x parts); C = 64 (whole 32-coil chunks, two
+    # per tile) takes its software-pipelined form
     rng = np.random.default_rng(ndim)
     coord, shape = _case(ndim)
     P = R.Plan(coord, shape, width=5, oversamp=1.5)
