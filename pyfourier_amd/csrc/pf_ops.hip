@@ -595,6 +595,9 @@ constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 #ifndef PF_SPREAD_UNROLL
 #define PF_SPREAD_UNROLL 1
 #endif
+#ifndef PF_SPREAD_REG_MIN
+#define PF_SPREAD_REG_MIN 1  // below: the LDS-staged spread (measured slower from 1 value per point up)
+#endif
 #ifndef PF_SPREAD_PIPE
 #define PF_SPREAD_PIPE 1  // plain 32-coil batches: software-pipelined copies
 #endif
@@ -1399,12 +1402,12 @@ struct RegChoice {
     RegLaunch fn = nullptr;
 };
 
-// The smallest register instance that holds the plan's tile (batches of at
-// least 8 values per point), or an empty choice.
+// The smallest register instance that holds the plan's tile, or an empty
+// choice (tiles larger than every instance use the LDS-staged spread).
 static RegChoice spread_reg_choice(const pf_plan* p, const pf_batch& b) {
     RegChoice c;
     const int V = b.Cb * b.R * b.Cv;
-    if (V < 8) return c;
+    if (V < PF_SPREAD_REG_MIN) return c;
     for (const RegTile& t : REG_TILES) {
         if (t.ndim != p->ndim || p->tile[0] > t.z || p->tile[1] > t.y || p->tile[2] > t.x) continue;
         if (!c.tile || t.z * t.y * t.x < c.tile->z * c.tile->y * c.tile->x) c.tile = &t;

@@ -28,12 +28,12 @@ def _case(ndim):
 
 @pytest.mark.parametrize("ndim", [2, 3])
 @pytest.mark.parametrize("chunk", [0, 16])
-@pytest.mark.parametrize("C", [8, 12, 20, 64])
+@pytest.mark.parametrize("C", [1, 8, 12, 20, 64])
 def test_all_spread_tiles_parity(cuda_device, ndim, chunk, C):
-    # C >= 8 values per point: the register spread runs, with 8, 16 or 32
-    # values per wave (8 and 12 take the small-batch instances, or 16 where a
-    # tile is too narrow for 8 x parts); C = 64 (whole 32-coil chunks, two
-    # per tile) takes its software-pipelined form
+    # the register spread runs with 8, 16 or 32 values per wave (1, 8 and 12
+    # take the small-batch instances, or 16 where a tile is too narrow for 8
+    # x parts); C = 64 (whole 32-coil chunks, two per tile) takes its
+    # software-pipelined form
     rng = np.random.default_rng(ndim)
     coord, shape = _case(ndim)
     P = R.Plan(coord, shape, width=5, oversamp=1.5)
@@ -49,6 +49,23 @@ def test_all_spread_tiles_parity(cuda_device, ndim, chunk, C):
         xg = pf.nufft_adj(torch.from_numpy(y).to(cuda_device), nufft_plan=plan, device=cuda_device)
         assert rel(yg.cpu().numpy().reshape(C, 1, -1), yr) < PARITY, f"forward, tile {tile}"
         assert rel(xg.cpu().numpy().reshape(xr.shape), xr) < PARITY, f"adjoint, tile {tile}"
+
+
+@pytest.mark.parametrize("ndim,tile", [(2, (32, 32)), (3, (8, 8, 8))])
+@pytest.mark.parametrize("C", [1, 8, 20])
+def test_lds_spread_large_tiles(cuda_device, ndim, tile, C):
+    """Tiles larger than every register instance take the LDS-staged spread."""
+    rng = np.random.default_rng(10 + ndim + C)
+    coord, shape = _case(ndim)
+    P = R.Plan(coord, shape, width=5, oversamp=1.5)
+    y = demo.complex_gaussian(rng, (C,) + coord.shape[:-1])
+    xr = R.nufft_adj(y.reshape(C, 1, -1), P)
+    for chunk in (0, 16):
+        plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=5, oversamp=1.5, tile=tile, chunk=chunk)
+        plan.to(cuda_device)
+        assert tuple(plan.interpolator.native(cuda_device).info.tile)[3 - ndim:] == tuple(tile)
+        xg = pf.nufft_adj(torch.from_numpy(y).to(cuda_device), nufft_plan=plan, device=cuda_device)
+        assert rel(xg.cpu().numpy().reshape(xr.shape), xr) < PARITY, f"adjoint, tile {tile}, chunk {chunk}"
 
 
 def test_autotune_plan(cuda_device):
