@@ -527,16 +527,24 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_fftcol256(int N
 // the other way round.  The transpose buffer uses a 17-entry row stride so
 // both maps are free of LDS bank conflicts.
 // ---------------------------------------------------------------------------
-constexpr int X256_ROW = 17;
+constexpr int X256_ROW = 17;  // 16-column transpose strides (Cartesian row passes)
 constexpr int X256_PLANE = 16 * X256_ROW + 4;
+// columns per block of the row passes, forward and inverse (32 measured no
+// faster than 16 on the 3-D headline: 2.59 / 2.31 ms vs 2.57 / 2.29)
+constexpr int X256_BF = 16;
+constexpr int X256_BA = 16;
 
-template <bool FWD>
-__global__ void __launch_bounds__(256, PF_FFT_WPE) k_fftx256(Geo g, Batch bt, const float2* __restrict__ twg,
-                                                           const float* __restrict__ apod,
-                                                           const float2* __restrict__ in, float2* __restrict__ out,
-                                                           int KC, int YG, int nkc, float factor) {
+// BX columns per block (16 or 32): 16 x BX threads
+template <bool FWD, int BX>
+__global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g, Batch bt,
+                                                                         const float2* __restrict__ twg,
+                                                                         const float* __restrict__ apod,
+                                                                         const float2* __restrict__ in,
+                                                                         float2* __restrict__ out, int KC, int YG,
+                                                                         int nkc, float factor) {
     constexpr int G = 256;
-    __shared__ float2 sh[16 * X256_PLANE];
+    constexpr int ROW = BX + 1, PLANE = 16 * ROW + 4;  // transpose strides (float2)
+    __shared__ float2 sh[16 * PLANE];
     __shared__ float2 tw[G];
     const int N = g.N[2];
     const int RC = bt.R * bt.Cv;
@@ -549,7 +557,7 @@ __global__ void __launch_bounds__(256, PF_FFT_WPE) k_fftx256(Geo g, Batch bt, co
     const float* apz = apod;
     const float* apy = apod + g.N[0];
     const float* apx = apod + g.N[0] + g.N[1];
-    tw[threadIdx.x] = twg[threadIdx.x];
+    if (threadIdx.x < G) tw[threadIdx.x] = twg[threadIdx.x];
     const int lo = N - N / 2, hi = G - N / 2;
     // column b -> image row and (k, cv) pair
     auto column = [&](int b, int64_t& row, int& kc) {
@@ -572,8 +580,8 @@ __global__ void __launch_bounds__(256, PF_FFT_WPE) k_fftx256(Geo g, Batch bt, co
     };
     float2 v[16];
     {
-        const int t = FWD ? (threadIdx.x & 15) : (threadIdx.x >> 4);
-        const int b = FWD ? (threadIdx.x >> 4) : (threadIdx.x & 15);
+        const int t = FWD ? (threadIdx.x & 15) : (threadIdx.x / BX);
+        const int b = FWD ? (threadIdx.x >> 4) : (threadIdx.x % BX);
         int64_t row;
         int kc;
         const bool ok = column(b, row, kc);
@@ -599,13 +607,13 @@ __global__ void __launch_bounds__(256, PF_FFT_WPE) k_fftx256(Geo g, Batch bt, co
 #pragma unroll
         for (int m1 = 1; m1 < 16; ++m1) v[m1] = c_mul(v[m1], tw[t * m1]);
 #pragma unroll
-        for (int m1 = 0; m1 < 16; ++m1) sh[m1 * X256_PLANE + t * X256_ROW + b] = v[m1];
+        for (int m1 = 0; m1 < 16; ++m1) sh[m1 * PLANE + t * ROW + b] = v[m1];
     }
     __syncthreads();
-    const int m1 = FWD ? (threadIdx.x >> 4) : (threadIdx.x & 15);
-    const int b = FWD ? (threadIdx.x & 15) : (threadIdx.x >> 4);
+    const int m1 = FWD ? (threadIdx.x / BX) : (threadIdx.x & 15);
+    const int b = FWD ? (threadIdx.x % BX) : (threadIdx.x >> 4);
 #pragma unroll
-    for (int tp = 0; tp < 16; ++tp) v[tp] = sh[m1 * X256_PLANE + tp * X256_ROW + b];
+    for (int tp = 0; tp < 16; ++tp) v[tp] = sh[m1 * PLANE + tp * ROW + b];
     dft16(v);  // v[m2] = X[m1 + 16 m2] of column b
     int64_t row;
     int kc;
@@ -958,8 +966,9 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
     const pf_fftk_axis& X = p->fftk->ax[2];
     const int KCv = b.K * b.Cv;
     if (X.G == 256 && b.L == 1 && !b.b0s) {
-        const int KC = std::min(KCv, 16);
-        const int YG = 16 / KC;
+        const int BX = fwd ? X256_BF : X256_BA;
+        const int KC = std::min(KCv, BX);
+        const int YG = BX / KC;
         const int nkc = (KCv + KC - 1) / KC;
         const int64_t rows = (int64_t)b.nfg * b.Cb * p->shape[0] * p->shape[1];
         const int64_t nblk = ((rows + YG - 1) / YG) * nkc;
@@ -967,11 +976,11 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         Geo g = geo_of(p);
         Batch t = batch_of(b);
         if (fwd)
-            hipLaunchKernelGGL(k_fftx256<true>, dim3((unsigned)nblk), dim3(256), 0, s, g, t, X.tw, p->apod, src, dst,
-                               KC, YG, nkc, 1.0f);
+            hipLaunchKernelGGL((k_fftx256<true, X256_BF>), dim3((unsigned)nblk), dim3(16 * X256_BF), 0, s, g, t,
+                               X.tw, p->apod, src, dst, KC, YG, nkc, 1.0f);
         else
-            hipLaunchKernelGGL(k_fftx256<false>, dim3((unsigned)nblk), dim3(256), 0, s, g, t, X.tw, p->apod, src, dst,
-                               KC, YG, nkc, factor);
+            hipLaunchKernelGGL((k_fftx256<false, X256_BA>), dim3((unsigned)nblk), dim3(16 * X256_BA), 0, s, g, t,
+                               X.tw, p->apod, src, dst, KC, YG, nkc, factor);
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
