@@ -295,6 +295,49 @@ def bench_subspace(pf, dev, steps):
             "plain_ms": round(t_plain, 3), "fused_ms": round(t_fused, 3), "speedup": round(t_plain / t_fused, 2)}
 
 
+def bench_autograd_3d(pf, dev, steps):
+    """3-D radial 128^3 x 8 coils (golden-means 8192 x 256), W=4, sigma 2: the
+    same L2 data-consistency loss with image and coordinate gradients as
+    bench_autograd, through the autograd function vs the plain calls (adjoint
+    for the image, one forward NUFFT of the ramp-weighted image per axis,
+    here three)."""
+    import torch
+
+    from pyfourier_amd import demo
+
+    N, C = 128, 8
+    coord = demo.golden_means_radial_3d(8192, 256)
+    plan = pf.plan_nufft(torch.from_numpy(coord), N, width=4, oversamp=2.0).to(dev)
+    x0 = torch.randn((C, N, N, N), dtype=torch.complex64, device=dev)
+    target = pf.nufft(x0, nufft_plan=plan, device=dev)
+    x = (x0 * 0.9).requires_grad_()
+    ct = torch.from_numpy(coord).to(dev).requires_grad_()
+
+    def autograd():
+        x.grad = None
+        ct.grad = None
+        loss = (pf.nufft(x, ct, nufft_plan=plan, device=dev) - target).abs().pow(2).sum()
+        loss.backward()
+
+    r = torch.arange(N, device=dev, dtype=torch.float32) - N // 2
+    ramps = [(-2j * math.pi / N) * r.reshape(1, 1, 1, N), (-2j * math.pi / N) * r.reshape(1, 1, N, 1),
+             (-2j * math.pi / N) * r.reshape(1, N, 1, 1)]
+    unit = N / (2 * math.pi)
+
+    def plain():
+        xd = x.detach()
+        res = pf.nufft(xd, nufft_plan=plan, device=dev) - target
+        gx = 2 * pf.nufft_adj(res, nufft_plan=plan, device=dev)
+        gk = [(2 * res.conj() * pf.nufft(ramp * xd, nufft_plan=plan, device=dev)).real.sum(0) * unit
+              for ramp in ramps]
+        return gx, gk
+
+    t_auto = _time(autograd, dev, steps)
+    t_plain = _time(plain, dev, steps)
+    return {"workload": "3-D radial 128^3 x 8 coils, golden-means 8192x256, image + coordinate gradients, W=4",
+            "plain_ms": round(t_plain, 3), "autograd_ms": round(t_auto, 3), "speedup": round(t_plain / t_auto, 2)}
+
+
 def bench_toeplitz(pf, dev, steps):
     """2-D Toeplitz Gram: 48 variable-density spiral interleaves x 4096
     samples, 256^2 x 16 coils, |k| density weights, kernel on the 512^2
@@ -547,7 +590,8 @@ def bench_autograd(pf, dev, steps):
 
 
 EXTRAS = (("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
-          ("autograd_b0", bench_autograd), ("toeplitz", bench_toeplitz), ("cartesian", bench_cartesian),
+          ("autograd_b0", bench_autograd), ("autograd_3d", bench_autograd_3d), ("toeplitz", bench_toeplitz),
+          ("cartesian", bench_cartesian),
           ("grog_sparse", bench_grog_sparse))
 
 

@@ -245,10 +245,13 @@ __global__ void __launch_bounds__(TB) k_gather_dcoord_w(GradParams P, const floa
 #pragma unroll
                 for (int b = 0; b < WY; ++b) {
                     const float2* py = pz + yo[b];
-                    const float wzy = wz * wy[b], dzy = dz * wy[b], wzdy = wz * dy[b];
+                    // the row's x-weighted and x-derivative-weighted sums,
+                    // folded into the three derivatives once per row
+                    float2 s0[CPL], s1[CPL];
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) s0[q] = s1[q] = make_float2(0.f, 0.f);
 #pragma unroll
                     for (int c = 0; c < WX; ++c) {
-                        const float tz = dzy * wx[c], ty = wzdy * wx[c], tx = wzy * dx[c];
                         float2 g[CPL];
                         if constexpr (CPL == 2) {
                             const float4 v = *reinterpret_cast<const float4*>(py + xo[c]);
@@ -259,13 +262,21 @@ __global__ void __launch_bounds__(TB) k_gather_dcoord_w(GradParams P, const floa
                         }
 #pragma unroll
                         for (int q = 0; q < CPL; ++q) {
-                            acc[0][q].x += tz * g[q].x;
-                            acc[0][q].y += tz * g[q].y;
-                            acc[1][q].x += ty * g[q].x;
-                            acc[1][q].y += ty * g[q].y;
-                            acc[2][q].x += tx * g[q].x;
-                            acc[2][q].y += tx * g[q].y;
+                            s0[q].x += wx[c] * g[q].x;
+                            s0[q].y += wx[c] * g[q].y;
+                            s1[q].x += dx[c] * g[q].x;
+                            s1[q].y += dx[c] * g[q].y;
                         }
+                    }
+                    const float wzy = wz * wy[b], dzy = dz * wy[b], wzdy = wz * dy[b];
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) {
+                        acc[0][q].x += dzy * s0[q].x;
+                        acc[0][q].y += dzy * s0[q].y;
+                        acc[1][q].x += wzdy * s0[q].x;
+                        acc[1][q].y += wzdy * s0[q].y;
+                        acc[2][q].x += wzy * s1[q].x;
+                        acc[2][q].y += wzy * s1[q].y;
                     }
                 }
             }

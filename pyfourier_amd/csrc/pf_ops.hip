@@ -612,9 +612,11 @@ constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 template <int ZY>  // 100 * TZ + TY
 constexpr bool spread_wreg() { return PF_SPREAD_WREG && ZY / 100 > 1 && ZY / 100 + ZY % 100 <= 12; }
 
-// VL values per wave (32, 16 or 8 for small batches): lane = (value, x part),
-// 64 / VL x parts of TXL points each across the tile's x extent.
-template <int TZ, int TY, int TXL, int VL = SPREAD_VL>
+// VL values per wave (32, 16 or 8 for small batches): lane = (value, part),
+// 64 / VL parts; a part is (z part, x part): ZP z parts of TZ / ZP rows (ZP > 1
+// only where the tile's x extent is narrower than 64 / VL) and the rest x
+// parts of TXL points each.
+template <int TZ, int TY, int TXL, int VL = SPREAD_VL, int ZP = 1>
 __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_spread_reg(GeoParams g, BatchParams B, const int4* __restrict__ items,
                                                    const int32_t* __restrict__ slist,
                                                    const float* __restrict__ taprec,
@@ -622,7 +624,10 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
                                                    const float2* __restrict__ ys, float2* __restrict__ grid,
                                                    float2* __restrict__ partial) {
     static_assert(VL == 8 || VL == 16 || VL == 32, "values per wave");
-    constexpr int XP = 64 / VL;  // x parts
+    static_assert((64 / VL) % ZP == 0 && TZ % ZP == 0, "z parts");
+    constexpr int P = 64 / VL;    // parts per value
+    constexpr int XP = P / ZP;    // x parts
+    constexpr int TZL = TZ / ZP;  // z rows per lane
     constexpr int EB = SPREAD_EB;
     constexpr int TX = XP * TXL;
     constexpr int NW = TZ + TY + TX;  // tile-local tap weights of one entry
@@ -642,7 +647,9 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
     const int vch = bid % nvch;
     const int lane = threadIdx.x;
     const int v = lane & (VL - 1);
-    const int half = lane / VL;
+    const int part = lane / VL;
+    const int half = part % XP;  // x part
+    const int zp = part / XP;    // z part
     const int u = vch * VL + v;
     const bool vok = u < V;
     const int uu = vok ? u : V - 1;
@@ -661,9 +668,9 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
     const int ez = min(g.tile[0], g.os[0] - oz), ey = min(g.tile[1], g.os[1] - oy);
     const int ex = min(g.tile[2], g.os[2] - ox);
 
-    float2 acc[TZ][TY][TXL];
+    float2 acc[TZL][TY][TXL];
 #pragma unroll
-    for (int a = 0; a < TZ; ++a)
+    for (int a = 0; a < TZL; ++a)
 #pragma unroll
         for (int b = 0; b < TY; ++b)
 #pragma unroll
@@ -733,8 +740,8 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
             }
             __builtin_amdgcn_s_waitcnt(0);
         } else {
-            // lane (value, x part) takes entries xp, xp + XP, ...
-            for (int j = half; j < n; j += XP) {
+            // lane (value, part) takes entries part, part + P, ...
+            for (int j = part; j < n; j += P) {
                 const int s = s_idx[j];
                 float2 val = vok ? ys[(int64_t)s * B.C + ycol] : make_float2(0.f, 0.f);
                 if (B.basis || B.b0t) {
@@ -776,9 +783,21 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
                 for (int b = 0; b < TY; ++b) wyr[b] = w[TZ + b];
             }
 #pragma unroll
-            for (int a = 0; a < TZ; ++a) {
-                if (!((mz >> a) & 1u)) continue;
-                const float wz = WR ? wzr[WR ? a : 0] : w[a];
+            for (int a = 0; a < TZL; ++a) {
+                // the row pass runs when any z part's row a is inside the
+                // footprint; a lane whose own row is outside reads weight 0
+                unsigned rows = 0;
+#pragma unroll
+                for (int zq = 0; zq < ZP; ++zq) rows |= mz >> (zq * TZL + a);
+                if (!(rows & 1u)) continue;
+                float wz;
+                if constexpr (WR) {
+                    wz = wzr[a];
+#pragma unroll
+                    for (int zq = 1; zq < ZP; ++zq) wz = zp == zq ? wzr[WR ? zq * TZL + a : 0] : wz;
+                } else {
+                    wz = w[zp * TZL + a];
+                }
 #pragma unroll
                 for (int b = 0; b < TY; ++b) {
                     if (!((my >> b) & 1u)) continue;
@@ -808,14 +827,15 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
         const int64_t slab = (int64_t)(B.nfg > 1 ? fg_frame : 0) * B.Cb + cb;
         float2* base = grid + slab * g.gtot * RC + rv;
 #pragma unroll
-        for (int a = 0; a < TZ; ++a)
+        for (int a = 0; a < TZL; ++a)
 #pragma unroll
             for (int b = 0; b < TY; ++b)
 #pragma unroll
                 for (int k = 0; k < TXL; ++k) {
                     const int lx = half * TXL + k;
-                    if (a < ez && b < ey && lx < ex) {
-                        const int64_t gpt = ((int64_t)(oz + a) * g.os[1] + (oy + b)) * g.os[2] + (ox + lx);
+                    const int lz = zp * TZL + a;
+                    if (lz < ez && b < ey && lx < ex) {
+                        const int64_t gpt = ((int64_t)(oz + lz) * g.os[1] + (oy + b)) * g.os[2] + (ox + lx);
                         base[gpt * RC] = acc[a][b][k];
                     }
                 }
@@ -823,13 +843,14 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1) k_sp
         const int Tx = g.tile[2], Ty = g.tile[1], Tz = g.tile[0];
         float2* dst = partial + ((int64_t)it.w * nvch + vch) * (int64_t)(Tx * Ty * Tz) * VL + v;
 #pragma unroll
-        for (int a = 0; a < TZ; ++a)
+        for (int a = 0; a < TZL; ++a)
 #pragma unroll
             for (int b = 0; b < TY; ++b)
 #pragma unroll
                 for (int k = 0; k < TXL; ++k) {
                     const int lx = half * TXL + k;
-                    if (a < Tz && b < Ty && lx < Tx) dst[(int64_t)((a * Ty + b) * Tx + lx) * VL] = acc[a][b][k];
+                    const int lz = zp * TZL + a;
+                    if (lz < Tz && b < Ty && lx < Tx) dst[(int64_t)((lz * Ty + b) * Tx + lx) * VL] = acc[a][b][k];
                 }
     }
 }
@@ -1348,26 +1369,29 @@ int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, fl
 using RegLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchParams&, const pf_slist&, const pf_plan*,
                            const float2*, float2*, float2*);
 
-template <int TZ, int TY, int TXL, int VL>
+template <int TZ, int TY, int TXL, int VL, int ZP>
 void launch_reg(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_slist& L,
                 const pf_plan* p, const float2* ys, float2* out, float2* partial) {
-    if constexpr (VL == SPREAD_VL) {
+    if constexpr (VL == SPREAD_VL && ZP == 1) {
         if (B.dma && PF_SPREAD_PIPE) {
             hipLaunchKernelGGL((k_spread_pipe<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist,
                                p->taprec, ys, out, partial);
             return;
         }
     }
-    hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL, VL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist, p->taprec,
-                       p->perm, ys, out, partial);
+    hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL, VL, ZP>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist,
+                       p->taprec, p->perm, ys, out, partial);
 }
 
-// the instance of tile (TZ, TY, TX) with VL values per wave, or null when the
-// tile's x extent has fewer points than the wave has x parts
+// the instance of tile (TZ, TY, TX) with VL values per wave: the 64 / VL
+// parts split x, or x and z in two halves when the x extent is too narrow;
+// null when neither fits
 template <int TZ, int TY, int TX, int VL>
 constexpr RegLaunch reg_fn() {
     if constexpr (TX * VL >= 64)
-        return &launch_reg<TZ, TY, TX * VL / 64, VL>;
+        return &launch_reg<TZ, TY, TX * VL / 64, VL, 1>;
+    else if constexpr (TX * VL * 2 >= 64 && TZ % 2 == 0)
+        return &launch_reg<TZ, TY, TX * VL * 2 / 64, VL, 2>;
     else
         return nullptr;
 }
