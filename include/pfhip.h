@@ -103,8 +103,8 @@ int pf_plan_create(const float* coord, int ndim, const int* shape, const int* wi
  * autotuner) instead of the built-in default; same reference interface as
  * pf_plan_create, whose plan_nufft has no such knob.
  * tile:  host array of ndim extents (z, y, x order) or NULL for the default;
- *        the register-resident spread runs for the tiles listed in
- *        pf_spread_tiles, any other tile uses the LDS-staged spread.
+ *        a tile that fits inside one listed by pf_spread_tiles uses the
+ *        register-resident spread, a larger one the LDS-staged spread.
  * chunk: spread-list entries per work item before a tile is split into
  *        chunks that a reduce pass sums in order (0: automatic, 64..1024:
  *        4x the mean tile load, or less so that the list makes ~4096 work
