@@ -218,8 +218,8 @@ int pf_plan_get_info(const pf_plan* p, pf_plan_info* info) {
     info->nframes = p->nframes;
     info->npts = p->npts;
     info->spread_entries = p->sl[0].nslist;
-    info->spread_items = p->sl[0].nsitems;
-    info->spread_slots = p->sl[0].nslots;
+    info->spread_items = p->sl[0].split[1].nsitems;
+    info->spread_slots = p->sl[0].split[1].nslots;
     return PF_OK;
 }
 

@@ -35,15 +35,31 @@ struct pf_fft_cache;
 // footprint touches the tile, split into chunks of at most max_chunk entries.
 // Tiles with several chunks accumulate into partial tiles that a second
 // kernel sums in chunk order (deterministic).
-struct pf_slist {
-    int32_t* slist = nullptr;    // sorted sample ids grouped by key
-    int64_t nslist = 0;
+// spread work items over one sorted entry list
+struct pf_sitems {
     int4* sitems = nullptr;      // (key, e0, e1, slot or -1)
     int nsitems = 0;
     int4* ritems = nullptr;      // (key, slot0, nslot, 0)
     int nritems = 0;
     int nslots = 0;
 };
+
+struct pf_slist {
+    int32_t* slist = nullptr;    // sorted sample ids grouped by key
+    int64_t nslist = 0;
+    // [0] heavy tiles split; [1] split further so that the items alone fill
+    // the GPU (the same arrays when that changes nothing)
+    pf_sitems split[2];
+};
+
+// waves of a spread launch above which the coarser split is used: a batch
+// with several value chunks per tile fills the GPU without splitting tiles
+// further, and every extra split costs a partial tile per value chunk
+constexpr int64_t PF_SPREAD_FILL = 8192;
+
+static inline const pf_sitems& pf_sitems_for(const pf_slist& L, int nvch) {
+    return (int64_t)L.split[0].nsitems * nvch >= PF_SPREAD_FILL ? L.split[0] : L.split[1];
+}
 
 struct pf_plan {
     int ndim = 0;          // spatial dims 1..3

@@ -1366,20 +1366,20 @@ int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, fl
 // Register-spread instances: tile (z, y, x) held by one wave of VL values
 // x (64 / VL) x parts of TXL points; VL = 32 holds 64 complex accumulators
 // per lane, the small-batch instances (VL = 16, 8) proportionally fewer.
-using RegLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchParams&, const pf_slist&, const pf_plan*,
-                           const float2*, float2*, float2*);
+using RegLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchParams&, const pf_slist&,
+                           const pf_sitems&, const pf_plan*, const float2*, float2*, float2*);
 
 template <int TZ, int TY, int TXL, int VL, int ZP>
 void launch_reg(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_slist& L,
-                const pf_plan* p, const float2* ys, float2* out, float2* partial) {
+                const pf_sitems& I, const pf_plan* p, const float2* ys, float2* out, float2* partial) {
     if constexpr (VL == SPREAD_VL && ZP == 1) {
         if (B.dma && PF_SPREAD_PIPE) {
-            hipLaunchKernelGGL((k_spread_pipe<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist,
+            hipLaunchKernelGGL((k_spread_pipe<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, I.sitems, L.slist,
                                p->taprec, ys, out, partial);
             return;
         }
     }
-    hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL, VL, ZP>), grid, dim3(64), 0, s, g, B, L.sitems, L.slist,
+    hipLaunchKernelGGL((k_spread_reg<TZ, TY, TXL, VL, ZP>), grid, dim3(64), 0, s, g, B, I.sitems, L.slist,
                        p->taprec, p->perm, ys, out, partial);
 }
 
@@ -1468,7 +1468,7 @@ size_t pf_spread_partial_bytes(const pf_plan* p, const pf_batch& b) {
     const int V = b.Cb * b.R * b.Cv;
     const int nvch = (V + VC - 1) / VC;
     const size_t tpts = (size_t)p->tile[0] * p->tile[1] * p->tile[2];
-    return (size_t)pf_slist_for(p, b).nslots * nvch * tpts * VC * sizeof(float2);
+    return (size_t)pf_sitems_for(pf_slist_for(p, b), nvch).nslots * nvch * tpts * VC * sizeof(float2);
 }
 
 int pf_spread_reg_tiles(int ndim, int* tiles, int cap) {
@@ -1495,11 +1495,12 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     const size_t lds = spread_lds_bytes(p, VC);
     if (lds > 160 * 1024) return pf_fail(PF_ERR_UNSUPPORTED, "spread tile exceeds LDS");
     const pf_slist& L = pf_slist_for(p, b);
-    if (L.nslots > 0 && !partial) return pf_fail(PF_ERR_ARG, "spread needs a partial-tile workspace");
-    const int64_t nb = (int64_t)L.nsitems * nvch;
+    const pf_sitems& I = pf_sitems_for(L, nvch);
+    if (I.nslots > 0 && !partial) return pf_fail(PF_ERR_ARG, "spread needs a partial-tile workspace");
+    const int64_t nb = (int64_t)I.nsitems * nvch;
     if (nb > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many spread blocks");
     const int nsub = (int)(((int64_t)p->tile[0] * p->tile[1] * p->tile[2] * VC + TB - 1) / TB);
-    if ((int64_t)L.nritems * nvch * nsub > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many reduce blocks");
+    if ((int64_t)I.nritems * nvch * nsub > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many reduce blocks");
     const RegChoice rc = spread_reg_choice(p, b);
     if (rc.fn) {
         // one value per coil (no basis, no B0) in whole 32-coil chunks: a
@@ -1509,11 +1510,11 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
                  (uint64_t)p->mtot * b.C * sizeof(float2) < (1ull << 32))
                     ? 1
                     : 0;
-        rc.fn(dim3((unsigned)nb), s, g, B, L, p, ys, grid, partial);
+        rc.fn(dim3((unsigned)nb), s, g, B, L, I, p, ys, grid, partial);
         PF_HIP(hipGetLastError());
-        if (L.nritems > 0) {
-            hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch * nsub)), dim3(TB), 0, s, g,
-                               B, VC, nsub, L.ritems, partial, grid);
+        if (I.nritems > 0) {
+            hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)I.nritems * nvch * nsub)), dim3(TB), 0, s, g,
+                               B, VC, nsub, I.ritems, partial, grid);
             PF_HIP(hipGetLastError());
         }
         return PF_OK;
@@ -1524,12 +1525,12 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_done = true;
     }
-    hipLaunchKernelGGL(k_spread, dim3((unsigned)nb), dim3(TB), lds, s, g, B, VC, L.sitems, L.slist, p->taprec,
+    hipLaunchKernelGGL(k_spread, dim3((unsigned)nb), dim3(TB), lds, s, g, B, VC, I.sitems, L.slist, p->taprec,
                        p->perm, ys, grid, partial);
     PF_HIP(hipGetLastError());
-    if (L.nritems > 0) {
-        hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)L.nritems * nvch * nsub)), dim3(TB), 0, s, g, B,
-                           VC, nsub, L.ritems, partial, grid);
+    if (I.nritems > 0) {
+        hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)I.nritems * nvch * nsub)), dim3(TB), 0, s, g, B,
+                           VC, nsub, I.ritems, partial, grid);
         PF_HIP(hipGetLastError());
     }
     return PF_OK;

@@ -395,20 +395,21 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
     // entries per work item: a chunk is summed serially by one wave, so a
     // tile far heavier than the rest (the k-space centre of radial and
     // spiral trajectories) bounds the kernel unless it is split; splitting
-    // tiles of ordinary weight only adds partial-tile traffic.  Split above
-    // four times the mean entries of a non-empty tile, and small enough that
-    // the list makes at least ~4096 work items (a 2-D grid has few tiles: one
-    // wave per tile would leave most of the GPU idle), within 64..1024.
-    int CH = p->max_chunk;
-    if (CH <= 0) {
+    // tiles of ordinary weight only adds partial-tile traffic.  Split [0]:
+    // above four times the mean entries of a non-empty tile, within
+    // 128..1024.  Split [1]: also small enough that the list makes at least
+    // ~4096 work items (a 2-D grid has few tiles: one wave per tile would
+    // leave most of the GPU idle), within 64..1024; the launch picks it for
+    // batches whose value chunks do not fill the GPU anyway.
+    int CH[2] = {p->max_chunk, p->max_chunk};
+    if (p->max_chunk <= 0) {
         int64_t nonempty = 0;
         for (uint64_t k = 0; k < nkeys; ++k) nonempty += he[k] > hs[k];
         const int64_t heavy = 4 * NE / std::max<int64_t>(1, nonempty);
         const int64_t fill = (NE + 4095) / 4096;
-        CH = (int)std::min<int64_t>(1024, std::max<int64_t>(64, std::min(heavy, fill)));
+        CH[0] = (int)std::min<int64_t>(1024, std::max<int64_t>(128, heavy));
+        CH[1] = (int)std::min<int64_t>(1024, std::max<int64_t>(64, std::min(heavy, fill)));
     }
-    std::vector<int4> items, ritems;
-    int slot = 0;
     // visit tiles in the sample order's Z-order (when it is used), frame by frame
     std::vector<uint64_t> order(nkeys);
     {
@@ -425,29 +426,38 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
         for (uint64_t f = 0; f < nkeys / p->ntile_tot; ++f)
             for (int64_t t = 0; t < p->ntile_tot; ++t) order[q++] = f * p->ntile_tot + tk[t].second;
     }
-    for (uint64_t k : order) {
-        int e0 = hs[k], e1 = he[k];
-        int n = e1 - e0;
-        if (n <= CH) {
-            items.push_back(make_int4((int)k, e0, e1, -1));
-        } else {
-            int nch = (n + CH - 1) / CH;
-            ritems.push_back(make_int4((int)k, slot, nch, 0));
-            for (int c = 0; c < nch; ++c) {
-                int a = e0 + (int)((int64_t)n * c / nch);
-                int b = e0 + (int)((int64_t)n * (c + 1) / nch);
-                items.push_back(make_int4((int)k, a, b, slot++));
+    for (int m = 0; m < 2; ++m) {
+        pf_sitems& S = out.split[m];
+        if (m == 1 && CH[1] == CH[0]) {
+            S = out.split[0];  // same arrays
+            break;
+        }
+        std::vector<int4> items, ritems;
+        int slot = 0;
+        for (uint64_t k : order) {
+            int e0 = hs[k], e1 = he[k];
+            int n = e1 - e0;
+            if (n <= CH[m]) {
+                items.push_back(make_int4((int)k, e0, e1, -1));
+            } else {
+                int nch = (n + CH[m] - 1) / CH[m];
+                ritems.push_back(make_int4((int)k, slot, nch, 0));
+                for (int c = 0; c < nch; ++c) {
+                    int a = e0 + (int)((int64_t)n * c / nch);
+                    int b = e0 + (int)((int64_t)n * (c + 1) / nch);
+                    items.push_back(make_int4((int)k, a, b, slot++));
+                }
             }
         }
+        S.nsitems = (int)items.size();
+        S.nritems = (int)ritems.size();
+        S.nslots = slot;
+        PF_HIP(hipMalloc(&S.sitems, std::max<size_t>(1, items.size()) * sizeof(int4)));
+        PF_HIP(hipMemcpy(S.sitems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+        PF_HIP(hipMalloc(&S.ritems, std::max<size_t>(1, ritems.size()) * sizeof(int4)));
+        if (!ritems.empty())
+            PF_HIP(hipMemcpy(S.ritems, ritems.data(), ritems.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
-    out.nsitems = (int)items.size();
-    out.nritems = (int)ritems.size();
-    out.nslots = slot;
-    PF_HIP(hipMalloc(&out.sitems, std::max<size_t>(1, items.size()) * sizeof(int4)));
-    PF_HIP(hipMemcpy(out.sitems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
-    PF_HIP(hipMalloc(&out.ritems, std::max<size_t>(1, ritems.size()) * sizeof(int4)));
-    if (!ritems.empty())
-        PF_HIP(hipMemcpy(out.ritems, ritems.data(), ritems.size() * sizeof(int4), hipMemcpyHostToDevice));
     return PF_OK;
 }
 
@@ -531,9 +541,13 @@ void pf_plan_free(pf_plan* p) {
         if (q) (void)hipFree(q);
     const bool alias = p->sl[1].slist == p->sl[0].slist;
     for (int m = 0; m < (alias ? 1 : 2); ++m) {
-        void* sp[] = {p->sl[m].slist, p->sl[m].sitems, p->sl[m].ritems};
-        for (void* q : sp)
-            if (q) (void)hipFree(q);
+        const pf_slist& L = p->sl[m];
+        if (L.slist) (void)hipFree(L.slist);
+        const bool same = L.split[1].sitems == L.split[0].sitems;
+        for (int k = 0; k < (same ? 1 : 2); ++k) {
+            if (L.split[k].sitems) (void)hipFree(L.split[k].sitems);
+            if (L.split[k].ritems) (void)hipFree(L.split[k].ritems);
+        }
     }
     if (p->fft) pf_fft_destroy(p->fft);
     if (p->fftk) pf_fftk_destroy(p->fftk);
