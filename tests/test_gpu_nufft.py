@@ -122,3 +122,25 @@ def test_nudft_accuracy_2d(cuda_device):
         ct = plan.interpolator.coord.reshape(-1, 2).numpy()
         yt = U.nudft(x[None], ct, [N, N])[0]
         assert rel(y.cpu().numpy().ravel(), yt) < bound
+
+
+@pytest.mark.parametrize("C,nsp,ns", [(4, 36, 64), (4, 37, 63), (3, 36, 64), (2, 1, 1)],
+                         ids=["16-byte", "odd-samples", "odd-batch", "one-sample"])
+def test_permutation_paths(cuda_device, C, nsp, ns):
+    """Sample permutations between user and tile order: 16-byte pieces for
+    even batch and sample counts, 8-byte pieces otherwise; with density
+    weights on both sides."""
+    rng = np.random.default_rng(C * 100 + nsp)
+    N = 40
+    coord = _radial2d(nsp, ns)
+    w = rng.uniform(0.5, 2.0, coord.shape[:-1]).astype(np.float32)
+    P = R.Plan(coord, N, width=4, oversamp=2.0)
+    x = demo.complex_gaussian(rng, (C, N, N))
+    plan = pf.plan_nufft(torch.from_numpy(coord), N, width=4, oversamp=2.0)
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, weight=w, device=cuda_device)
+    yr = R.nufft(x[:, None], P, weight=w)[:, 0].reshape((C,) + coord.shape[:-1])
+    assert rel(y.cpu().numpy(), yr) < PARITY
+    k = demo.complex_gaussian(rng, (C,) + coord.shape[:-1])
+    xa = pf.nufft_adj(torch.from_numpy(k).to(cuda_device), nufft_plan=plan, weight=w, device=cuda_device)
+    xr = R.nufft_adj(k.reshape(C, 1, -1), P, weight=w)[:, 0]
+    assert rel(xa.cpu().numpy(), xr) < PARITY
