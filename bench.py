@@ -73,10 +73,10 @@ def stage_bytes(name, C, N3, G3, M, d):
         "ifft_crop": C * G3 * 8 + C * N3 * 8,
         "deapod_pad": C * N3 * 8 + C * G3 * 8,
         "fft_fwd": 2 * C * G3 * 8,
-        "interp": C * G3 * 8 + C * M * 8 + M * 4 * d,
+        "interp": C * G3 * 8 + C * M * 8 + M * 4 * (d + 1),
         "unsort": 2 * C * M * 8 + M * 8,
         "sort": 2 * C * M * 8 + M * 8,
-        "spread": C * M * 8 + M * 4 * d + C * G3 * 8,
+        "spread": C * M * 8 + M * 4 * (d + 1) + C * G3 * 8,
         "fft_adj": 2 * C * G3 * 8,
         "crop_deapod": 2 * C * N3 * 8,
     }

@@ -1,0 +1,29 @@
+"""bench.py's contract on the host: one GPU only, and the byte model its
+roofline figure is priced on."""
+
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout.strip() == ""
+
+
+def test_headline_algorithmic_bytes():
+    # 3-D 128^3 x 32 coils, sigma 2 (256^3 grid), 8192 x 256 samples: grid
+    # + samples + 4 (d + 1) bytes per sample for the gather and the spread
+    C, N, G, M, d = 32, 128, 256, 8192 * 256, 3
+    grid, samples, coords = C * G**3 * 8, C * M * 8, M * 4 * (d + 1)
+    assert bench.stage_bytes("interp", C, N**3, G**3, M, d) == grid + samples + coords == 4_865_392_640
+    assert bench.stage_bytes("spread", C, N**3, G**3, M, d) == 4_865_392_640
+    assert bench.stage_bytes("unsort", C, N**3, G**3, M, d) == 2 * samples + M * 8
