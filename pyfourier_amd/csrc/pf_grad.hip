@@ -163,8 +163,11 @@ __global__ void __launch_bounds__(TB) k_gather_dcoord(GradParams P, const float*
 // values each) first compute the sample's 3 x W derivative weights together
 // into LDS, then every lane walks the taps with the y/x loops unrolled (all
 // loads of a z row in flight) and 32-bit in-slab offsets.
-template <int CPL, int WY, int WX>
-__global__ void __launch_bounds__(TB) k_gather_dcoord_w(GradParams P, const float* __restrict__ taprec,
+// DEEP (3-D plans): the compiler may plan for 2 waves per SIMD and keep more
+// tap loads in flight per wave, as in the forward gather's 3-D instances
+template <int CPL, int WY, int WX, bool DEEP = false>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, DEEP ? 2 : 8)))
+k_gather_dcoord_w(GradParams P, const float* __restrict__ taprec,
                                                         const float* __restrict__ cos_sorted,
                                                         const int32_t* __restrict__ perm,
                                                         const float2* __restrict__ grid,
@@ -314,11 +317,11 @@ __global__ void __launch_bounds__(TB) k_gather_dcoord_w(GradParams P, const floa
 
 using GradLaunch = void (*)(dim3, hipStream_t, const GradParams&, const pf_plan*, const float2*, const float2*, float*);
 
-template <int CPL, int WY, int WX>
+template <int CPL, int WY, int WX, bool DEEP = false>
 void launch_dcoord_w(dim3 grid, hipStream_t s, const GradParams& P, const pf_plan* p, const float2* g,
                      const float2* h, float* out) {
-    hipLaunchKernelGGL((k_gather_dcoord_w<CPL, WY, WX>), grid, dim3(TB), 0, s, P, p->taprec, p->cos_sorted, p->perm,
-                       g, h, out);
+    hipLaunchKernelGGL((k_gather_dcoord_w<CPL, WY, WX, DEEP>), grid, dim3(TB), 0, s, P, p->taprec, p->cos_sorted,
+                       p->perm, g, h, out);
 }
 
 // [cpl - 1][wy == 1 ? 0 : 1][wx - 1]
@@ -331,6 +334,16 @@ const GradLaunch DCOORD_W[2][2][PF_WMAX] = {
       &launch_dcoord_w<2, 1, 5>, &launch_dcoord_w<2, 1, 6>, &launch_dcoord_w<2, 1, 7>, &launch_dcoord_w<2, 1, 8>},
      {nullptr, &launch_dcoord_w<2, 2, 2>, &launch_dcoord_w<2, 3, 3>, &launch_dcoord_w<2, 4, 4>,
       &launch_dcoord_w<2, 5, 5>, &launch_dcoord_w<2, 6, 6>, &launch_dcoord_w<2, 7, 7>, &launch_dcoord_w<2, 8, 8>}},
+};
+
+// square y/x widths of 3-D plans: [cpl - 1][w - 1]
+const GradLaunch DCOORD_W_DEEP[2][PF_WMAX] = {
+    {nullptr, &launch_dcoord_w<1, 2, 2, true>, &launch_dcoord_w<1, 3, 3, true>, &launch_dcoord_w<1, 4, 4, true>,
+     &launch_dcoord_w<1, 5, 5, true>, &launch_dcoord_w<1, 6, 6, true>, &launch_dcoord_w<1, 7, 7, true>,
+     &launch_dcoord_w<1, 8, 8, true>},
+    {nullptr, &launch_dcoord_w<2, 2, 2, true>, &launch_dcoord_w<2, 3, 3, true>, &launch_dcoord_w<2, 4, 4, true>,
+     &launch_dcoord_w<2, 5, 5, true>, &launch_dcoord_w<2, 6, 6, true>, &launch_dcoord_w<2, 7, 7, true>,
+     &launch_dcoord_w<2, 8, 8, true>},
 };
 
 }  // namespace
@@ -365,7 +378,8 @@ int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* g
     const int wy = p->width[1], wx = p->width[2];
     const bool idx32 = (int64_t)pf_gtot(p) * b.R * b.Cv < (1ll << 31);
     if (idx32 && (wy == 1 || wy == wx)) {
-        if (GradLaunch fn = DCOORD_W[cpl - 1][wy == 1 ? 0 : 1][wx - 1]) {
+        const bool deep = p->width[0] > 1 && wy == wx && wy > 1;
+        if (GradLaunch fn = deep ? DCOORD_W_DEEP[cpl - 1][wx - 1] : DCOORD_W[cpl - 1][wy == 1 ? 0 : 1][wx - 1]) {
             const int spb = TB / (b.C / cpl);
             fn(dim3((unsigned)((p->mtot + spb - 1) / spb)), s, P, p, grid, h, out);
             PF_HIP(hipGetLastError());
