@@ -27,3 +27,15 @@ def test_headline_algorithmic_bytes():
     assert bench.stage_bytes("interp", C, N**3, G**3, M, d) == grid + samples + coords == 4_865_392_640
     assert bench.stage_bytes("spread", C, N**3, G**3, M, d) == 4_865_392_640
     assert bench.stage_bytes("unsort", C, N**3, G**3, M, d) == 2 * samples + M * 8
+
+
+def test_headline_traffic_found_for_the_dominant_kernels():
+    # the committed rocprof counter summary names the kernels bench.py reports
+    import argparse
+
+    args = argparse.Namespace(width=6, small=False)
+    for stage in ("interp", "spread"):
+        t = bench.measured_traffic(stage, args)
+        assert t is not None, stage
+        # at least the compulsory bytes (grid once, samples once) moved
+        assert t >= 32 * 256**3 * 8 + 32 * 8192 * 256 * 8
