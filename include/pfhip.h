@@ -74,9 +74,9 @@ typedef struct pf_plan_info {
     int nframes;
     long long npts;
     int tile[3];
-    long long spread_entries;
-    int spread_items;
-    int spread_slots;
+    long long spread_entries; /* (sample, tile) pairs of the spread list */
+    int spread_items;         /* work items of the finer split (batches with few value chunks) */
+    int spread_slots;         /* partial tiles that split uses */
 } pf_plan_info;
 
 /* Message of the last failed call on this thread ("" if none). */
