@@ -28,11 +28,11 @@ def _case(ndim):
 
 @pytest.mark.parametrize("ndim", [2, 3])
 @pytest.mark.parametrize("chunk", [0, 16])
-@pytest.mark.parametrize("C", [1, 8, 12, 20, 64])
+@pytest.mark.parametrize("C", [1, 8, 12, 16, 20, 64])
 def test_all_spread_tiles_parity(cuda_device, ndim, chunk, C):
-    # the register spread runs with 8, 16 or 32 values per wave (1, 8 and 12
-    # take the small-batch instances, or 16 where a tile is too narrow for 8
-    # x parts); C = 64 (whole 32-coil chunks, two per tile) takes its
+    # the register spread runs with 8, 16 or 32 values per wave (1, 8, 12
+    # and 16 take the small-batch instances); batches of whole 8-, 16- or
+    # 32-value chunks (8, 16, 64: two chunks per tile) take the
     # software-pipelined form
     rng = np.random.default_rng(ndim)
     coord, shape = _case(ndim)
