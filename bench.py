@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the NUDFT error check")
     ap.add_argument("--cpu-coils", type=int, default=32, help="coils in the CPU baseline sample")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU runs (median) after one warm-up")
     ap.add_argument("--small", action="store_true", help="64^3 x 8 coils quick run")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched / autograd comparisons")
     ap.add_argument("--only", default=None, help="run one comparison of the extras by name and print its JSON")
@@ -83,9 +84,11 @@ def stage_bytes(name, C, N3, G3, M, d):
     return b[name]
 
 
-def cpu_baseline(coord, xc, w, N, width, ncoils_sample, C_full):
+def cpu_baseline(coord, xc, w, N, width, ncoils_sample, C_full, runs=5):
     """Time the reference algorithm on the host: C/OpenMP restatement of the
-    Numba loops + torch CPU apod/pad/FFT, one fwd+adj on a coil sample."""
+    Numba loops + torch CPU apod/pad/FFT.  One warm-up fwd+adj, then the
+    median of `runs` fwd+adj on the coil sample.  Also returns the last run's
+    forward output and the adjoint of that output, for the parity check."""
     import torch
 
     from oracle import port as Pt
@@ -96,15 +99,18 @@ def cpu_baseline(coord, xc, w, N, width, ncoils_sample, C_full):
     P = R.Plan(coord, N, width=width, oversamp=2.0)
     pp = Pt.PortPlan(P)
     wf = w.reshape(1, 1, -1)
-    # warm-up on one coil (thread pools, page-in)
-    y = Pt.nufft(xc[:1, None], pp) * wf
-    Pt.nufft_adj(y * wf, pp)
     xs = xc[:ncoils_sample, None]
-    t0 = time.perf_counter()
-    y = Pt.nufft(xs, pp) * wf
-    Pt.nufft_adj(y * wf, pp)
-    dt = time.perf_counter() - t0
-    per_full = dt * C_full / ncoils_sample
+    times = []
+    for r in range(runs + 1):
+        t0 = time.perf_counter()
+        y = Pt.nufft(xs, pp) * wf
+        xa = Pt.nufft_adj(y * wf, pp)
+        dt = time.perf_counter() - t0
+        log(f"cpu baseline run {r} ({'warm-up' if r == 0 else 'timed'}): {dt:.2f} s")
+        if r > 0:
+            times.append(dt)
+    med = float(np.median(times))
+    per_full = med * C_full / ncoils_sample
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -114,16 +120,62 @@ def cpu_baseline(coord, xc, w, N, width, ncoils_sample, C_full):
                     break
     except OSError:
         pass
-    return {
+    info = {
         "value": 1.0 / per_full,
         "unit": "fwd+adj/s",
         "cores": ncores,
         "kind": "port",
-        "sample": f"{ncoils_sample} of {C_full} coils, full trajectory, one fwd+adj after a 1-coil warm-up "
-                  f"({dt:.1f} s), scaled x{C_full / ncoils_sample:g}; C/OpenMP restatement of the Numba "
-                  f"loops + torch CPU FFT, no gc.collect overheads; CPU: {model}",
+        "sample": f"{ncoils_sample} of {C_full} coils, full trajectory; median of {runs} fwd+adj after a "
+                  f"full-size warm-up ({', '.join(f'{t:.2f}' for t in times)} s)"
+                  + (f", scaled x{C_full / ncoils_sample:g}" if ncoils_sample != C_full else "")
+                  + f"; C/OpenMP restatement of the Numba loops + torch CPU FFT, no gc.collect overheads; "
+                  f"CPU: {model}",
         "torch_threads": torch.get_num_threads(),
     }
+    return info, y, xa
+
+
+def parity_vs_port(pf, plan, y_port, x_port, xc_dev, w_dev, dev):
+    """Relative L2 of the GPU forward (on the same coil images) and of the GPU
+    adjoint (on the port's own forward output, so both adjoints see the same
+    input) against the CPU port's full-size outputs."""
+    import torch
+
+    C = y_port.shape[0]
+    y = pf.nufft(xc_dev[:C], nufft_plan=plan, weight=w_dev, device=dev)
+    yp = torch.from_numpy(np.ascontiguousarray(y_port.reshape(y.shape)).astype(np.complex64)).to(dev)
+    e_fwd = float(torch.linalg.vector_norm(y - yp) / torch.linalg.vector_norm(yp))
+    xa = pf.nufft_adj(yp, nufft_plan=plan, weight=w_dev, device=dev)
+    xp = torch.from_numpy(np.ascontiguousarray(x_port.reshape(xa.shape)).astype(np.complex64)).to(dev)
+    e_adj = float(torch.linalg.vector_norm(xa - xp) / torch.linalg.vector_norm(xp))
+    del y, yp, xa, xp
+    return {"fwd_rel_l2": float(f"{e_fwd:.3e}"), "adj_rel_l2": float(f"{e_adj:.3e}"), "coils": int(C),
+            "checker": "oracle/port.py (C/OpenMP + torch CPU FFT, complex64), full size"}
+
+
+def adjoint_accuracy(pf, plan, y_dev, w_dev, coord_grid, N, dev, nlines=8, chunk=1 << 16):
+    """Relative L2 error of the adjoint (coil 0, weighted input) against the
+    FP64 adjoint NUDFT x_r = sum_j w_j y_j exp(+2 pi i k_j . r / N) scaled by
+    1/Ghat (norm=None), on `nlines` random x-lines of the image."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    zs = rng.integers(0, N, nlines)
+    ys = rng.integers(0, N, nlines)
+    xa = pf.nufft_adj(y_dev[:1], nufft_plan=plan, weight=w_dev, device=dev)[0]
+    got = xa[torch.from_numpy(zs).to(dev), torch.from_numpy(ys).to(dev)].cpu().numpy()  # (nlines, N)
+    yw = (y_dev[0].reshape(-1) * w_dev.reshape(-1)).cpu().numpy().astype(np.complex128)
+    c = coord_grid.astype(np.float64)
+    r = np.arange(N) - N // 2
+    ref = np.zeros((nlines, N), dtype=np.complex128)
+    for s in range(0, c.shape[0], chunk):
+        cs = c[s:s + chunk]
+        ex = np.exp(2j * math.pi * np.outer(cs[:, 0], r) / N)  # (m, N) along x
+        v = yw[s:s + chunk][None, :] * np.exp(2j * math.pi * (np.outer(zs - N // 2, cs[:, 2])
+                                                                + np.outer(ys - N // 2, cs[:, 1])) / N)
+        ref += v @ ex
+    ref /= float(np.prod(plan.os_shape))
+    return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
 
 def accuracy(pf, plan, x, xc_dev, y_dev, coord_grid, N, dev, nsub=1024):
@@ -338,6 +390,69 @@ def bench_autograd_3d(pf, dev, steps):
             "plain_ms": round(t_plain, 3), "autograd_ms": round(t_auto, 3), "speedup": round(t_plain / t_auto, 2)}
 
 
+def bench_cfg1_latency(pf, dev, steps):
+    """2-D plain NUFFT, the reference's CPU-runnable case: 128^2 complex64
+    image (seed 0), golden-angle radial 64 spokes x 256 samples in radians,
+    one coil, oversampling 2, width 4.  Latency of one fwd+adj (drop-in
+    calls, wall clock), the same pair replayed from a captured HIP graph,
+    and parity of both directions against the float64 restatement plus the
+    forward's error against the FP64 NUDFT, at this size."""
+    import torch
+
+    from oracle import nudft as U
+    from oracle import restate as R
+    from pyfourier_amd import demo
+
+    rng = np.random.default_rng(0)
+    N = 128
+    coord = demo.golden_angle_radial_2d(64, 256)
+    x = demo.complex_gaussian(rng, (N, N))
+    plan = pf.plan_nufft(torch.from_numpy(coord), N, width=4, oversamp=2.0).to(dev)
+    xd = torch.from_numpy(x).to(dev)
+    yk = torch.from_numpy(demo.complex_gaussian(rng, coord.shape[:-1])).to(dev)
+
+    def pair():
+        pf.nufft(xd, nufft_plan=plan, device=dev)
+        pf.nufft_adj(yk, nufft_plan=plan, device=dev)
+
+    n = max(50, steps * 20)
+    t_call = _time(pair, dev, n, warmup=5)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        y_g = pf.nufft(xd, nufft_plan=plan, device=dev)
+        x_g = pf.nufft_adj(yk, nufft_plan=plan, device=dev)
+    t_graph = _time(g.replay, dev, n, warmup=5)
+    g.replay()
+    torch.cuda.synchronize(dev)
+    P = R.Plan(coord, N, width=4, oversamp=2.0)
+    yr = R.nufft(x[None, None], P)[0, 0]
+    xr = R.nufft_adj(yk.cpu().numpy().reshape(1, 1, -1), P)[0, 0]
+    e_f = float(np.linalg.norm(y_g.cpu().numpy().ravel() - yr) / np.linalg.norm(yr))
+    e_a = float(np.linalg.norm(x_g.cpu().numpy() - xr) / np.linalg.norm(xr))
+    yt = U.nudft(x[None], P.coord.reshape(-1, 2), [N, N])[0]
+    e_n = float(np.linalg.norm(y_g.cpu().numpy().ravel() - yt) / np.linalg.norm(yt))
+    return {"workload": "2-D 128^2, golden-angle radial 64x256, 1 coil, sigma 2, W=4",
+            "fwd_adj_us": round(t_call * 1000.0, 1), "fwd_adj_graph_us": round(t_graph * 1000.0, 1),
+            "parity_vs_restatement": {"fwd_rel_l2": float(f"{e_f:.2e}"), "adj_rel_l2": float(f"{e_a:.2e}")},
+            "fwd_rel_err_vs_nudft": float(f"{e_n:.2e}")}
+
+
+def _gram_exact_rows(x0, coord_grid, w, N, rows, chunk=2048):
+    """FP64 A^H W A x on image rows (2-D, one coil): A the exact NUDFT."""
+    from oracle import nudft as U
+
+    y = U.nudft(x0[None], coord_grid, [N, N], chunk=chunk)[0] * w.reshape(-1)
+    c = coord_grid.astype(np.float64)
+    r = np.arange(N) - N // 2
+    out = np.zeros((len(rows), N), dtype=np.complex128)
+    for s in range(0, c.shape[0], 1 << 15):
+        cs = c[s:s + (1 << 15)]
+        ex = np.exp(2j * math.pi * np.outer(cs[:, 0], r) / N)
+        v = y[s:s + (1 << 15)][None, :] * np.exp(2j * math.pi * np.outer(np.asarray(rows) - N // 2, cs[:, 1]) / N)
+        out += v @ ex
+    return out
+
+
 def bench_toeplitz(pf, dev, steps):
     """2-D Toeplitz Gram: 48 variable-density spiral interleaves x 4096
     samples, 256^2 x 16 coils, |k| density weights, kernel on the 512^2
@@ -370,9 +485,20 @@ def bench_toeplitz(pf, dev, steps):
     diff = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
     t_g = _time(gram, dev, steps)
     t_e = _time(explicit, dev, steps)
+    # both against the FP64 normal operator (1/Ghat) A^H W A x (exact NUDFT,
+    # Ghat = ceil(1.25 N)^2 as both are planned) on 8 random rows of coil 0
+    rows = sorted(np.random.default_rng(13).choice(N, 8, replace=False).tolist())
+    cg = plan.interpolator.coord.reshape(-1, 2).numpy()
+    ref = _gram_exact_rows(x[0].cpu().numpy(), cg, w.cpu().numpy(), N, rows) / float(np.prod(plan.os_shape))
+    ridx = torch.tensor(rows, device=dev)
+    e_g = float(np.linalg.norm(a[0, ridx].cpu().numpy() - ref) / np.linalg.norm(ref))
+    e_e = float(np.linalg.norm(b[0, ridx].cpu().numpy() - ref) / np.linalg.norm(ref))
     return {"workload": "2-D Toeplitz Gram, 256^2 x 16 coils, 48 spirals x 4096, kernel on 512^2",
             "gram_ms": round(t_g, 3), "explicit_ms": round(t_e, 3), "speedup": round(t_e / t_g, 2),
             "rel_diff_vs_explicit": round(diff, 5),
+            "gram_rel_err_vs_fp64_AhWA": float(f"{e_g:.2e}"),
+            "explicit_rel_err_vs_fp64_AhWA": float(f"{e_e:.2e}"),
+            "fp64_check": "coil 0, 8 image rows, exact NUDFT normal operator",
             "multiply_bytes": 2 * C * (2 * N) ** 2 * 8 + (2 * N) ** 2 * 8}
 
 
@@ -589,7 +715,7 @@ def bench_autograd(pf, dev, steps):
             "plain_ms": round(t_plain, 3), "autograd_ms": round(t_auto, 3), "speedup": round(t_plain / t_auto, 2)}
 
 
-EXTRAS = (("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
+EXTRAS = (("cfg1_latency", bench_cfg1_latency), ("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
           ("autograd_b0", bench_autograd), ("autograd_3d", bench_autograd_3d), ("toeplitz", bench_toeplitz),
           ("cartesian", bench_cartesian),
           ("grog_sparse", bench_grog_sparse))
@@ -655,10 +781,19 @@ def main():
 
     if acc is not None:
         log(f"accuracy vs NUDFT: {acc}")
+    if acc is not None:
+        acc["adj_rel_err_vs_nudft"] = adjoint_accuracy(pf, plan, y_nw, w_dev, coord_grid, N, dev)
+        acc["adj_lines_checked"] = 8
+        log(f"adjoint accuracy vs NUDFT: {acc['adj_rel_err_vs_nudft']:.3e}")
     cpu = None
+    parity = None
     if not args.no_cpu:
-        cpu = cpu_baseline(coord, xc, w, N, args.width, min(args.cpu_coils, C), C)
+        cpu, y_port, x_port = cpu_baseline(coord, xc, w, N, args.width, min(args.cpu_coils, C), C,
+                                           runs=args.cpu_runs)
         log(f"cpu baseline: {cpu['value']:.4f} fwd+adj/s on {cpu['cores']} cores")
+        parity = parity_vs_port(pf, plan, y_port, x_port, xc_dev, w_dev, dev)
+        del y_port, x_port
+        log(f"parity vs port: {parity}")
 
     copy_gbs = stream_copy(dev)
     log(f"device copy: {copy_gbs} GB/s")
@@ -700,6 +835,7 @@ def main():
         "stages": stage_info,
         "plan_s": round(plan_s, 3),
         "accuracy": acc,
+        "parity_vs_port": parity,
         "compare_width": cmp,
         "vs_plain": extras,
         "cpu_baseline": cpu,

@@ -10,8 +10,13 @@
  *     unless marked "host".  complex data are interleaved complex64
  *     (float re, float im), passed as void*.
  *   - Every call only enqueues work on `stream` (a hipStream_t passed as
- *     void*, NULL = default stream) and returns without synchronising, except
- *     pf_plan_create which synchronises once when the plan is built.
+ *     void*, NULL = default stream) and returns without synchronising or
+ *     allocating, except the pf_*_create calls, which allocate and
+ *     synchronise once when the object is built (fused-FFT twiddle tables
+ *     included), so operator calls may be captured into a HIP graph from the
+ *     first one.  One exception: grid sizes with prime factors other than
+ *     2, 3 and 5 run rocFFT, whose plan and work buffer for a new batch size
+ *     are made (under the plan's lock) at the first call with that size.
  *   - Every call returns 0 on success or a PF_ERR_* code; pf_last_error()
  *     returns a thread-local message for the last failure.  Arguments are
  *     validated on the host before any launch.
@@ -163,6 +168,14 @@ int pf_nufft_adj(pf_plan* plan, const pf_batch_desc* desc, const void* kspace, v
                  const float* weight, int norm, void* workspace, size_t workspace_bytes,
                  void* stream);
 
+/* Names of the kernel instances pf_nufft / pf_nufft_adj run for `desc`, as
+ * host text "gather=<instance> spread=<instance>[+k_spread_reduce]
+ * fft=fused|rocfft" (NUL-terminated, at most cap bytes).  No reference
+ * counterpart (the reference's dispatch tables are Python dicts,
+ * src/pyfourier/_subroutines/_grid/_cart2noncart/__init__.py:17-18 and
+ * _noncart2cart/__init__.py:17-18); lets tests and profiles name what ran. */
+int pf_describe_launch(const pf_plan* plan, const pf_batch_desc* desc, char* buf, size_t cap);
+
 /* ---- stage entry points (the reference's kernel plug-in points) ------- */
 
 /* Grid bytes for one call: nfg*C*R*prod(os_shape)*8. */
@@ -241,12 +254,15 @@ int pf_interp_grad(const pf_plan* plan, const pf_batch_desc* desc, const void* g
 
 typedef struct pf_toep pf_toep;
 
+/* Largest per-voxel kernel block R = nseg * ncoeff a Gram operator accepts. */
+#define PF_TOEP_RMAX 128
+
 /* Build a Gram operator from its kernel.  Replaces the GramMatrix of
  * plan_spgram, src/pyfourier/_sparsegram/_plan.py:20-270.
  * kernel: (prod(2N), R, R) complex64 device array, R = nseg * ncoeff, the
  *         uncentred FFT of the wrapped point-spread functions on the 2N grid
  *         (index order of an uncentred FFT), already scaled as the operator
- *         needs; copied into the object.
+ *         needs; copied into the object.  R <= PF_TOEP_RMAX.
  * shape:  host (ndim,) image matrix N (z, y, x).  Embedding sizes 2^a 3^b 5^c
  *         (<= 4096 per axis) run the pruned fused FFT passes, others rocFFT. */
 int pf_toeplitz_create(const void* kernel, int ndim, const int* shape, int nseg, int ncoeff, void* stream,

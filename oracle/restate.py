@@ -159,23 +159,44 @@ def kernel_scale(beta, width):
     return total
 
 
-def normalize_coordinates(coord, shape_xyz):
+def normalize_coordinates(coord, shape_xyz, units="auto"):
     """Map coordinates to grid units [-N/2, N/2).
 
-    Intent of _subroutines/_traj.py:10-54 with defect A.11 fixed (the
-    reference divides each point by its own radius).  The unit is detected
-    once from the largest radius r: r <= 0.5 -> cycles/FOV (times N),
-    r <= pi -> radians (times N/(2 pi)), otherwise grid units (unchanged).
+    Intent of _subroutines/_traj.py:10-54 ("input can have any normalization,
+    e.g. 1/m, rad/m, -0.5:0.5, -pi:pi") with defect A.11 fixed (the reference
+    divides each point by its own radius, :37-38,47-48).  One factor for the
+    whole trajectory, from the largest |k_a| over samples and axes, m, and the
+    matrix N: m <= 0.5 -> cycles/FOV (times N), m <= pi -> radians (times
+    N/(2 pi)), |k_a| <= N_a/2 on every axis -> grid units (unchanged),
+    otherwise any other unit with m as the k-space edge (times N/(2 m)).
+    `units` forces one of "cycles", "radians", "grid", "max".
     """
     coord = np.asarray(coord, dtype=np.float32)
+    if not np.isfinite(coord).all():
+        raise ValueError("coordinates must be finite")
     shape_xyz = np.asarray(shape_xyz, dtype=np.float32)
-    r = float(np.sqrt((coord.astype(np.float64) ** 2).sum(axis=-1)).max()) if coord.size else 0.0
-    if r <= 0.5 + 1e-5:
+    d = coord.shape[-1]
+    amax = np.abs(coord.reshape(-1, d)).max(axis=0).astype(np.float64) if coord.size else np.zeros(d)
+    m = float(amax.max()) if d else 0.0
+    if units == "auto":
+        if m <= 0.5 + 1e-5:
+            units = "cycles"
+        elif m <= math.pi + 1e-5:
+            units = "radians"
+        elif np.all(amax <= 0.5 * shape_xyz.astype(np.float64) * (1 + 1e-6) + 1e-5):
+            units = "grid"
+        else:
+            units = "max"
+    if units == "cycles":
         f = 1.0
-    elif r <= math.pi + 1e-5:
+    elif units == "radians":
         f = 1.0 / (2.0 * math.pi)
-    else:
+    elif units == "max" and m > 0.0:
+        f = 0.5 / m
+    elif units in ("grid", "max"):
         return coord.copy()
+    else:
+        raise ValueError(f"unknown units {units!r}")
     return coord * (shape_xyz * np.float32(f))
 
 
@@ -576,6 +597,60 @@ def toeplitz_apply(x, T, shape):
     xp = resize(np.asarray(x, dtype=np.complex128), big)
     yp = fftc(T * fftc(xp, d, centered=False), d, inverse=True, centered=False)
     return resize(yp, list(shape))
+
+
+def toeplitz_kernel_block_exact(coord_grid, shape, coefs, weight=None, ghat=1.0):
+    """FP64 exact per-voxel R x R Toeplitz kernel of the subspace / field-map
+    Gram, (R, R, 2N...):
+
+        T[r, r'] = FFT_u(wrap(PSF_rr')) / ghat,
+        PSF_rr'(dr) = sum_j w_j conj(c_r(j)) c_r'(j) exp(+2 pi i k_j . dr / N)
+
+    coefs (M, R): each sample's coefficient c_r(j).  Subspace: c_k(j) =
+    basis[frame(j), k] (the basis-weighted delta of
+    src/pyfourier/_sparsegram/_plan.py:148-156, adjoint-transformed with the
+    basis so T[k, k'] = sum_f conj(B[f, k]) B[f, k'] PSF_f); field map: c_l(j)
+    = Bt[l, t(j)] (:175-183); both: c_(l, k) = Bt[l, t] basis[f, k], r = l*K + k.
+    The kernel layout of :228-239 (one R x R matrix per voxel) with the exact
+    A^H W A scale (defect A.16: the reference divides by mean|T|).
+    """
+    from .nudft import nudft_adj
+
+    coord = np.asarray(coord_grid, dtype=np.float64).reshape(-1, len(shape))
+    c = np.asarray(coefs, dtype=np.complex128).reshape(coord.shape[0], -1)
+    R_ = c.shape[1]
+    w = np.ones(coord.shape[0]) if weight is None else np.asarray(weight, dtype=np.float64).reshape(-1)
+    y = w[None, None, :] * np.conj(c.T)[:, None, :] * c.T[None, :, :]  # (R, R, M)
+    big = [2 * n for n in shape]
+    psf = nudft_adj(y.reshape(R_ * R_, -1), 2.0 * coord, big).reshape((R_, R_) + tuple(big))
+    d = len(shape)
+    axes = tuple(range(-d, 0))
+    return np.fft.fftn(np.fft.ifftshift(psf, axes=axes), axes=axes) / ghat
+
+
+def toeplitz_apply_block(x, T, shape, b0_spatial=None):
+    """Subspace / field-map Toeplitz Gram of one batch item.
+
+    x (K, *N) -> (K, *N).  T (R, R, 2N...) from toeplitz_kernel_block_exact,
+    R = L*K; b0_spatial (L, *N) spatial maps Ct_l or None (L = 1).
+      u_(l,k) = FFT_u(pad_2N(Ct_l * x_k))                 (_sparsegram.py:26-53, 59-73)
+      v_r     = sum_r' T[r, r'] u_r'   per voxel          (_bmatvec/_bdot.py:130-157)
+      out_k   = sum_l conj(Ct_l) * crop_N(IFFT_u(v_(l,k))) (_sparsegram.py:44-51, 75-81)
+    The per-voxel product is the matrix-vector product the Gram needs; the
+    reference's _dot_nb (_bdot.py:119-127) indexes the vector by the output
+    index j where the product needs the summation index i.
+    """
+    x = np.asarray(x, dtype=np.complex128)
+    d = len(shape)
+    K = x.shape[0]
+    big = [2 * n for n in shape]
+    maps = np.ones((1,) + tuple(shape)) if b0_spatial is None else np.asarray(b0_spatial, dtype=np.complex128)
+    L = maps.shape[0]
+    u = np.stack([fftc(resize(maps[l] * x[k], big), d, centered=False) for l in range(L) for k in range(K)])
+    v = np.einsum("rs...,s...->r...", np.asarray(T), u)
+    yv = np.stack([resize(fftc(v[r], d, inverse=True, centered=False), list(shape)) for r in range(L * K)])
+    yv = yv.reshape((L, K) + tuple(shape))
+    return np.einsum("l...,lk...->k...", np.conj(maps), yv)
 
 
 # --------------------------------------------------------------------------

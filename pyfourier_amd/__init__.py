@@ -5,7 +5,7 @@ INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
 
     plan_nufft, nufft, nufft_adj      forward / adjoint NUFFT (+ subspace, B0)
     plan_spgram, sparse_gram          Toeplitz-embedded Gram operator
-    pipe_menon_dcf                    iterative density compensation
+    pipemenon.pipe_menon_dcf          iterative density compensation (also top level)
     plan_fft, fftn, ifftn             masked Cartesian FFT (+ subspace, B0)
     plan_gram, gram                   dense Cartesian Gram operator
     fft, ifft                         centred FFT utilities
@@ -27,7 +27,8 @@ from ._traj import normalize_coordinates
 from ._b0 import mri_exp_approx
 from ._sparsegram import GramMatrix, plan_spgram, sparse_gram
 from ._batch import NUFFTBatch, plan_nufft_batch
-from ._dcomp import pipe_menon_dcf
+from . import pipemenon
+from .pipemenon import pipe_menon_dcf
 from ._tune import autotune_plan, spread_tiles
 from ._fft import FFTPlan, fft, fftn, ifft, ifftn, plan_fft
 from ._gram import gram, plan_gram
@@ -48,6 +49,7 @@ __all__ = [
     "NUFFTBatch",
     "plan_nufft_batch",
     "pipe_menon_dcf",
+    "pipemenon",
     "autotune_plan",
     "spread_tiles",
     "FFTPlan",

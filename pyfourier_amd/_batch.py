@@ -82,7 +82,7 @@ class NUFFTBatch:
     """
 
     def __init__(self, coords, shapes, nbatch=1, width=4, oversamp=1.25, weights=None, basis=None, norm=None,
-                 device="cuda", graph=True, group=True, tile=None, chunk=0):
+                 device="cuda", graph=True, group=True, tile=None, chunk=0, units="auto"):
         if len(coords) != len(shapes):
             raise ValueError("coords and shapes must have the same length")
         if len(coords) == 0:
@@ -90,7 +90,8 @@ class NUFFTBatch:
         self.device = resolve_device(device)
         self.norm_code = _norm_code(norm)
         self.use_graph = bool(graph)
-        self.plans = [plan_nufft(_to_torch(c).detach().to(torch.float32), sh, width, oversamp, tile=tile, chunk=chunk)
+        self.plans = [plan_nufft(_to_torch(c).detach().to(torch.float32), sh, width, oversamp, tile=tile, chunk=chunk,
+                                 units=units)
                       for c, sh in zip(coords, shapes)]
         self.images = [None] * len(coords)
         self.kspace = [None] * len(coords)
@@ -192,7 +193,7 @@ class NUFFTBatch:
 
 
 def plan_nufft_batch(coords, shapes, nbatch=1, width=4, oversamp=1.25, weights=None, basis=None, norm=None,
-                     device="cuda", graph=True, group=True, tile=None, chunk=0) -> NUFFTBatch:
+                     device="cuda", graph=True, group=True, tile=None, chunk=0, units="auto") -> NUFFTBatch:
     """Plan an uneven batch of NUFFT problems (see NUFFTBatch)."""
     return NUFFTBatch(coords, shapes, nbatch, width, oversamp, weights, basis, norm, device, graph, group, tile,
-                      chunk)
+                      chunk, units)

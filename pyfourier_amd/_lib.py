@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libpfhip.so")
 PF_NORM_NONE = 0
 PF_NORM_ORTHO = 1
 PF_NORM_ADJOINT = 2
+PF_TOEP_RMAX = 128  # include/pfhip.h: largest Gram kernel block nseg * ncoeff
 
 _ERRORS = {1: "invalid argument", 2: "HIP error", 3: "rocFFT error", 4: "allocation failed", 5: "unsupported"}
 
@@ -82,6 +83,7 @@ _SIG = {
     "pf_workspace_size": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, ctypes.POINTER(ctypes.c_size_t)]),
     "pf_nufft": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _I, _VP, ctypes.c_size_t, _VP]),
     "pf_nufft_adj": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _I, _VP, ctypes.c_size_t, _VP]),
+    "pf_describe_launch": (_I, [_VP, ctypes.POINTER(BatchDesc), ctypes.c_char_p, ctypes.c_size_t]),
     "pf_grid_size": (_I, [_VP, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_size_t)]),
     "pf_deapod_pad": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP]),
     "pf_fft": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _I, _VP]),

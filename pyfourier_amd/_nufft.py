@@ -139,6 +139,14 @@ class _Call:
         d.nt = int(bt.shape[1]) if bt is not None else 1
         return d
 
+    def describe(self, C: int, seg=(None, None)) -> str:
+        """Names of the gather / spread kernel instances a batch of C runs
+        (pf_describe_launch), e.g. for tests and profiles."""
+        buf = ctypes.create_string_buffer(512)
+        desc = self.desc(C, seg)
+        _lib.check(_lib.lib().pf_describe_launch(self.native.handle, ctypes.byref(desc), buf, 512))
+        return buf.value.decode()
+
     # ---- native operator calls on (C, nimg, *N) / (C, F, npts) ------------
     def forward(self, x: torch.Tensor, norm_code: int, out: torch.Tensor | None = None) -> torch.Tensor:
         C = x.shape[0]

@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 from ._b0 import mri_exp_approx
-from ._traj import normalize_coordinates, unit_factors
+from ._traj import check_finite, normalize_coordinates, unit_factors
 
 
 def resolve_device(device) -> torch.device:
@@ -224,19 +224,22 @@ def plan_nufft(
     L_batch_size=None,
     tile=None,
     chunk=0,
-    grid_units=False,
+    units="auto",
 ):
     """Precompute a NUFFT plan (same signature as the reference plan_nufft).
 
     coord: (ncontrasts, nviews, nsamples, ndim) in any unit, last axis (x, y[, z]);
     shape: image matrix (scalar or (z, y, x)); width/oversamp: scalar or per axis (z, y, x).
-    grid_units (extension): skip the unit detection, coord is in grid units.
     tile, chunk (extensions): the sort/spread tile (z, y, x) and the spread
     work-item size, e.g. from pyfourier_amd.autotune_plan; None/0 = defaults.
+    units (extension): "auto" (detect), "grid", "radians", "cycles" or "max"
+    (any other unit, largest |k| = k-space edge); see normalize_coordinates.
+    Non-finite coordinates raise ValueError.
     """
     coord = torch.as_tensor(coord)
     if torch.is_complex(coord):
         raise TypeError("coordinates must be real")
+    check_finite(coord)
     ndim = int(coord.shape[-1])
     if ndim not in (1, 2, 3):
         raise ValueError("coordinates must have 1, 2 or 3 spatial dims")
@@ -245,13 +248,8 @@ def plan_nufft(
     width = _per_axis(width, ndim, np.int16)
     oversamp = _per_axis(oversamp, ndim, np.float32)
     shape = _per_axis(shape, ndim, np.int64)
-    if grid_units:
-        # caller guarantees grid units (e.g. the sparse FFT's integer indexes)
-        units = (1.0,) * ndim
-        coord = coord.to(torch.float32)
-    else:
-        units = unit_factors(coord, shape[::-1].tolist())
-        coord = normalize_coordinates(coord, shape[::-1].tolist())
+    unit_scale = unit_factors(coord, shape[::-1].tolist(), units)
+    coord = normalize_coordinates(coord, shape[::-1].tolist(), units=units)
     # Cartesian axes get oversamp 1 and width 1 (src/pyfourier/_nufft/_plan.py:139-146)
     is_cart = np.asarray(
         [bool(torch.allclose(coord[..., a], torch.round(coord[..., a]))) for a in range(ndim)][::-1]
@@ -271,7 +269,7 @@ def plan_nufft(
         ndim=ndim,
         dshape=tuple(int(s) for s in coord.shape[1:-1]),
         nframes=int(coord.shape[0]),
-        unit_scale=units,
+        unit_scale=unit_scale,
         tile=None if tile is None else tuple(int(t) for t in tile),
         chunk=int(chunk),
     )

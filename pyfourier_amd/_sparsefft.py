@@ -55,7 +55,7 @@ def plan_spfft(indexes, shape, zmap=None, L=6, nbins=(40, 40), dt=None, T=None, 
             "Input coordinates must lie on Cartesian grid, got non-uniform coord! Please use NUFFT instead."
         )
     plan = plan_nufft(torch.round(c).to(torch.float32), shp, width=1, oversamp=1.0, zmap=zmap, L=L, nbins=nbins,
-                      dt=dt, T=T, L_batch_size=L_batch_size, grid_units=True)
+                      dt=dt, T=T, L_batch_size=L_batch_size, units="grid")
     it = plan.interpolator
     cidx = (torch.round(c) + torch.floor(nxyz / 2)).to(torch.int64)
     cidx = cidx.reshape((it.nframes,) + tuple(it.dshape) + (ndim,))

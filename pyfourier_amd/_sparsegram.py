@@ -98,20 +98,30 @@ def plan_spgram(
     oversamp=None,
     width=6,
     device="cpu",
+    units="auto",
 ):
     """Compute the Toeplitz kernel of A^H W A (same signature as the reference).
 
     coord (ncontrasts, nviews, nsamples, ndim); shape: image matrix; basis (ncontrasts, ncoeff);
     weight broadcastable to (ncontrasts, nviews, nsamples); device must name a GPU.
+    units (extension): coordinate unit, as for plan_nufft.  The per-voxel block
+    R = (field-map segments) x (subspace coefficients) is at most PF_TOEP_RMAX.
     """
     dev = resolve_device(device)
+    kb = 1 if basis is None else int(_to_torch(basis).shape[-1])
+    rb = kb * (int(L) if zmap is not None else 1)
+    if rb > _lib.PF_TOEP_RMAX:
+        raise ValueError(
+            f"the Gram kernel block R = nseg * ncoeff = {rb} exceeds {_lib.PF_TOEP_RMAX}; "
+            "use fewer field-map segments (L) or subspace coefficients"
+        )
     coord = _to_torch(coord).to(torch.float32)
     d = int(coord.shape[-1])
     while coord.dim() < 4:
         coord = coord[None]
     F, nv, ns = (int(s) for s in coord.shape[:3])
     shp = [int(shape)] * d if np.isscalar(shape) else [int(s) for s in list(shape)[-d:]]
-    kg = normalize_coordinates(coord, shp[::-1]).to(dev)  # grid units
+    kg = normalize_coordinates(coord, shp[::-1], units=units).to(dev)  # grid units
     M = F * nv * ns
 
     # per-sample coefficients c_r(j), r = (l, k)
@@ -145,7 +155,7 @@ def plan_spgram(
 
     # PSFs with the adjoint NUFFT on the 2N grid (coordinates 2k, one frame)
     big = [2 * s for s in shp]
-    plan2 = plan_nufft(2.0 * kg.reshape(1, 1, M, d).cpu(), big, width=width, oversamp=2.0)
+    plan2 = plan_nufft(2.0 * kg.reshape(1, 1, M, d).cpu(), big, width=width, oversamp=2.0, units="grid")
     plan2.to(dev)
     y = w[None, :] * torch.conj(coefs.T)[:, None, :] * coefs.T[None, :, :]  # (R, R, M)
     call = _Call(plan2, dev, None, None, None)

@@ -81,6 +81,17 @@ WsLayout layout(const pf_plan* p, const pf_batch& b) {
     return w;
 }
 
+// FFT state a plan needs before its first operator call: the fused pruned
+// FFT's twiddle tables for grid sizes 2^a 3^b 5^c (built here, so calls
+// neither allocate nor synchronise), otherwise the rocFFT plan cache, whose
+// per-batch-size rocFFT plans are made on first use under its lock.
+int plan_fft_state(pf_plan* p) {
+    int rc = pf_ops_prepare();
+    if (rc != PF_OK) return rc;
+    if (pf_fftk_supported(p)) return pf_fftk_init(p);
+    return pf_fft_cache_init(p);
+}
+
 }  // namespace
 
 // Deapodisation vectors per axis (src/pyfourier/_subroutines/_apod.py:27-41):
@@ -144,6 +155,7 @@ int pf_plan_create_tiled(const float* coord, int ndim, const int* shape, const i
         return rc;
     }
     rc = pf_plan_build(p, coord, static_cast<hipStream_t>(stream));
+    if (rc == PF_OK) rc = plan_fft_state(p);
     if (rc != PF_OK) {
         pf_plan_free(p);
         return rc;
@@ -184,7 +196,8 @@ int pf_plan_create_ragged(const float* coord, int ndim, const int* shape, const 
         pf_plan_free(p);
         return pf_fail(PF_ERR_HIP, std::string("frame offsets: ") + hipGetErrorString(he));
     }
-    if ((rc = pf_plan_upload_apod(p)) != PF_OK || (rc = pf_plan_build(p, coord, static_cast<hipStream_t>(stream))) != PF_OK) {
+    if ((rc = pf_plan_upload_apod(p)) != PF_OK || (rc = pf_plan_build(p, coord, static_cast<hipStream_t>(stream))) != PF_OK ||
+        (rc = plan_fft_state(p)) != PF_OK) {
         pf_plan_free(p);
         return rc;
     }
@@ -228,6 +241,19 @@ int pf_plan_get_permutation(const pf_plan* p, int* perm, void* stream) {
     PF_HIP(hipMemcpyAsync(perm, p->perm, p->mtot * sizeof(int), hipMemcpyDeviceToDevice,
                           static_cast<hipStream_t>(stream)));
     return PF_OK;
+}
+
+int pf_describe_launch(const pf_plan* p, const pf_batch_desc* d, char* buf, size_t cap) {
+    pf_batch b;
+    int rc = make_batch(p, d, &b);
+    if (rc != PF_OK) return rc;
+    if (!buf || cap == 0) return pf_fail(PF_ERR_ARG, "null output buffer");
+    const std::string text = "gather=" + pf_gather_name(p, b) + " spread=" + pf_spread_name(p, b) +
+                             " fft=" + (pf_fftk_supported(p) ? "fused" : "rocfft");
+    const size_t n = std::min(cap - 1, text.size());
+    std::copy(text.begin(), text.begin() + n, buf);
+    buf[n] = 0;
+    return text.size() < cap ? PF_OK : pf_fail(PF_ERR_ARG, "output buffer too small");
 }
 
 int pf_grid_size(const pf_plan* p, const pf_batch_desc* d, size_t* bytes) {

@@ -378,7 +378,7 @@ int pf_cart_create(const float* mask, int mframes, int ndim, const int* shape, v
     int rc = c->geo ? pf_plan_params(c->geo, ndim, shape, w, sg, 1, 1, nullptr, 0)
                     : pf_fail(PF_ERR_ALLOC, "host allocation failed");
     if (rc == PF_OK) rc = pf_plan_upload_apod(c->geo);
-    if (rc == PF_OK && pf_fftk_supported(c->geo)) rc = pf_fftk_init(c->geo);
+    if (rc == PF_OK) rc = pf_fftk_supported(c->geo) ? pf_fftk_init(c->geo) : pf_fft_cache_init(c->geo);
     if (rc == PF_OK && mask) {
         c->mframes = mframes;
         const size_t n = (size_t)mframes * pf_ntot(c->geo) * sizeof(float);

@@ -10,6 +10,7 @@
 #include <rocfft/rocfft.h>
 
 #include <map>
+#include <new>
 #include <mutex>
 #include <tuple>
 
@@ -40,10 +41,21 @@ void pf_fft_destroy(pf_fft_cache* c) {
     delete c;
 }
 
-static int get_entry(pf_plan* p, int RC, int inverse, pf_fft_entry** out) {
+int pf_fft_cache_init(pf_plan* p) {
     std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
     if (g_rocfft_setup != rocfft_status_success) return pf_fail(PF_ERR_FFT, "rocfft_setup failed");
-    if (!p->fft) p->fft = new pf_fft_cache();
+    if (!p->fft) p->fft = new (std::nothrow) pf_fft_cache();
+    if (!p->fft) return pf_fail(PF_ERR_ALLOC, "host allocation failed");
+    return PF_OK;
+}
+
+static int get_entry(pf_plan* p, int RC, int inverse, pf_fft_entry** out) {
+    // the cache exists from plan creation, so concurrent first calls only
+    // meet under its lock
+    if (!p->fft) {
+        int rc = pf_fft_cache_init(p);
+        if (rc != PF_OK) return rc;
+    }
     std::lock_guard<std::mutex> lk(p->fft->mu);
     auto key = std::make_tuple(RC, inverse);
     auto it = p->fft->entries.find(key);

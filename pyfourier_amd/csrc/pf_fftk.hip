@@ -914,6 +914,14 @@ int pf_fftk_init(pf_plan* p) {
     return PF_OK;
 }
 
+// The fused-FFT state (twiddle tables, raised LDS limits) is built once when a
+// plan or operator is created; operator calls only check that it is there, so
+// they neither allocate nor synchronise and may be captured into a graph.
+int pf_fftk_ready(const pf_plan* p) {
+    if (p->fftk) return PF_OK;
+    return pf_fail(PF_ERR_ARG, "plan has no fused-FFT state (it was not built by a pf_*_create call)");
+}
+
 void pf_fftk_destroy(pf_fftk* f) {
     if (!f) return;
     for (auto& X : f->ax)
@@ -1050,7 +1058,7 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
 
 // image -> grid (deapodise, pad, FFT); `extra` holds pf_fftk_extra_bytes
 int pf_fftk_forward(pf_plan* p, const pf_batch& b, const float2* img, float2* grid, float2* extra, hipStream_t s) {
-    int rc = pf_fftk_init(p);
+    int rc = pf_fftk_ready(p);
     if (rc != PF_OK) return rc;
     const int64_t RC = (int64_t)b.R * b.Cv;
     const int64_t FC = (int64_t)b.nfg * b.Cb;
@@ -1070,7 +1078,7 @@ int pf_fftk_forward(pf_plan* p, const pf_batch& b, const float2* img, float2* gr
 // grid -> image (inverse FFT, crop, deapodise, times factor); the grid is consumed
 int pf_fftk_adjoint(pf_plan* p, const pf_batch& b, float2* grid, float2* img, float2* extra, float factor,
                     hipStream_t s) {
-    int rc = pf_fftk_init(p);
+    int rc = pf_fftk_ready(p);
     if (rc != PF_OK) return rc;
     const int64_t RC = (int64_t)b.R * b.Cv;
     const int64_t FC = (int64_t)b.nfg * b.Cb;
@@ -1173,7 +1181,7 @@ static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src
 
 int pf_fftk_cart(pf_plan* p, int64_t planes, int T, const float* mask, int mframes, bool fwd, const float2* src,
                  float2* dst, float factor, hipStream_t s) {
-    int rc = pf_fftk_init(p);
+    int rc = pf_fftk_ready(p);
     if (rc != PF_OK) return rc;
     const int64_t Nz = p->shape[0], Ny = p->shape[1], Nx = p->shape[2];
     const CartEpi none{nullptr, 1, 0, 1.0f};

@@ -140,12 +140,18 @@ int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2*
                    const float* weight, hipStream_t s);
 int pf_fft_exec(pf_plan* p, const pf_batch& b, float2* grid, int inverse, hipStream_t s);
 void pf_fft_destroy(pf_fft_cache* c);
+int pf_fft_cache_init(pf_plan* p);  // rocFFT setup + the plan's (empty) cache
 size_t pf_spread_partial_bytes(const pf_plan* p, const pf_batch& b);
 int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b);
+// names of the gather / spread kernel instances a batch runs (diagnostics)
+std::string pf_gather_name(const pf_plan* p, const pf_batch& b);
+std::string pf_spread_name(const pf_plan* p, const pf_batch& b);
+int pf_ops_prepare();  // kernel attributes, set at plan creation
 
 // fused deapodise/pad/FFT and inverse FFT/crop/deapodise (pf_fftk.hip)
 bool pf_fftk_supported(const pf_plan* p);
 int pf_fftk_init(pf_plan* p);
+int pf_fftk_ready(const pf_plan* p);  // PF_OK when pf_fftk_init has run
 void pf_fftk_destroy(struct pf_fftk* f);
 size_t pf_fftk_extra_bytes(const pf_plan* p, const pf_batch& b);
 int pf_fftk_forward(pf_plan* p, const pf_batch& b, const float2* img, float2* grid, float2* extra, hipStream_t s);

@@ -1636,22 +1636,46 @@ const GatherLaunch GATHER_W[2][2][PF_WMAX] = {
       gw_sq<2, 8>()}},
 };
 
-int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, float2* ys, hipStream_t s) {
-    GeoParams g = geo(p);
-    BatchParams B = bparams(b);
-    const int cpl = b.Cv % 2 == 0 ? 2 : 1;
+// Which gather instance a batch runs: the unrolled k_gather_w for square (or
+// 1-D) y/x widths with 32-bit grid offsets (the DEEP two-wave instances for
+// 3-D), else the generic k_gather; cpl = coils per lane.
+struct GatherChoice {
+    GatherLaunch fn = nullptr;  // null: generic k_gather<cpl>
+    int cpl = 1;
+    bool deep = false;
+};
+
+static GatherChoice gather_choice(const pf_plan* p, const pf_batch& b) {
+    GatherChoice c;
+    c.cpl = b.Cv % 2 == 0 ? 2 : 1;
     const int wy = p->width[1], wx = p->width[2];
     const bool idx32 = (int64_t)pf_gtot(p) * b.R * b.Cv < (1ll << 31);
     if (idx32 && (wy == 1 || wy == wx)) {
-        const bool deep = p->width[0] > 1 && wy == wx && wy > 1;
-        if (GatherLaunch fn = deep ? GATHER_W_DEEP[cpl - 1][wx - 1] : GATHER_W[cpl - 1][wy == 1 ? 0 : 1][wx - 1]) {
-            const int64_t threads = p->mtot * (b.C / cpl);
-            fn(dim3(nblk(threads, GTB)), s, g, B, p, grid, ys);
-            PF_HIP(hipGetLastError());
-            return PF_OK;
-        }
+        c.deep = p->width[0] > 1 && wy == wx && wy > 1;
+        c.fn = c.deep ? GATHER_W_DEEP[c.cpl - 1][wx - 1] : GATHER_W[c.cpl - 1][wy == 1 ? 0 : 1][wx - 1];
+        if (!c.fn) c.deep = false;
     }
-    if (b.Cv % 2 == 0) {
+    return c;
+}
+
+std::string pf_gather_name(const pf_plan* p, const pf_batch& b) {
+    const GatherChoice c = gather_choice(p, b);
+    if (!c.fn) return "k_gather<" + std::to_string(c.cpl) + ">";
+    return "k_gather_w<" + std::to_string(c.cpl) + "," + std::to_string(p->width[1]) + "," +
+           std::to_string(p->width[2]) + "," + (c.deep ? "true" : "false") + ">";
+}
+
+int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, float2* ys, hipStream_t s) {
+    GeoParams g = geo(p);
+    BatchParams B = bparams(b);
+    const GatherChoice c = gather_choice(p, b);
+    if (c.fn) {
+        const int64_t threads = p->mtot * (b.C / c.cpl);
+        c.fn(dim3(nblk(threads, GTB)), s, g, B, p, grid, ys);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    if (c.cpl == 2) {
         int64_t threads = p->mtot * (b.C / 2);
         hipLaunchKernelGGL(k_gather<2>, dim3(nblk(threads, TB)), dim3(TB), 0, s, g, B, p->mtot, p->taprec,
                            p->perm, grid, ys);
@@ -1747,6 +1771,38 @@ static RegChoice spread_reg_choice(const pf_plan* p, const pf_batch& b) {
     return c;
 }
 
+// Plain values (no basis, no B0) in whole chunks of the wave's VL coils: a
+// wave's values of a sample are one contiguous run, copied global -> LDS with
+// 32-bit offsets (sorted samples under 4 GiB) by the pipelined instances.
+static bool spread_dma(const pf_plan* p, const pf_batch& b, const RegChoice& rc) {
+    return !b.basis && !b.b0t && b.R == 1 && rc.vl > 0 && b.C % rc.vl == 0 &&
+           (uint64_t)p->mtot * b.C * sizeof(float2) < (1ull << 32);
+}
+
+std::string pf_spread_name(const pf_plan* p, const pf_batch& b) {
+    const RegChoice rc = spread_reg_choice(p, b);
+    std::string name;
+    if (!rc.fn) {
+        name = "k_spread";
+    } else {
+        const RegTile& t = *rc.tile;
+        const int zp = t.x * rc.vl >= 64 ? 1 : 2;
+        const int txl = t.x * rc.vl * zp / 64;
+        const std::string tile = std::to_string(t.z) + "," + std::to_string(t.y) + "," + std::to_string(txl);
+        if (spread_dma(p, b, rc) && PF_SPREAD_PIPE)
+            name = (rc.vl == SPREAD_VL && zp == 1) ? "k_spread_pipe<" + tile + ">"
+                                                    : "k_spread_pipe_s<" + tile + "," + std::to_string(rc.vl) + "," +
+                                                          std::to_string(zp) + ">";
+        else
+            name = "k_spread_reg<" + tile + "," + std::to_string(rc.vl) + "," + std::to_string(zp) + ">";
+    }
+    const int VC = pf_spread_coil_chunk(p, b);
+    const int V = b.Cb * b.R * b.Cv;
+    const int nvch = (V + VC - 1) / VC;
+    if (pf_sitems_for(pf_slist_for(p, b), nvch).nritems > 0) name += "+k_spread_reduce";
+    return name;
+}
+
 int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b) {
     const RegChoice rc = spread_reg_choice(p, b);
     if (rc.fn) return rc.vl;
@@ -1806,13 +1862,7 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     if ((int64_t)I.nritems * nvch * nsub > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many reduce blocks");
     const RegChoice rc = spread_reg_choice(p, b);
     if (rc.fn) {
-        // one value per coil (no basis, no B0) in whole chunks of the wave's
-        // VL coils: a wave's values of a sample are one contiguous run, copied
-        // global -> LDS with 32-bit offsets (sorted samples under 4 GiB)
-        B.dma = (!b.basis && !b.b0t && b.R == 1 && b.C % rc.vl == 0 &&
-                 (uint64_t)p->mtot * b.C * sizeof(float2) < (1ull << 32))
-                    ? 1
-                    : 0;
+        B.dma = spread_dma(p, b, rc) ? 1 : 0;
         rc.fn(dim3((unsigned)nb), s, g, B, L, I, p, ys, grid, partial);
         PF_HIP(hipGetLastError());
         if (I.nritems > 0) {
@@ -1822,12 +1872,6 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
         }
         return PF_OK;
     }
-    static bool attr_done = false;
-    if (!attr_done) {
-        PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_spread),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_done = true;
-    }
     hipLaunchKernelGGL(k_spread, dim3((unsigned)nb), dim3(TB), lds, s, g, B, VC, I.sitems, L.slist, p->taprec,
                        p->perm, ys, grid, partial);
     PF_HIP(hipGetLastError());
@@ -1836,6 +1880,14 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
                            VC, nsub, I.ritems, partial, grid);
         PF_HIP(hipGetLastError());
     }
+    return PF_OK;
+}
+
+// Kernel attributes the operator launches rely on, set once per plan at
+// creation (on the plan's device) rather than on the call path.
+int pf_ops_prepare() {
+    PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_spread), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
     return PF_OK;
 }
 

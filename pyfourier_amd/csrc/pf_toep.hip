@@ -18,6 +18,7 @@
 // the sampling mask (or its per-voxel subspace/B0 matrix) in FFT order.
 #include <algorithm>
 #include <new>
+#include <string>
 
 #include "pf_internal.h"
 
@@ -72,6 +73,36 @@ __global__ void __launch_bounds__(TB) k_toep_mul(const float2* __restrict__ T, f
     }
 }
 
+// Blocks larger than any unrolled instance (16 < R <= PF_TOEP_RMAX): the same
+// in-place product with each thread's input vector staged in LDS (column-major
+// over the block's threads, so the q-th values of consecutive threads are
+// consecutive words).
+constexpr int TB_GEN = 64;
+__global__ void __launch_bounds__(TB_GEN) k_toep_mul_lds(const float2* __restrict__ T, float2* __restrict__ grid,
+                                                         int64_t gtot, int R, int Cv, int Cb, int nfr) {
+    extern __shared__ float2 u_lds[];  // (R, TB_GEN)
+    const int64_t total = (int64_t)nfr * Cb * gtot * Cv;
+    const int64_t e = (int64_t)blockIdx.x * TB_GEN + threadIdx.x;
+    if (e >= total) return;  // no block-wide barrier below: each thread uses only its own column
+    const int cv = (int)(e % Cv);
+    const int64_t pt = (e / Cv) % gtot;
+    const int64_t fcb = e / (Cv * gtot);
+    const int64_t f = fcb / Cb;
+    float2* g = grid + (fcb * gtot + pt) * (int64_t)R * Cv + cv;
+    const float2* t = T + (f * gtot + pt) * (int64_t)R * R;
+    float2* u = u_lds + threadIdx.x;
+    for (int q = 0; q < R; ++q) u[q * TB_GEN] = g[(int64_t)q * Cv];
+    for (int r = 0; r < R; ++r) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int q = 0; q < R; ++q) {
+            const float2 a = t[r * R + q], v = u[q * TB_GEN];
+            acc.x += a.x * v.x - a.y * v.y;
+            acc.y += a.x * v.y + a.y * v.x;
+        }
+        g[(int64_t)r * Cv] = acc;
+    }
+}
+
 pf_batch toep_batch(const pf_toep* t, int C, const float2* b0s) {
     pf_batch b;
     b.C = C;
@@ -108,7 +139,9 @@ int toep_create(const void* kernel, int ndim, const int* shape, int nfr, int nse
     if (nseg < 1 || ncoeff < 1 || nfr < 1) return pf_fail(PF_ERR_ARG, "nframes, nseg and ncoeff must be >= 1");
     if (nfr > 1 && ncoeff > 1) return pf_fail(PF_ERR_ARG, "per-frame kernels and a subspace basis exclude each other");
     const int R = nseg * ncoeff;
-    if (R > 16) return pf_fail(PF_ERR_UNSUPPORTED, "Toeplitz kernel blocks larger than 16x16");
+    if (R > PF_TOEP_RMAX)
+        return pf_fail(PF_ERR_UNSUPPORTED, "Toeplitz kernel blocks larger than " + std::to_string(PF_TOEP_RMAX) + "x" +
+                                               std::to_string(PF_TOEP_RMAX) + " (nseg * ncoeff)");
     pf_toep* t = new (std::nothrow) pf_toep();
     if (!t) return pf_fail(PF_ERR_ALLOC, "host allocation failed");
     t->R = R;
@@ -120,7 +153,7 @@ int toep_create(const void* kernel, int ndim, const int* shape, int nfr, int nse
     float s[3] = {sigma, sigma, sigma};
     int rc = t->geo ? pf_plan_params(t->geo, ndim, shape, w, s, 1, 1, nullptr, 0) : pf_fail(PF_ERR_ALLOC, "host allocation");
     if (rc == PF_OK) rc = pf_plan_upload_apod(t->geo);
-    if (rc == PF_OK && pf_fftk_supported(t->geo)) rc = pf_fftk_init(t->geo);
+    if (rc == PF_OK) rc = pf_fftk_supported(t->geo) ? pf_fftk_init(t->geo) : pf_fft_cache_init(t->geo);
     if (rc == PF_OK) {
         const size_t n = (size_t)nfr * pf_gtot(t->geo) * R * R * sizeof(float2);
         hipError_t e = hipMalloc(&t->T, n);
@@ -198,8 +231,14 @@ int pf_toeplitz_apply(pf_toep* t, int batch, const void* image, void* out, const
         hipLaunchKernelGGL(k_toep_mul<1>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, 1, b.Cv, b.Cb, b.nfg);
     else if (t->R <= 4)
         hipLaunchKernelGGL(k_toep_mul<4>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, t->R, b.Cv, b.Cb, b.nfg);
-    else
+    else if (t->R <= 16)
         hipLaunchKernelGGL(k_toep_mul<16>, dim3(nb), dim3(TB), 0, s, t->T, grid, gtot, t->R, b.Cv, b.Cb, b.nfg);
+    else {
+        const int64_t ngen = (total + TB_GEN - 1) / TB_GEN;
+        if (ngen > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "grid too large for the generic Toeplitz multiply");
+        hipLaunchKernelGGL(k_toep_mul_lds, dim3((unsigned)ngen), dim3(TB_GEN), (size_t)t->R * TB_GEN * sizeof(float2), s,
+                           t->T, grid, gtot, t->R, b.Cv, b.Cb, b.nfg);
+    }
     PF_HIP(hipGetLastError());
     const float factor = (float)(1.0 / (double)gtot);
     if (fused) return pf_fftk_adjoint(t->geo, b, grid, static_cast<float2*>(out), extra, factor, s);

@@ -1,4 +1,6 @@
-"""Pipe-Menon density compensation: drop-in pipe_menon_dcf.
+"""Pipe-Menon density compensation: drop-in pipemenon.pipe_menon_dcf
+(the reference exports this module as pyfourier.pipemenon,
+src/pyfourier/__init__.py:15).
 
 Mirrors src/pyfourier/_dcomp/pipemenon.py:9-115: starting from w = 1,
 iterate w <- w / |G G^H w| with G the forward NUFFT under norm="ortho"
@@ -49,7 +51,7 @@ def pipe_menon_dcf(coord, shape, niter=30, threadsperblock=128, device=None, wid
             side = torch.cuda.Stream(dev)
             side.wait_stream(torch.cuda.current_stream(dev))
             with torch.cuda.stream(side):
-                step()  # warm-up (lazily built FFT state); counts as iteration 1
+                step()  # warm-up on a side stream; counts as iteration 1
             torch.cuda.current_stream(dev).wait_stream(side)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):

@@ -102,7 +102,8 @@ def test_device_resolution():
         (lambda: demo.golden_means_radial_3d(64, 32), (20, 24, 32)),
         (lambda: demo.variable_density_spiral_2d(8, 512, 48), 48),
         (lambda: demo.golden_angle_radial_2d(8, 16) / (2 * np.pi), 32),  # cycles/FOV units
-        (lambda: demo.golden_angle_radial_2d(8, 16) * 10.0, 32),  # grid units
+        (lambda: demo.golden_angle_radial_2d(8, 16) * 3.0, 32),  # grid units (max |k| 3 pi <= N/2)
+        (lambda: demo.golden_angle_radial_2d(8, 16) * 10.0, 32),  # other units (max |k| 10 pi > N/2)
     ],
 )
 def test_normalisation_matches_oracle(maker, N):
@@ -113,12 +114,94 @@ def test_normalisation_matches_oracle(maker, N):
     np.testing.assert_array_equal(mine, R.normalize_coordinates(c, shape[::-1]))
 
 
-def test_normalisation_units():
+# Closed forms of every unit branch (_subroutines/_traj.py:10-54 intent:
+# any unit -> grid units [-N/2, N/2)), independent of both implementations.
+def _norm(k, shape_xyz, **kw):
+    return pf.normalize_coordinates(torch.from_numpy(np.asarray(k, np.float32)), shape_xyz, **kw).numpy()
+
+
+def test_normalisation_radians():
     k = np.array([[-math.pi, 0.0], [0.5 * math.pi, 0.25]], dtype=np.float32)
-    out = pf.normalize_coordinates(torch.from_numpy(k), [64, 32]).numpy()
-    np.testing.assert_allclose(out, k * np.array([64, 32]) / (2 * math.pi), rtol=1e-6)
-    g = np.array([[-10.0, 3.0], [7.5, -20.0]], dtype=np.float32)  # grid units pass through
-    np.testing.assert_array_equal(pf.normalize_coordinates(torch.from_numpy(g), [64, 64]).numpy(), g)
+    np.testing.assert_allclose(_norm(k, [64, 32]), k * np.array([64, 32]) / (2 * math.pi), rtol=1e-6)
+    # a square k-space in radians: corners at radius pi*sqrt(2), edge at pi
+    sq = np.array([[-math.pi, -math.pi], [math.pi * 0.99, math.pi * 0.5]], dtype=np.float32)
+    np.testing.assert_allclose(_norm(sq, [64, 64]), sq * 64 / (2 * math.pi), rtol=1e-6)
+    assert _norm(sq, [64, 64]).min() == pytest.approx(-32.0, rel=1e-6)
+
+
+def test_normalisation_cycles():
+    k = np.array([[-0.5, 0.125], [0.25, -0.375], [0.0, 0.4]], dtype=np.float32)
+    # k * N: exact in float32 for these dyadic values
+    np.testing.assert_array_equal(_norm(k, [64, 32]), k * np.array([64, 32], np.float32))
+    # a square k-space in cycles/FOV (corners at radius 0.707 > 0.5)
+    sq = np.array([[-0.5, -0.5], [0.25, 0.5]], dtype=np.float32)
+    np.testing.assert_array_equal(_norm(sq, [16, 16]), sq * 16)
+
+
+def test_normalisation_grid_units_pass_through():
+    g = np.array([[-10.0, 3.0], [7.5, -20.0], [31.99, 32.0]], dtype=np.float32)
+    np.testing.assert_array_equal(_norm(g, [64, 64]), g)
+    # per-axis bound: |k_x| <= Nx/2 and |k_y| <= Ny/2 (the radius may exceed both)
+    g2 = np.array([[-32.0, -8.0], [31.0, 7.9]], dtype=np.float32)
+    np.testing.assert_array_equal(_norm(g2, [64, 16]), g2)
+    # a small grid-unit trajectory (max |k| <= pi) is radians unless said otherwise
+    small = np.array([[-3.0, 0.0], [1.5, 2.0]], dtype=np.float32)
+    np.testing.assert_allclose(_norm(small, [64, 64]), small * 64 / (2 * math.pi), rtol=1e-6)
+    np.testing.assert_array_equal(_norm(small, [64, 64], units="grid"), small)
+
+
+def test_normalisation_physical_units():
+    # 1/m: resolution 1 mm -> k_max = 500 /m; the edge maps to N/2
+    kmax = 500.0
+    k = np.array([[-kmax, 0.0], [0.5 * kmax, 0.25 * kmax], [0.1 * kmax, -kmax]], dtype=np.float32)
+    out = _norm(k, [128, 96])
+    np.testing.assert_allclose(out, k / kmax * np.array([64.0, 48.0]), rtol=1e-6)
+    assert out[0, 0] == pytest.approx(-64.0, rel=1e-7) and out[2, 1] == pytest.approx(-48.0, rel=1e-7)
+    # rad/m, 3-D, forced: same rule
+    k3 = np.array([[0.0, 0.0, -3141.6], [1570.8, -1000.0, 200.0]], dtype=np.float32)
+    np.testing.assert_allclose(_norm(k3, [32, 32, 16], units="max"), k3 / 3141.6 * np.array([16, 16, 8]), rtol=1e-6)
+
+
+def test_normalisation_center_keyword_and_units_argument():
+    k = np.array([[-0.5, 0.25]], dtype=np.float32)
+    np.testing.assert_array_equal(_norm(k, [64, 32], center=False), k * np.array([64, 32], np.float32) + [32, 16])
+    with pytest.raises(ValueError, match="units"):
+        _norm(k, [64, 32], units="furlongs")
+
+
+@pytest.mark.parametrize("bad", [float("nan"), float("inf"), -float("inf")])
+def test_non_finite_coordinates_rejected(bad):
+    c = demo.golden_angle_radial_2d(4, 16).astype(np.float32)
+    c[1, 3, 0] = bad
+    with pytest.raises(ValueError, match="finite"):
+        pf.plan_nufft(torch.from_numpy(c), 32, width=4, oversamp=2.0)
+    with pytest.raises(ValueError, match="finite"):
+        pf.normalize_coordinates(torch.from_numpy(c), [32, 32])
+    with pytest.raises(ValueError, match="finite"):
+        R.normalize_coordinates(c, [32, 32])
+
+
+def test_reference_import_surface():
+    """The reference's top-level names, including the pipemenon module
+    (src/pyfourier/__init__.py:9-15)."""
+    for name in ("fftn", "ifftn", "plan_fft", "sparse_fftn", "sparse_ifftn", "plan_spfft", "nufft", "nufft_adj",
+                 "plan_nufft", "gram", "plan_gram", "sparse_gram", "plan_spgram", "fft", "ifft"):
+        assert callable(getattr(pf, name)), name
+    assert callable(pf.pipemenon.pipe_menon_dcf)
+    import pyfourier_amd.pipemenon as pm
+
+    assert pm.pipe_menon_dcf is pf.pipe_menon_dcf
+
+
+def test_gram_block_limit_checked_before_planning():
+    """R = nseg * ncoeff above PF_TOEP_RMAX is refused before any PSF is
+    computed (no GPU touched: the check precedes the native calls)."""
+    c = demo.golden_angle_radial_2d(4, 16).astype(np.float32)
+    basis = np.ones((1, _lib.PF_TOEP_RMAX + 1), np.complex64)
+    with pytest.raises(ValueError, match="exceeds"):
+        pf.plan_spgram(torch.from_numpy(c), 16, basis=basis, device="cuda:0")
+    text = open(os.path.join(ROOT, "include", "pfhip.h")).read()
+    assert f"#define PF_TOEP_RMAX {_lib.PF_TOEP_RMAX}" in text
 
 
 def test_plan_parameters_on_host():

@@ -39,3 +39,22 @@ def test_headline_traffic_found_for_the_dominant_kernels():
         assert t is not None, stage
         # at least the compulsory bytes (grid once, samples once) moved
         assert t >= 32 * 256**3 * 8 + 32 * 8192 * 256 * 8
+
+
+def test_fp64_gram_rows_equal_full_normal_operator():
+    """bench.py's row-wise FP64 A^H W A (the Toeplitz workload's reference)
+    equals the full exact normal operator on the same rows."""
+    import numpy as np
+
+    from oracle import nudft as U
+    from pyfourier_amd import demo
+
+    rng = np.random.default_rng(3)
+    N = 16
+    c = demo.variable_density_spiral_2d(3, 64, N).reshape(-1, 2).astype(np.float64) * N / (2 * np.pi)
+    w = rng.uniform(0.5, 1.5, c.shape[0])
+    x = demo.complex_gaussian(rng, (N, N))
+    full = U.nudft_adj((w * U.nudft(x[None], c, [N, N])[0])[None], c, [N, N])[0]
+    rows = [1, 6, 11]
+    got = bench._gram_exact_rows(x, c, w, N, rows)
+    assert np.linalg.norm(got - full[rows]) / np.linalg.norm(full[rows]) < 1e-12

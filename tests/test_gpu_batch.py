@@ -2,8 +2,8 @@
 
 Every replay must equal the per-item drop-in calls on the same plans (the
 same kernels in the same order: bit-identical), replays are repeated with
-eager work and synchronisation in between, and one item is checked against
-the CPU restatement.
+eager work and synchronisation in between, and every item is checked against
+the CPU restatement, forward and adjoint.
 """
 
 import numpy as np
@@ -62,7 +62,8 @@ def test_batch_replays_equal_per_item_calls(cuda_device, norm, group):
         torch.cuda.synchronize()
         for o, r in zip(out, ref_y):
             assert torch.equal(o.reshape(r.shape), r), f"forward replay {rep}"
-        y0 = out[0].clone()
+        # the k-space views are the adjoint's input buffers too: keep copies
+        fwd = [o.clone() for o in out]
         # eager work between replays
         junk = torch.randn(1 << 20, device=cuda_device).sum()
         torch.cuda.synchronize()
@@ -71,10 +72,13 @@ def test_batch_replays_equal_per_item_calls(cuda_device, norm, group):
         for o, r in zip(outa, ref_x):
             assert torch.equal(o.reshape(r.shape), r), f"adjoint replay {rep}"
         del junk
-    # one item against the restatement
-    P = R.Plan(coords[0], shapes[0], width=4, oversamp=2.0)
-    yr = R.nufft(imgs[0].cpu().numpy(), P, weight=weights[0], norm=norm)
-    assert rel(y0.cpu().numpy(), yr) < PARITY
+    # every item against the restatement, forward and adjoint (last replay)
+    for i in range(len(coords)):
+        P = R.Plan(coords[i], shapes[i], width=4, oversamp=2.0)
+        yr = R.nufft(imgs[i].cpu().numpy(), P, weight=weights[i], norm=norm)
+        assert rel(fwd[i].cpu().numpy(), yr) < PARITY, f"item {i} forward"
+        xr = R.nufft_adj(ksps[i].cpu().numpy(), P, weight=weights[i], norm=norm)
+        assert rel(outa[i].cpu().numpy(), xr) < PARITY, f"item {i} adjoint"
 
 
 def test_batch_input_buffers_in_place(cuda_device):

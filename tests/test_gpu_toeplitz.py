@@ -1,8 +1,10 @@
 """GPU tests of the Toeplitz Gram (plan_spgram / sparse_gram).
 
-The Gram must equal the normal operator nufft_adj(w * nufft(x)) of the NUFFT
-it replaces up to kernel-approximation error, and the oracle's Toeplitz apply
-with the FP64-exact PSF up to the PSF's own NUFFT error.
+The Gram must equal the oracle's Toeplitz apply with FP64-exact PSFs (the
+plain kernel, or the per-voxel R x R block of the subspace and field-map
+Grams) up to the PSF's own NUFFT error, and the normal operator
+nufft_adj(w * nufft(x)) of the NUFFT it replaces up to kernel-approximation
+error.
 """
 
 import math
@@ -74,8 +76,17 @@ def test_gram_subspace(cuda_device):
     basis = demo.t2_subspace_basis(rng, F, K)
     G = pf.plan_spgram(torch.from_numpy(coord), N, basis=torch.from_numpy(basis), oversamp=2.0,
                        device=cuda_device)
-    x = torch.from_numpy(demo.complex_gaussian(rng, (C, K, N, N))).to(cuda_device)
+    xh = demo.complex_gaussian(rng, (C, K, N, N))
+    x = torch.from_numpy(xh).to(cuda_device)
     gx = pf.sparse_gram(x, G, device=cuda_device)
+    # oracle: R x R block Toeplitz (per-voxel K x K, _bmatvec/_bdot.py:130-157)
+    # with the FP64-exact PSFs, c_k(j) = basis[frame(j), k]
+    P = R.Plan(coord, N, width=6, oversamp=2.0)
+    M = coord.shape[1] * coord.shape[2]
+    coefs = basis[np.repeat(np.arange(F), M)]
+    T = R.toeplitz_kernel_block_exact(P.coord.reshape(-1, 2), (N, N), coefs, ghat=float(np.prod(P.os_shape)))
+    ref = np.stack([R.toeplitz_apply_block(xh[c], T, (N, N)) for c in range(C)])
+    assert rel(gx.cpu().numpy(), ref) < 1e-4
     plan = pf.plan_nufft(torch.from_numpy(coord), N, width=6, oversamp=2.0)
     b = torch.from_numpy(basis)
     ex = pf.nufft_adj(pf.nufft(x, nufft_plan=plan, basis=b, device=cuda_device), nufft_plan=plan, basis=b,
@@ -91,12 +102,48 @@ def test_gram_b0(cuda_device):
     t = np.linspace(0, 8e-3, 512).astype(np.float32)
     G = pf.plan_spgram(torch.from_numpy(coord), N, zmap=torch.from_numpy(zmap), T=torch.from_numpy(t), L=L,
                        nbins=40, oversamp=2.0, device=cuda_device)
-    x = torch.from_numpy(demo.complex_gaussian(rng, (C, N, N))).to(cuda_device)
+    xh = demo.complex_gaussian(rng, (C, N, N))
+    x = torch.from_numpy(xh).to(cuda_device)
     gx = pf.sparse_gram(x, G, device=cuda_device)
+    # oracle: R x R block Toeplitz over the L segments (_bdot.py:143-157 with
+    # the segment loop of _sparsegram.py:26-53), FP64-exact PSFs, c_l(j) = Bt[l, t(j)]
+    bt, ct = pf.mri_exp_approx(torch.from_numpy(zmap), torch.from_numpy(t), L, (40, 40))
+    bt, ct = bt.numpy(), ct.numpy()
+    P = R.Plan(coord, N, width=6, oversamp=2.0)
+    M = coord.shape[-3] * coord.shape[-2]
+    coefs = bt[:, np.arange(M) % coord.shape[-2]].T
+    T = R.toeplitz_kernel_block_exact(P.coord.reshape(-1, 2), (N, N), coefs, ghat=float(np.prod(P.os_shape)))
+    ref = np.stack([R.toeplitz_apply_block(xh[c][None], T, (N, N), ct)[0] for c in range(C)])
+    assert rel(gx.cpu().numpy(), ref) < 1e-4
     plan = pf.plan_nufft(torch.from_numpy(coord), N, width=6, oversamp=2.0, zmap=torch.from_numpy(zmap),
                          T=torch.from_numpy(t), L=L, nbins=40)
     ex = pf.nufft_adj(pf.nufft(x, nufft_plan=plan, device=cuda_device), nufft_plan=plan, device=cuda_device)
     assert rel(gx.cpu().numpy(), ex.cpu().numpy()) < 2e-4
+
+
+def test_gram_subspace_b0_generic_block(cuda_device):
+    """Subspace basis and field map together: R = L * K = 24 > 16, the
+    block size beyond the unrolled instances (LDS-staged product)."""
+    rng = np.random.default_rng(5)
+    F, K, L, N, C, ns = 6, 4, 6, 16, 2, 96
+    base = demo.variable_density_spiral_2d(1, ns, N)[0]
+    coord = np.stack([base @ np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]]).T
+                      for a in 2.39996 * np.arange(F)])[:, None].astype(np.float32)  # (F, 1, ns, 2)
+    basis = demo.t2_subspace_basis(rng, F, K).astype(np.complex64)
+    zmap = demo.smooth_field_map(rng, (N, N), 60.0)
+    t = np.linspace(0, 5e-3, ns).astype(np.float32)
+    G = pf.plan_spgram(torch.from_numpy(coord), N, basis=torch.from_numpy(basis), zmap=torch.from_numpy(zmap),
+                       T=torch.from_numpy(t), L=L, nbins=40, oversamp=2.0, device=cuda_device)
+    xh = demo.complex_gaussian(rng, (C, K, N, N))
+    gx = pf.sparse_gram(torch.from_numpy(xh).to(cuda_device), G, device=cuda_device).cpu().numpy()
+    bt, ct = pf.mri_exp_approx(torch.from_numpy(zmap), torch.from_numpy(t), L, (40, 40))
+    bt, ct = bt.numpy(), ct.numpy()
+    P = R.Plan(coord, N, width=6, oversamp=2.0)
+    M = F * ns
+    coefs = (bt[:, np.arange(M) % ns].T[:, :, None] * basis[np.arange(M) // ns][:, None, :]).reshape(M, L * K)
+    T = R.toeplitz_kernel_block_exact(P.coord.reshape(-1, 2), (N, N), coefs, ghat=float(np.prod(P.os_shape)))
+    ref = np.stack([R.toeplitz_apply_block(xh[c], T, (N, N), ct) for c in range(C)])
+    assert rel(gx, ref) < 1e-4
 
 
 def test_gram_autograd_is_self_adjoint(cuda_device):

@@ -215,6 +215,63 @@ def test_toeplitz_exact_kernel_equals_gram_matrix():
     assert rel(gx, explicit) < 1e-10
 
 
+@pytest.mark.parametrize("case", ["subspace", "b0", "subspace+b0"])
+def test_toeplitz_block_equals_segmented_normal_operator(case):
+    """The R x R block Toeplitz restatement with the exact PSF equals the
+    normal operator A^H W A of the NUDFT-based subspace / segmented field-map
+    model, built directly (no kernel):
+      A x = { y_f = sum_l Bt[l, t] * NUDFT_f(Ct_l * sum_k basis[f, k] x_k) }
+      A^H y = { sum_l conj(Ct_l) * sum_f conj(basis[f, k]) NUDFT_f^H(conj(Bt[l, t]) y_f) }.
+    Complex basis and field-map factors, so a transposed block, a missing
+    conjugate or a swapped (l, k) order fails."""
+    rng = np.random.default_rng(17)
+    N, F, K, L, ns = 10, 3, 2, 3, 40
+    shape = (N, N)
+    coord = np.stack([demo.golden_angle_radial_2d(2, ns)[0] @ np.array(
+        [[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]]).T for a in (0.0, 0.9, 2.1)])[:, None]
+    cg = R.Plan(coord, N, width=4, oversamp=2.0).coord.reshape(F, -1, 2).astype(np.float64)  # (F, 2*ns, 2)
+    M = cg.shape[1]
+    w = rng.uniform(0.5, 1.5, (F, M))
+    use_b = case != "b0"
+    use_z = case != "subspace"
+    Kb = K if use_b else 1
+    Lz = L if use_z else 1
+    basis = (rng.standard_normal((F, Kb)) + 1j * rng.standard_normal((F, Kb))) if use_b else np.ones((F, 1))
+    bt = (rng.standard_normal((Lz, M)) + 1j * rng.standard_normal((Lz, M))) if use_z else np.ones((1, M))
+    ct = (rng.standard_normal((Lz,) + shape) + 1j * rng.standard_normal((Lz,) + shape)) if use_z else None
+    maps = np.ones((1,) + shape) if ct is None else ct
+    x = demo.complex_gaussian(rng, (Kb,) + shape).astype(np.complex128)
+
+    out = np.zeros_like(x)
+    for f in range(F):
+        xf = np.tensordot(basis[f], x, axes=(0, 0))  # sum_k basis[f, k] x_k
+        yf = sum(bt[l] * U.nudft((maps[l] * xf)[None], cg[f], list(shape))[0] for l in range(Lz))
+        yf = w[f] * yf
+        back = sum(np.conj(maps[l]) * U.nudft_adj((np.conj(bt[l]) * yf)[None], cg[f], list(shape))[0]
+                   for l in range(Lz))
+        out += np.conj(basis[f])[:, None, None] * back[None]
+
+    frame = np.repeat(np.arange(F), M)
+    tidx = np.tile(np.arange(M), F)
+    coefs = (bt[:, tidx].T[:, :, None] * basis[frame][:, None, :]).reshape(F * M, Lz * Kb)
+    T = R.toeplitz_kernel_block_exact(cg.reshape(-1, 2), shape, coefs, w.reshape(-1))
+    gx = R.toeplitz_apply_block(x, T, shape, ct)
+    assert rel(gx, out) < 1e-10
+
+
+def test_toeplitz_block_reduces_to_plain():
+    rng = np.random.default_rng(18)
+    N = 12
+    coord = demo.golden_angle_radial_2d(6, 24)
+    cg = R.Plan(coord, N, width=4, oversamp=2.0).coord.reshape(-1, 2)
+    w = rng.uniform(0.5, 1.5, cg.shape[0])
+    x = demo.complex_gaussian(rng, (N, N))
+    T1 = R.toeplitz_kernel_exact(cg, (N, N), w)
+    Tb = R.toeplitz_kernel_block_exact(cg, (N, N), np.ones((cg.shape[0], 1)), w)
+    assert rel(Tb[0, 0], T1) < 1e-12
+    assert rel(R.toeplitz_apply_block(x[None], Tb, (N, N))[0], R.toeplitz_apply(x, T1, (N, N))) < 1e-12
+
+
 def test_port_matches_numpy_restatement():
     from oracle import port as Pt
 
