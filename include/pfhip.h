@@ -176,6 +176,26 @@ int pf_nufft_adj(pf_plan* plan, const pf_batch_desc* desc, const void* kspace, v
  * _noncart2cart/__init__.py:17-18); lets tests and profiles name what ran. */
 int pf_describe_launch(const pf_plan* plan, const pf_batch_desc* desc, char* buf, size_t cap);
 
+/* Fixed workspace of pf_nufft_noalloc: one device array per GPU. */
+#define PF_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* pf_nufft (op = 0, in = image, out = k-space) or pf_nufft_adj (op = 1,
+ * in = k-space, out = image) without a workspace argument and without any
+ * allocation: the workspace is a fixed array of PF_NOALLOC_SCRATCH_BYTES
+ * bytes in device memory, one per GPU, held until the process ends (its
+ * address is looked up when the first plan is created on that GPU).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory at call time (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same results as pf_nufft /
+ * pf_nufft_adj.  Limits: the call's pf_workspace_size must not exceed
+ * PF_NOALLOC_SCRATCH_BYTES and the grid sizes must be 2^a 3^b 5^c (rocFFT
+ * sizes allocate), else PF_ERR_UNSUPPORTED before any launch; the current
+ * device must be the plan's.  Every call on one GPU shares its array:
+ * calls that overlap in time on one GPU (other streams, other threads)
+ * race and must be serialised by the caller. */
+int pf_nufft_noalloc(pf_plan* plan, const pf_batch_desc* desc, int op, const void* in, void* out,
+                     const float* weight, int norm, void* stream);
+
 /* ---- stage entry points (the reference's kernel plug-in points) ------- */
 
 /* Grid bytes for one call: nfg*C*R*prod(os_shape)*8. */

@@ -19,6 +19,7 @@ PF_NORM_NONE = 0
 PF_NORM_ORTHO = 1
 PF_NORM_ADJOINT = 2
 PF_TOEP_RMAX = 128  # include/pfhip.h: largest Gram kernel block nseg * ncoeff
+PF_NOALLOC_SCRATCH_BYTES = 64 << 20  # include/pfhip.h: pf_nufft_noalloc's fixed workspace
 
 _ERRORS = {1: "invalid argument", 2: "HIP error", 3: "rocFFT error", 4: "allocation failed", 5: "unsupported"}
 
@@ -83,6 +84,7 @@ _SIG = {
     "pf_workspace_size": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, ctypes.POINTER(ctypes.c_size_t)]),
     "pf_nufft": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _I, _VP, ctypes.c_size_t, _VP]),
     "pf_nufft_adj": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _I, _VP, ctypes.c_size_t, _VP]),
+    "pf_nufft_noalloc": (_I, [_VP, ctypes.POINTER(BatchDesc), _I, _VP, _VP, _VP, _I, _VP]),
     "pf_describe_launch": (_I, [_VP, ctypes.POINTER(BatchDesc), ctypes.c_char_p, ctypes.c_size_t]),
     "pf_grid_size": (_I, [_VP, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_size_t)]),
     "pf_deapod_pad": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP]),

@@ -88,6 +88,9 @@ WsLayout layout(const pf_plan* p, const pf_batch& b) {
 int plan_fft_state(pf_plan* p) {
     int rc = pf_ops_prepare();
     if (rc != PF_OK) return rc;
+    // the no-allocation entry point's scratch address on this device; a
+    // failure here only disables that entry point (it then reports it)
+    if (pf_noalloc_prepare() != PF_OK) pf_set_error("");
     if (pf_fftk_supported(p)) return pf_fftk_init(p);
     return pf_fft_cache_init(p);
 }

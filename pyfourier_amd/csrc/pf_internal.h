@@ -147,6 +147,7 @@ int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b);
 std::string pf_gather_name(const pf_plan* p, const pf_batch& b);
 std::string pf_spread_name(const pf_plan* p, const pf_batch& b);
 int pf_ops_prepare();  // kernel attributes, set at plan creation
+int pf_noalloc_prepare();  // this device's pf_nufft_noalloc scratch address, at plan creation
 
 // fused deapodise/pad/FFT and inverse FFT/crop/deapodise (pf_fftk.hip)
 bool pf_fftk_supported(const pf_plan* p);
