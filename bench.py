@@ -228,7 +228,7 @@ def time_width(pf, stage_timer, coord, N, width, xc_dev, w_dev, dev, args):
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "nufft3d_128cube_32coil_w6_traffic.json")
-STAGE_KERNELS = {"spread": ("k_spread_pipe", "k_spread_reduce"), "interp": ("k_gather_w",)}
+STAGE_KERNELS = {"spread": ("k_spread_pipe", "k_spread_reduce"), "interp": ("k_gather_col",)}
 
 
 def measured_traffic(stage, args):

@@ -22,7 +22,13 @@ namespace {
 const size_t ALIGN = 256;
 size_t align_up(size_t n) { return (n + ALIGN - 1) / ALIGN * ALIGN; }
 
-int make_batch(const pf_plan* p, const pf_batch_desc* d, pf_batch* b) {
+// Grid layout of a call: the forward grid of batches the LDS-staged gather
+// runs on is coil-pair major (pf_gather_col_layout); every other grid keeps
+// all batch items innermost.  Entry points of the forward pipeline pass FWD,
+// so separate stage calls (pad+FFT, interp, interp_grad) agree on it.
+enum { DIR_ANY = 0, DIR_FWD = 1 };
+
+int make_batch(const pf_plan* p, const pf_batch_desc* d, pf_batch* b, int dir = DIR_ANY) {
     if (!p) return pf_fail(PF_ERR_ARG, "null plan");
     if (!d) return pf_fail(PF_ERR_ARG, "null batch description");
     if (d->batch < 1) return pf_fail(PF_ERR_ARG, "batch must be >= 1");
@@ -54,6 +60,10 @@ int make_batch(const pf_plan* p, const pf_batch_desc* d, pf_batch* b) {
     b->nt = has_b0 ? d->nt : 1;
     b->nfg = b->basis ? 1 : p->nframes;
     b->nimg = b->basis ? b->K : p->nframes;
+    if (dir == DIR_FWD && pf_gather_col_layout(p, *b)) {
+        b->Cv = 2;
+        b->Cb = b->C / 2;
+    }
     return PF_OK;
 }
 
@@ -247,11 +257,12 @@ int pf_plan_get_permutation(const pf_plan* p, int* perm, void* stream) {
 }
 
 int pf_describe_launch(const pf_plan* p, const pf_batch_desc* d, char* buf, size_t cap) {
-    pf_batch b;
-    int rc = make_batch(p, d, &b);
+    pf_batch bf, ba;
+    int rc = make_batch(p, d, &bf, DIR_FWD);
+    if (rc == PF_OK) rc = make_batch(p, d, &ba);
     if (rc != PF_OK) return rc;
     if (!buf || cap == 0) return pf_fail(PF_ERR_ARG, "null output buffer");
-    const std::string text = "gather=" + pf_gather_name(p, b) + " spread=" + pf_spread_name(p, b) +
+    const std::string text = "gather=" + pf_gather_name(p, bf) + " spread=" + pf_spread_name(p, ba) +
                              " fft=" + (pf_fftk_supported(p) ? "fused" : "rocfft");
     const size_t n = std::min(cap - 1, text.size());
     std::copy(text.begin(), text.begin() + n, buf);
@@ -279,7 +290,7 @@ int pf_spread_workspace_size(const pf_plan* p, const pf_batch_desc* d, size_t* b
 
 int pf_workspace_size(const pf_plan* p, const pf_batch_desc* d, int op, size_t* bytes) {
     pf_batch b;
-    int rc = make_batch(p, d, &b);
+    int rc = make_batch(p, d, &b, op == 0 ? DIR_FWD : DIR_ANY);
     if (rc != PF_OK) return rc;
     if (!bytes) return pf_fail(PF_ERR_ARG, "null output");
     if (op != 0 && op != 1) return pf_fail(PF_ERR_ARG, "op must be 0 (forward) or 1 (adjoint)");
@@ -290,7 +301,7 @@ int pf_workspace_size(const pf_plan* p, const pf_batch_desc* d, int op, size_t* 
 
 int pf_deapod_pad(const pf_plan* p, const pf_batch_desc* d, const void* image, void* grid, void* stream) {
     pf_batch b;
-    int rc = make_batch(p, d, &b);
+    int rc = make_batch(p, d, &b, DIR_FWD);
     if (rc != PF_OK) return rc;
     if (!image || !grid) return pf_fail(PF_ERR_ARG, "null array");
     return pf_launch_deapod_pad(p, b, static_cast<const float2*>(image), static_cast<float2*>(grid),
@@ -299,7 +310,7 @@ int pf_deapod_pad(const pf_plan* p, const pf_batch_desc* d, const void* image, v
 
 int pf_fft(pf_plan* p, const pf_batch_desc* d, void* grid, int inverse, void* stream) {
     pf_batch b;
-    int rc = make_batch(p, d, &b);
+    int rc = make_batch(p, d, &b, inverse ? DIR_ANY : DIR_FWD);
     if (rc != PF_OK) return rc;
     if (!grid) return pf_fail(PF_ERR_ARG, "null array");
     return pf_fft_exec(p, b, static_cast<float2*>(grid), inverse ? 1 : 0, static_cast<hipStream_t>(stream));
@@ -307,7 +318,7 @@ int pf_fft(pf_plan* p, const pf_batch_desc* d, void* grid, int inverse, void* st
 
 int pf_interp(const pf_plan* p, const pf_batch_desc* d, const void* grid, void* ksp_sorted, void* stream) {
     pf_batch b;
-    int rc = make_batch(p, d, &b);
+    int rc = make_batch(p, d, &b, DIR_FWD);
     if (rc != PF_OK) return rc;
     if (!grid || !ksp_sorted) return pf_fail(PF_ERR_ARG, "null array");
     return pf_launch_gather(p, b, static_cast<const float2*>(grid), static_cast<float2*>(ksp_sorted),
@@ -371,7 +382,7 @@ int pf_workspace_offsets(const pf_plan* p, const pf_batch_desc* d, size_t* grid_
 int pf_interp_grad(const pf_plan* p, const pf_batch_desc* d, const void* grid, const void* h_sorted,
                    const float* axis_scale, int accumulate, float* dcoord, void* stream) {
     pf_batch b;
-    int rc = make_batch(p, d, &b);
+    int rc = make_batch(p, d, &b, DIR_FWD);
     if (rc != PF_OK) return rc;
     if (!grid || !h_sorted || !dcoord) return pf_fail(PF_ERR_ARG, "null array");
     float sc[3] = {1.f, 1.f, 1.f};
@@ -384,7 +395,7 @@ int pf_interp_grad(const pf_plan* p, const pf_batch_desc* d, const void* grid, c
 int pf_pad_fft(pf_plan* p, const pf_batch_desc* d, const void* image, void* grid, void* fft_workspace,
                void* stream) {
     pf_batch b;
-    int rc = make_batch(p, d, &b);
+    int rc = make_batch(p, d, &b, DIR_FWD);
     if (rc != PF_OK) return rc;
     if (!image || !grid) return pf_fail(PF_ERR_ARG, "null array");
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -417,7 +428,7 @@ int pf_ifft_crop(pf_plan* p, const pf_batch_desc* d, void* grid, void* image, fl
 int pf_nufft(pf_plan* p, const pf_batch_desc* d, const void* image, void* kspace, const float* weight, int norm,
              void* workspace, size_t workspace_bytes, void* stream) {
     pf_batch b;
-    int rc = make_batch(p, d, &b);
+    int rc = make_batch(p, d, &b, DIR_FWD);
     if (rc != PF_OK) return rc;
     if (!image || !kspace) return pf_fail(PF_ERR_ARG, "null array");
     if (norm < PF_NORM_NONE || norm > PF_NORM_ADJOINT) return pf_fail(PF_ERR_ARG, "norm must be 0, 1 or 2");

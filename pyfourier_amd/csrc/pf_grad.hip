@@ -15,6 +15,7 @@
 // (reference defect A.14: no -2 pi i / N factor, invalid reshape).
 #include <algorithm>
 
+#include "pf_gcol.h"
 #include "pf_internal.h"
 
 namespace {
@@ -346,6 +347,140 @@ const GradLaunch DCOORD_W_DEEP[2][PF_WMAX] = {
      &launch_dcoord_w<2, 8, 8, true>},
 };
 
+
+// LDS-staged derivative gather for the batches the forward gather stages (3-D,
+// one width W = 3..8 on all axes, plain values, coil-pair-major grid: Cv = 2,
+// Cb = C / 2).  Same work items, lane order and window staging as the forward
+// LDS gather: one block per (frame, 16 x 16 column, DZ z planes) window, one
+// lane per sample, the window of each coil pair copied to LDS and read from
+// there.  Per (z, y) row a lane forms the x-weighted and x-derivative-weighted
+// sums of its W taps and folds them into the three derivatives; Re(conj(h) D)
+// accumulates over the coil pairs in registers, so each sample is written once
+// and no cross-lane reduction is needed.
+template <int W>
+__global__ void __launch_bounds__(GCOL_TB, 2)
+    k_gather_dcoord_col(GradParams P, const int4* __restrict__ items, const int32_t* __restrict__ gids,
+                        const float* __restrict__ taprec, const float* __restrict__ cos_sorted,
+                        const int32_t* __restrict__ perm, const float2* __restrict__ grid,
+                        const float2* __restrict__ h, float* __restrict__ dout, int ncy, int ncx, int nzw) {
+    using S = GColShape<W>;
+    __shared__ float4 win[S::NPTS];
+    const unsigned bid = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int4 it = items[bid];
+    const int zw = it.x % nzw;
+    const int cx = (it.x / nzw) % ncx;
+    const int cy = (it.x / (nzw * ncx)) % ncy;
+    const int f = it.x / (nzw * ncx * ncy);
+    const int z0 = zw * S::DZ, y0 = cy * PF_GCOL, x0 = cx * PF_GCOL;
+    const int ncp = P.C / 2;
+    const float4* pairs = reinterpret_cast<const float4*>(grid) + (int64_t)(P.nfg > 1 ? f : 0) * P.Cb * P.gtot;
+    const int tid = threadIdx.x, wave = tid >> 6;
+
+    int goff[S::NK];
+#pragma unroll
+    for (int k = 0; k < S::NK; ++k) {
+        const int p = k * GCOL_TB + tid;
+        const int pl = p / S::PP, rem = p - pl * S::PP;
+        const int py = rem / S::PX, px = rem - py * S::PX;
+        const int z = (z0 + pl) % P.os[0], y = (y0 + py) % P.os[1], x = (x0 + px) % P.os[2];
+        goff[k] = p < S::NPTS ? (z * P.os[1] + y) * P.os[2] + x : -1;
+    }
+
+    const int t0 = gids[it.y + tid];
+    const bool act = t0 >= 0;
+    const int t = act ? t0 : 0;
+    int base = 0;
+    float w[3][W], dw[3][W];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < W; ++k) w[a][k] = dw[a][k] = 0.f;
+    if (act) {
+        const float* rec = taprec + (int64_t)t * P.rec;
+        const int* ri = reinterpret_cast<const int*>(rec);
+        base = ((ri[0] - z0) * S::PX + (ri[1] - y0)) * S::PX + (ri[2] - x0);
+        constexpr double half = 0.5 * W;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double cd = (double)cos_sorted[(int64_t)t * 3 + a];
+            const double x0d = ceil(cd - half);
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                w[a][k] = rec[P.woff[a] + k];
+                dw[a][k] = (float)(-kb_deriv((x0d + k - cd) / half, (double)P.beta[a]) / half);
+            }
+        }
+    }
+    const float4* hp = reinterpret_cast<const float4*>(h + (int64_t)t * P.C);
+    float part[3] = {0.f, 0.f, 0.f};
+
+    for (int cp = 0; cp < ncp; ++cp) {
+        const float4* pair = pairs + (int64_t)cp * P.gtot;
+#pragma unroll
+        for (int k = 0; k < S::NK; ++k)
+            if (goff[k] >= 0) gcol_copy16(pair + goff[k], win + k * GCOL_TB + wave * 64);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (act) {
+            float4 D[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) D[a] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int a = 0; a < W; ++a)
+#pragma unroll
+                for (int b = 0; b < W; ++b) {
+                    const float4* row = win + base + a * S::PP + b * S::PX;
+                    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+#pragma unroll
+                    for (int c = 0; c < W; ++c) {
+                        const float4 v = row[c];
+                        s0.x += w[2][c] * v.x;
+                        s0.y += w[2][c] * v.y;
+                        s0.z += w[2][c] * v.z;
+                        s0.w += w[2][c] * v.w;
+                        s1.x += dw[2][c] * v.x;
+                        s1.y += dw[2][c] * v.y;
+                        s1.z += dw[2][c] * v.z;
+                        s1.w += dw[2][c] * v.w;
+                    }
+                    const float dzy = dw[0][a] * w[1][b], wzdy = w[0][a] * dw[1][b], wzy = w[0][a] * w[1][b];
+                    D[0].x += dzy * s0.x;
+                    D[0].y += dzy * s0.y;
+                    D[0].z += dzy * s0.z;
+                    D[0].w += dzy * s0.w;
+                    D[1].x += wzdy * s0.x;
+                    D[1].y += wzdy * s0.y;
+                    D[1].z += wzdy * s0.z;
+                    D[1].w += wzdy * s0.w;
+                    D[2].x += wzy * s1.x;
+                    D[2].y += wzy * s1.y;
+                    D[2].z += wzy * s1.z;
+                    D[2].w += wzy * s1.w;
+                }
+            const float4 hv = hp[cp];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                part[a] += hv.x * D[a].x + hv.y * D[a].y + hv.z * D[a].z + hv.w * D[a].w;  // Re(conj(h) D)
+        }
+        __syncthreads();  // the next pair's copies overwrite the window
+    }
+    if (act) write_dcoord(P, perm, t, part, dout);
+}
+
+template <int W>
+void launch_dcoord_col(hipStream_t s, const GradParams& P, const pf_plan* p, const float2* g, const float2* h,
+                       float* out) {
+    hipLaunchKernelGGL((k_gather_dcoord_col<W>), dim3((unsigned)p->gl.nitems), dim3(GCOL_TB), 0, s, P, p->gl.items,
+                       p->gl.ids, p->taprec, p->cos_sorted, p->perm, g, h, out, p->gl.ncol[0], p->gl.ncol[1],
+                       p->gl.nzw);
+}
+
+using DcoordColLaunch = void (*)(hipStream_t, const GradParams&, const pf_plan*, const float2*, const float2*, float*);
+
+// [W - 3], W = 3 .. 8
+const DcoordColLaunch DCOORD_COL[6] = {&launch_dcoord_col<3>, &launch_dcoord_col<4>, &launch_dcoord_col<5>,
+                                       &launch_dcoord_col<6>, &launch_dcoord_col<7>, &launch_dcoord_col<8>};
+
 }  // namespace
 
 int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* grid, const float2* h, const float* scale,
@@ -374,6 +509,11 @@ int pf_launch_gather_dcoord(const pf_plan* p, const pf_batch& b, const float2* g
     P.ndim = p->ndim;
     P.accumulate = accumulate ? 1 : 0;
     for (int a = 0; a < 3; ++a) P.scale[a] = scale[a];
+    if (b.Cv == 2 && pf_gather_col_layout(p, b)) {
+        DCOORD_COL[p->width[2] - 3](s, P, p, grid, h, out);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
     const int cpl = (b.Cv % 2 == 0 && b.C >= 2) ? 2 : 1;
     const int wy = p->width[1], wx = p->width[2];
     const bool idx32 = (int64_t)pf_gtot(p) * b.R * b.Cv < (1ll << 31);

@@ -61,6 +61,34 @@ static inline const pf_sitems& pf_sitems_for(const pf_slist& L, int nvch) {
     return (int64_t)L.split[0].nsitems * nvch >= PF_SPREAD_FILL ? L.split[0] : L.split[1];
 }
 
+// Gather column lists (3-D plans with one kernel width on all axes): the
+// sorted samples regrouped by window key = (frame, y column, x column, z
+// window) of their first tap, where a column is PF_GCOL x PF_GCOL grid points
+// in (y, x) and a z window pf_gather_dz(W) planes; every key's grid block
+// (plus the kernel halo) is staged in LDS once per coil pair and all its
+// samples read their taps from there.
+#ifndef PF_GCOL
+#define PF_GCOL 16
+#endif
+#ifndef PF_GCOL_MAXS
+#define PF_GCOL_MAXS 256  // lanes per work item (one sample each, one block)
+#endif
+#ifndef PF_GCOL_DZ6
+#define PF_GCOL_DZ6 5
+#endif
+// z planes per window: (DZ + W - 1) * (PF_GCOL + W - 1)^2 points of 16 bytes
+// stay near 70 KB (two blocks per CU)
+__host__ __device__ constexpr int pf_gather_dz(int W) {
+    return W == 6 ? PF_GCOL_DZ6 : (W <= 6 ? 11 - W : (W == 7 ? 3 : 2));
+}
+struct pf_glist {
+    int32_t* ids = nullptr;  // lane slots: sorted-sample index t or -1, PF_GCOL_MAXS per item
+    int4* items = nullptr;   // (key, first lane slot, 0, 0)
+    int nitems = 0;
+    int ncol[2] = {0, 0};    // columns along y, x
+    int nzw = 0;             // z windows
+};
+
 struct pf_plan {
     int ndim = 0;          // spatial dims 1..3
     int shape[3];          // image matrix (z, y, x), leading 1s for d<3
@@ -91,6 +119,7 @@ struct pf_plan {
     // identical (aliased) when nframes == 1
     pf_slist sl[2];
     int max_chunk = 0;    // spread entries per work item (0: automatic)
+    pf_glist gl;          // LDS-staged gather lists (empty when not built)
 
     pf_fft_cache* fft = nullptr;   // rocFFT plans (grid sizes with other prime factors)
     struct pf_fftk* fftk = nullptr; // fused pruned FFT (sizes 2^a 3^b 5^c)
@@ -147,6 +176,9 @@ int pf_spread_coil_chunk(const pf_plan* p, const pf_batch& b);
 std::string pf_gather_name(const pf_plan* p, const pf_batch& b);
 std::string pf_spread_name(const pf_plan* p, const pf_batch& b);
 int pf_ops_prepare();  // kernel attributes, set at plan creation
+// the LDS-staged gather applies to this batch (its forward grid is then laid
+// out with Cv = 2 coil pairs, Cb = C / 2)
+bool pf_gather_col_layout(const pf_plan* p, const pf_batch& b);
 int pf_noalloc_prepare();  // this device's pf_nufft_noalloc scratch address, at plan creation
 
 // fused deapodise/pad/FFT and inverse FFT/crop/deapodise (pf_fftk.hip)

@@ -232,6 +232,23 @@ __global__ void k_bounds(const uint32_t* __restrict__ keys, int64_t n, int32_t* 
     if (e == n - 1 || keys[e + 1] != k) end[k] = (int32_t)(e + 1);
 }
 
+// Window key of every sorted sample for the LDS-staged gather: (frame,
+// column of its first y tap, column of its first x tap, window of its first z
+// tap), and the LDS index of its first tap inside that window (the window is
+// (DZ + W - 1) planes of (PF_GCOL + W - 1)^2 points); the values are the
+// sorted indices themselves.
+__global__ void k_gather_keys(const float* __restrict__ rec, TapParams P, int ncy, int ncx, int nzw, int dz,
+                              uint32_t* __restrict__ keys, int32_t* __restrict__ vals, int32_t* __restrict__ base) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.mtot) return;
+    const int* ri = reinterpret_cast<const int*>(rec + t * P.rec);
+    const int64_t f = ri[3];
+    keys[t] = (uint32_t)((((f * ncy) + ri[1] / PF_GCOL) * ncx + ri[2] / PF_GCOL) * nzw + ri[0] / dz);
+    vals[t] = (int32_t)t;
+    const int px = PF_GCOL + P.width[2] - 1;
+    base[t] = ((ri[0] % dz) * px + ri[1] % PF_GCOL) * px + ri[2] % PF_GCOL;
+}
+
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() {
@@ -461,6 +478,8 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
     return PF_OK;
 }
 
+static int build_glist(pf_plan* p, const TapParams& P, hipStream_t stream);
+
 int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
     TapParams P;
     P.ndim = p->ndim;
@@ -529,14 +548,126 @@ int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
     PF_HIP(hipStreamSynchronize(stream));
     int rc = build_slist(p, P, 0, stream, p->sl[0]);
     if (rc != PF_OK) return rc;
+    if ((rc = build_glist(p, P, stream)) != PF_OK) return rc;
     if (p->nframes > 1) return build_slist(p, P, 1, stream, p->sl[1]);
     p->sl[1] = p->sl[0];
     return PF_OK;
 }
 
+// Lane order of one window's samples.  One LDS pass serves 16 lanes of
+// 16-byte reads (64 banks of 4 bytes); it is free of bank conflicts when the
+// lanes' first taps are distinct modulo 16 LDS slots, or identical (a
+// broadcast), since every tap adds the same offset to all of them.  The
+// samples are dealt in sweeps over the residue classes mod 16 (fullest class
+// first), one first-tap position with all of its samples per class and
+// sweep, so consecutive lanes mostly fall in distinct classes; every lane is
+// filled (the gather's arithmetic, not the LDS, bounds it), and the caller
+// pads only the end of the window's last block.
+static void balance_lanes(const int32_t* ids, const int32_t* base, int n, std::vector<int32_t>& lanes) {
+    std::vector<std::pair<int32_t, int32_t>> pb(n);  // (first-tap position, sample)
+    for (int i = 0; i < n; ++i) pb[i] = {base[i], ids[i]};
+    std::stable_sort(pb.begin(), pb.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    struct Pos {
+        int first, count;
+    };
+    std::vector<Pos> bucket[16];
+    for (int i = 0; i < n;) {
+        int j = i;
+        while (j < n && pb[j].first == pb[i].first) ++j;
+        bucket[pb[i].first & 15].push_back({i, j - i});
+        i = j;
+    }
+    size_t head[16] = {0};
+    int order[16];
+    for (;;) {
+        for (int q = 0; q < 16; ++q) order[q] = q;
+        std::stable_sort(order, order + 16, [&](int a, int b) {
+            return bucket[a].size() - head[a] > bucket[b].size() - head[b];
+        });
+        if (head[order[0]] >= bucket[order[0]].size()) break;
+        for (int k = 0; k < 16; ++k) {
+            const int q = order[k];
+            if (head[q] >= bucket[q].size()) break;
+            const Pos& ps = bucket[q][head[q]++];
+            for (int m = 0; m < ps.count; ++m) lanes.push_back(pb[ps.first + m].second);
+        }
+    }
+}
+
+// Gather lists of the LDS-staged gather (see pf_glist); left empty (the
+// direct-read gather runs) for plans it does not cover.
+static int build_glist(pf_plan* p, const TapParams& P, hipStream_t stream) {
+    const int W = p->width[0];
+    if (p->ndim != 3 || W < 3 || p->width[1] != W || p->width[2] != W) return PF_OK;
+    const int dz = pf_gather_dz(W);
+    pf_glist& L = p->gl;
+    L.ncol[0] = (p->os[1] + PF_GCOL - 1) / PF_GCOL;
+    L.ncol[1] = (p->os[2] + PF_GCOL - 1) / PF_GCOL;
+    L.nzw = (p->os[0] + dz - 1) / dz;
+    const uint64_t nkeys = (uint64_t)p->nframes * L.ncol[0] * L.ncol[1] * L.nzw;
+    if (nkeys >= (1ull << 31)) return PF_OK;
+    const int64_t M = p->mtot;
+    const int TB = 256;
+    const unsigned nbits = bits_for(nkeys + 1);
+    DevBuf keys, keys2, vals, vals2, bvals, bvals2, tmp;
+    PF_HIP(keys.alloc(M * 4));
+    PF_HIP(keys2.alloc(M * 4));
+    PF_HIP(vals.alloc(M * 4));
+    PF_HIP(vals2.alloc(M * 4));
+    PF_HIP(bvals.alloc(M * 4));
+    PF_HIP(bvals2.alloc(M * 4));
+    hipLaunchKernelGGL(k_gather_keys, dim3(nblk(M, TB)), dim3(TB), 0, stream, p->taprec, P, L.ncol[0], L.ncol[1], L.nzw,
+                       dz, keys.as<uint32_t>(), vals.as<int32_t>(), bvals.as<int32_t>());
+    PF_HIP(hipGetLastError());
+    // sort (key, sample) pairs, then carry the first-tap positions along
+    // through the sorted sample ids (vals2[i] is the sorted index t)
+    size_t rbytes = 0;
+    PF_HIP(rocprim::radix_sort_pairs(nullptr, rbytes, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<int32_t>(),
+                                     vals2.as<int32_t>(), (size_t)M, 0, nbits, stream));
+    PF_HIP(tmp.alloc(rbytes));
+    PF_HIP(rocprim::radix_sort_pairs(tmp.p, rbytes, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<int32_t>(),
+                                     vals2.as<int32_t>(), (size_t)M, 0, nbits, stream));
+    std::vector<uint32_t> hk(M);
+    std::vector<int32_t> hid(M), hb(M);
+    PF_HIP(hipMemcpyAsync(hk.data(), keys2.p, M * 4, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipMemcpyAsync(hid.data(), vals2.p, M * 4, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipMemcpyAsync(hb.data(), bvals.p, M * 4, hipMemcpyDeviceToHost, stream));  // indexed by t
+    PF_HIP(hipStreamSynchronize(stream));
+    std::vector<int32_t> lanes, hbs;
+    std::vector<int4> items;
+    lanes.reserve((size_t)M + M / 4);
+    for (int64_t e0 = 0; e0 < M;) {
+        int64_t e1 = e0;
+        while (e1 < M && hk[e1] == hk[e0]) ++e1;
+        const int n = (int)(e1 - e0);
+        hbs.resize(n);
+        for (int i = 0; i < n; ++i) hbs[i] = hb[hid[e0 + i]];
+        const size_t l0 = lanes.size();
+        balance_lanes(hid.data() + e0, hbs.data(), n, lanes);
+        // items of PF_GCOL_MAXS lanes (whole groups); the last one padded
+        const size_t nl = lanes.size() - l0;
+        const size_t nit = (nl + PF_GCOL_MAXS - 1) / PF_GCOL_MAXS;
+        lanes.resize(l0 + nit * PF_GCOL_MAXS, -1);
+        for (size_t k = 0; k < nit; ++k)
+            items.push_back(make_int4((int)hk[e0], (int)(l0 + k * PF_GCOL_MAXS), 0, 0));
+        e0 = e1;
+    }
+    if ((int64_t)lanes.size() >= (1ll << 31)) return PF_OK;
+    int64_t placed = 0;
+    for (int32_t v : lanes) placed += v >= 0;
+    if (placed != M) return pf_fail(PF_ERR_ARG, "gather lane assignment lost samples (internal error)");
+    L.nitems = (int)items.size();
+    PF_HIP(hipMalloc(&L.ids, std::max<size_t>(1, lanes.size()) * sizeof(int32_t)));
+    PF_HIP(hipMemcpy(L.ids, lanes.data(), lanes.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    PF_HIP(hipMalloc(&L.items, std::max<size_t>(1, items.size()) * sizeof(int4)));
+    if (!items.empty())
+        PF_HIP(hipMemcpy(L.items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return PF_OK;
+}
+
 void pf_plan_free(pf_plan* p) {
     if (!p) return;
-    void* ptrs[] = {p->perm, p->iperm, p->taprec, p->cos_sorted, p->apod, p->foff};
+    void* ptrs[] = {p->perm, p->iperm, p->taprec, p->cos_sorted, p->apod, p->foff, p->gl.ids, p->gl.items};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     const bool alias = p->sl[1].slist == p->sl[0].slist;

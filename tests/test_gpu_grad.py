@@ -76,6 +76,34 @@ def test_adjoint_3d_ortho_weighted(cuda_device):
     assert rel(yt.grad.cpu().numpy().reshape(C, 1, -1), yk) < PARITY
 
 
+@pytest.mark.parametrize(
+    "width,N,C,F",
+    [
+        (3, 20, 4, 1),
+        (6, 24, 6, 1),
+        (6, 16, 4, 2),  # two frames: one grid per frame
+        (8, 16, 2, 1),
+    ],
+)
+def test_forward_3d_staged_window(cuda_device, width, N, C, F):
+    """3-D plans with one width on all axes and an even batch: the
+    derivative gather reads the coil-pair-major grid through the LDS-staged
+    window (k_gather_dcoord_col<W>), one lane per sample over all coil pairs."""
+    print(f"dcoord col: W={width} N={N} C={C} F={F}", flush=True)
+    rng = np.random.default_rng(40 + width + C)
+    base = demo.golden_means_radial_3d(60 * F, 32)
+    coord = np.stack([base[f::F] for f in range(F)]) if F > 1 else base
+    x = demo.complex_gaussian(rng, (C,) + ((F,) if F > 1 else ()) + (N, N, N))
+    v = demo.complex_gaussian(rng, (C,) + coord.shape[:-1])
+    ct = torch.from_numpy(coord).requires_grad_()
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), ct, width=width, oversamp=2.0, device=cuda_device)
+    y.backward(gradient=torch.from_numpy(v).to(cuda_device))
+    P = R.Plan(coord, N, width=width, oversamp=2.0)
+    xr = x.reshape((C, F, N, N, N))
+    want = R.coord_grad(xr, P, v.reshape(C, F, -1)) * _radian_units((N, N, N))
+    assert rel(ct.grad.numpy().reshape(-1, 3), want) < GRAD_TOL
+
+
 def test_subspace_multiframe_cycles(cuda_device):
     rng = np.random.default_rng(2)
     N, F, K, C = 32, 4, 2, 2
