@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--small", action="store_true", help="64^3 x 8 coils quick run")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched / autograd comparisons")
     ap.add_argument("--only", default=None, help="run one comparison of the extras by name and print its JSON")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
+                    help="write the HBM traffic file from two rocprofv3 --pmc runs of this script "
+                         "(FETCH_SIZE, WRITE_SIZE; csv output named run) and exit")
     return ap.parse_args()
 
 
@@ -246,6 +249,56 @@ def measured_traffic(stage, args):
             return None
         total += hits[0]["read_bytes"] + hits[0]["write_bytes"]
     return int(total)
+
+
+def _pmc_rows(d, counter):
+    """(kernel name, counter value) per dispatch of one rocprofv3 csv pass."""
+    import csv
+
+    path = os.path.join(d, "run_counter_collection.csv")
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                name = r["Kernel_Name"].replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0]
+                yield name.strip(), float(r["Counter_Value"])
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_bytes=2 << 30, N=128, C=32, M=8192 * 256, G=256 ** 3):
+    """Per-dispatch HBM bytes of every kernel of the headline run, from a
+    FETCH_SIZE and a WRITE_SIZE pass (KiB per dispatch), each scaled by what
+    the same counter reads for the 2 GiB device copy of stream_copy
+    (`__amd_rocclr_copyBuffer`: copy_bytes read and copy_bytes written)."""
+    import collections
+
+    per = {}
+    for key, d, counter in (("read", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        vals = collections.defaultdict(list)
+        for name, v in _pmc_rows(d, counter):
+            vals[name].append(v)
+        copy = [v for n, vs in vals.items() if "copyBuffer" in n for v in vs]
+        if not copy:
+            raise SystemExit(f"{d}: no device-copy dispatch to calibrate {counter} against")
+        kib = max(copy)  # the 2 GiB copy is the largest copyBuffer dispatch
+        per[key] = (copy_bytes / (kib * 1024.0), kib, vals)
+    alg = C * G * 8 + C * M * 8 + M * 16  # grid + samples + coordinates and weight
+    kernels = {}
+    for name in sorted(set(per["read"][2]) | set(per["write"][2])):
+        if "copyBuffer" in name:
+            continue
+        rv, wv = per["read"][2].get(name, []), per["write"][2].get(name, [])
+        kernels[name] = {
+            "read_bytes": int(round(np.mean(rv) * 1024 * per["read"][0])) if rv else None,
+            "write_bytes": int(round(np.mean(wv) * 1024 * per["write"][0])) if wv else None,
+            "dispatches": max(len(rv), len(wv)),
+            "algorithmic_bytes": alg if name.split("<")[0] in ("k_gather_col", "k_spread_pipe") else None,
+        }
+    out = {"calibration": {"fetch_factor": round(per["read"][0], 4), "write_factor": round(per["write"][0], 4),
+                           "copy_fetch_kib": per["read"][1], "copy_write_kib": per["write"][1]},
+           "kernels": kernels}
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    return out
 
 
 def _time(fn, dev, steps, warmup=2):
@@ -723,6 +776,10 @@ EXTRAS = (("cfg1_latency", bench_cfg1_latency), ("batched_uneven", bench_uneven_
 
 def main():
     args = parse()
+    if args.summarize_pmc:
+        out = summarize_pmc(*args.summarize_pmc)
+        log(f"wrote {TRAFFIC_FILE}: {len(out['kernels'])} kernels")
+        return
     if args.gpus != 1:
         print(
             f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",

@@ -8,6 +8,7 @@
 namespace {
 
 constexpr int GCOL_TB = PF_GCOL_MAXS;
+constexpr int GCOL_OUT = 4;  // coil pairs per output store of the forward gather
 
 template <int W>
 struct GColShape {

@@ -44,6 +44,8 @@ struct pf_sitems {
     int nslots = 0;
 };
 
+constexpr int PF_SLIST_PAD = 16;  // readable ids past the end of a spread list
+
 struct pf_slist {
     int32_t* slist = nullptr;    // sorted sample ids grouped by key
     int64_t nslist = 0;

@@ -455,6 +455,10 @@ __global__ void __launch_bounds__(GCOL_TB, 2)
     }
     float4* out = reinterpret_cast<float4*>(ys + (int64_t)t * B.C);
 
+    // the values of GCOL_OUT consecutive pairs are kept and stored together:
+    // one 64-byte piece of the sample's row instead of four 16-byte pieces
+    // written far apart in time (each a partial cache-line write to HBM)
+    float4 o[GCOL_OUT];
     for (int cp = 0; cp < ncp; ++cp) {
         // window -> LDS: lane i of wave w, copy k lands at point k*256 + w*64 + i
         const float4* pair = pairs + (int64_t)cp * g.gtot;
@@ -463,7 +467,18 @@ __global__ void __launch_bounds__(GCOL_TB, 2)
             if (goff[k] >= 0) gcol_copy16(pair + goff[k], win + k * GCOL_TB + wave * 64);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (act) out[cp] = gcol_taps<W>(win, base, wz, wy, wx);
+        if (act) {
+            const float4 r = gcol_taps<W>(win, base, wz, wy, wx);
+            const int q = cp % GCOL_OUT;
+#pragma unroll
+            for (int i = 0; i < GCOL_OUT; ++i)
+                if (i == q) o[i] = r;
+            if (q == GCOL_OUT - 1 || cp == ncp - 1) {
+#pragma unroll
+                for (int i = 0; i < GCOL_OUT; ++i)
+                    if (i <= q) out[cp - q + i] = o[i];
+            }
+        }
         __syncthreads();  // the next pair's copies overwrite the window
     }
 }
@@ -727,6 +742,9 @@ constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 #endif
 #ifndef PF_SPREAD_PIPE
 #define PF_SPREAD_PIPE 1  // plain 32-coil batches: software-pipelined copies
+#endif
+#ifndef PF_SPREAD_MFMA
+#define PF_SPREAD_MFMA 1  // plain 32-coil batches on 128-point 3-D tiles: matrix-core accumulation
 #endif
 #ifndef PF_SPREAD_WREG
 #define PF_SPREAD_WREG 1  // an entry's z/y weights in registers (measured faster)
@@ -1248,6 +1266,381 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1)
                     if (a < Tz && b < Ty && lx < Tx) dst[(int64_t)((a * Ty + b) * Tx + lx) * VL] = acc[a][b][k];
                 }
     }
+}
+
+// -------------------------------------------------------------------------
+// Matrix-core spread: the same items, entry batches, copies and summation
+// order over batches as k_spread_pipe, with the accumulation of a batch done
+// by MFMA.  For a batch of SPIPE_EB = 16 entries the tile update is a product
+//
+//     acc[p, n] += sum_k A[p, k] * B[k, n],   A[p, k] = wz * wy * wx of entry k
+//                                             at tile point p (0 outside its
+//                                             footprint), B[k, n] = re/im part
+//                                             n of entry k's 32 values,
+//
+// p over the tile's 128 points (4 row blocks of 32), n over 64 real columns
+// (2 blocks of 32): 8 v_mfma_f32_32x32x16_f16 per product.  FP32 operands are
+// split into fp16 pairs, x = hi + lo with |x - hi - lo| <= 2^-22 |x|, and
+// three products hi*hi + hi*lo + lo*hi accumulate in FP32 (the dropped lo*lo
+// is below 2^-22 relative), so the result matches FP32 arithmetic to rounding
+// level while the tile's arithmetic runs on the matrix cores instead of the
+// vector ALUs.  fp16 range: weights are normalised by the kernel's peak
+// value (w <= 1), values by a power of two per tile and value chunk, taken
+// from the largest |value| among the tile's samples (k_smax's per-sample
+// maxima) so that scaled values stay below 2^13; both are undone when the
+// tile is written.  Deterministic: a fixed instruction sequence per tile.
+// -------------------------------------------------------------------------
+// per sorted sample t and 32-value chunk v: max |re|, |im| over the chunk's
+// values, smax[t * nvch + v]; 16 lanes per (t, v), one 16-byte piece each
+__global__ void __launch_bounds__(TB) k_smax(int64_t mtot, int C, const float2* __restrict__ ys,
+                                             float* __restrict__ smax) {
+    const int nvch = C / SPREAD_VL;
+    const int64_t gid = (int64_t)blockIdx.x * TB + threadIdx.x;
+    const int64_t row = gid >> 4;  // (t, v) pair
+    const int piece = (int)(gid & 15);
+    float m = 0.f;
+    if (row < mtot * nvch) {
+        const int64_t t = row / nvch;
+        const int v = (int)(row - t * nvch);
+        const float4 q = reinterpret_cast<const float4*>(ys + t * C + (int64_t)v * SPREAD_VL)[piece];
+        m = fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)), fmaxf(fabsf(q.z), fabsf(q.w)));
+    }
+    int x = __float_as_int(m);
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    if (piece == 15 && row < mtot * nvch) smax[row] = __int_as_float(x);
+}
+
+typedef _Float16 pf_half2 __attribute__((ext_vector_type(2)));
+typedef _Float16 pf_half8 __attribute__((ext_vector_type(8)));
+typedef float pf_f2 __attribute__((ext_vector_type(2)));
+typedef float pf_f16 __attribute__((ext_vector_type(16)));
+
+// (a, b) -> fp16 pairs hi = rn(x), lo = rn(x - hi)
+__device__ __forceinline__ void smfma_split(float a, float b, pf_half2& hi, pf_half2& lo) {
+    const pf_f2 x = {a, b};
+    hi = __builtin_convertvector(x, pf_half2);
+    lo = __builtin_convertvector(x - __builtin_convertvector(hi, pf_f2), pf_half2);
+}
+
+__device__ __forceinline__ pf_half8 smfma_cat(pf_half2 a, pf_half2 b, pf_half2 c, pf_half2 d) {
+    return __builtin_shufflevector(__builtin_shufflevector(a, b, 0, 1, 2, 3), __builtin_shufflevector(c, d, 0, 1, 2, 3),
+                                   0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+#ifndef PF_SMFMA_OCC
+#define PF_SMFMA_OCC 2  // waves per SIMD the register budget is sized for
+#endif
+constexpr int SMFMA_NCOPY = SPIPE_EB * SPIPE_RP / 64 + SPIPE_EB / 4;  // copy instructions per batch
+
+// max over the wave of non-negative floats (integer order of their bits),
+// by DPP row shifts and row broadcasts; the result is read from lane 63
+__device__ __forceinline__ float smfma_wave_max(float v) {
+    int x = __float_as_int(v);
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+
+// copies of one batch into one record slot and one value slot: always
+// SMFMA_NCOPY copy instructions (entries past n repeat entry 0), so the wait
+// for a batch can count the copies issued after it; lane j < 16 of sid
+// holds entry j's sample id
+__device__ __forceinline__ void smfma_issue(float4 (&s_rec)[SPIPE_EB][SPIPE_RP], float2 (&val)[SPIPE_EB][SPREAD_VL],
+                                            int sid, int n, int rec4, const float* __restrict__ taprec,
+                                            const char* __restrict__ ybase, int C, int vch) {
+    const int lane = threadIdx.x;
+    const int c = lane & (SPIPE_RP - 1);
+    const int cs = c < rec4 ? c : 0;
+#pragma unroll
+    for (int k = 0; k < SPIPE_EB * SPIPE_RP / 64; ++k) {
+        const int e = k * (64 / SPIPE_RP) + (lane >> 3);
+        const int se = __shfl(sid, e < n ? e : 0);
+        const float* src = taprec + (int64_t)se * (rec4 * 4) + 4 * cs;
+        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_rec[k * (64 / SPIPE_RP)][0],
+                                         16, 0, 0);
+    }
+    const int sub = lane >> 4, p16 = lane & 15;
+#pragma unroll
+    for (int q = 0; q < SPIPE_EB / 4; ++q) {
+        const int j = q * 4 + sub;
+        const unsigned sj = (unsigned)__shfl(sid, j < n ? j : 0);
+        const unsigned off = (sj * (unsigned)C + (unsigned)(vch * SPREAD_VL)) * 8u + (unsigned)p16 * 16u;
+        __builtin_amdgcn_global_load_lds(ybase + off, (__attribute__((address_space(3))) void*)&val[q * 4][0], 16, 0,
+                                         0);
+    }
+}
+
+// the batch's weight rows: entry e's weights at this tile's TZ z, TY y and TX
+// x positions (0 outside the footprint; the x weights times wnorm), rows of
+// entries n .. 15 zero.  Four lanes per entry, one 16-byte piece of the row
+// each; every record read is issued before any select or store (a position
+// outside the footprint reads the entry's first weight of that axis and
+// drops it).
+template <int TZ, int TY, int TX>
+__device__ __forceinline__ void smfma_weights(const float4 (&s_rec)[SPIPE_EB][SPIPE_RP], int n, const GeoParams& g,
+                                              int oz, int oy, int ox, float wnorm,
+                                              float (&s_w)[SPIPE_EB][TZ + TY + TX]) {
+    constexpr int NW = TZ + TY + TX;
+    static_assert(NW == 16 && TZ % 4 == 0 && TY % 4 == 0 && TX % 4 == 0, "one axis per 16-byte row piece");
+    const int lane = threadIdx.x;
+    const int e = lane >> 2, k0 = 4 * (lane & 3);  // entry, first slot of the piece
+    const int a = k0 < TZ ? 0 : (k0 < TZ + TY ? 1 : 2);
+    const int p0 = k0 - (a == 0 ? 0 : (a == 1 ? TZ : TZ + TY));  // tile position of the first slot
+    // selected from values held in scalar registers (a per-lane index into
+    // the parameters, or a select the compiler turns into one, would read
+    // the argument block through vector memory)
+    int os = __builtin_amdgcn_readfirstlane(g.os[2]), W = __builtin_amdgcn_readfirstlane(g.width[2]);
+    int wo = __builtin_amdgcn_readfirstlane(g.woff[2]);
+    if (a == 1) {
+        os = __builtin_amdgcn_readfirstlane(g.os[1]);
+        W = __builtin_amdgcn_readfirstlane(g.width[1]);
+        wo = __builtin_amdgcn_readfirstlane(g.woff[1]);
+    }
+    if (a == 0) {
+        os = __builtin_amdgcn_readfirstlane(g.os[0]);
+        W = __builtin_amdgcn_readfirstlane(g.width[0]);
+        wo = __builtin_amdgcn_readfirstlane(g.woff[0]);
+    }
+    const int o = a == 0 ? oz : (a == 1 ? oy : ox);
+    const float* rec_l = reinterpret_cast<const float*>(&s_rec[e][0]);
+    const int* ri = reinterpret_cast<const int*>(rec_l);
+    int base = o - (a == 0 ? ri[0] : (a == 1 ? ri[1] : ri[2]));
+    base += base < 0 ? os : 0;
+    const bool live = e < n;
+    float wv[4];
+    bool in[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int i = base + p0 + k;
+        i -= i >= os ? os : 0;
+        in[k] = live && i < W;
+        wv[k] = rec_l[wo + (in[k] ? i : 0)];
+    }
+    const float sc = a == 2 ? wnorm : 1.0f;
+    reinterpret_cast<float4*>(s_w[e])[lane & 3] =
+        make_float4(in[0] ? wv[0] * sc : 0.f, in[1] ? wv[1] * sc : 0.f, in[2] ? wv[2] * sc : 0.f,
+                    in[3] ? wv[3] * sc : 0.f);
+}
+
+// tile TZ x TY x TX = 128 points; row block mb holds z rows mb * ZR ..
+// (ZR = 32 / (TY * TX)), row i of a block is (z - mb * ZR, y, x) row-major.
+// Copies run two batches ahead: records in two slots, values in three.
+template <int TZ, int TY, int TX>
+__global__ void __launch_bounds__(64, PF_SMFMA_OCC)
+    k_spread_mfma(GeoParams g, BatchParams B, const int4* __restrict__ items, const int32_t* __restrict__ slist,
+                  const float* __restrict__ taprec, const float2* __restrict__ ys, const float* __restrict__ smax,
+                  float2* __restrict__ grid, float2* __restrict__ partial, float wnorm, float wunnorm) {
+    static_assert(TZ * TY * TX == 128 && 32 % (TY * TX) == 0 && TX % 2 == 0, "128-point tile, whole z rows per block");
+    constexpr int VL = SPREAD_VL;
+    constexpr int EB = SPIPE_EB;
+    constexpr int ZR = 32 / (TY * TX);
+    static_assert(EB == 16, "one MFMA K block per entry batch");
+    __shared__ __attribute__((aligned(16))) float4 s_rec0[EB][SPIPE_RP];
+    __shared__ __attribute__((aligned(16))) float4 s_rec1[EB][SPIPE_RP];
+    __shared__ __attribute__((aligned(16))) float2 s_val0[EB][VL];
+    __shared__ __attribute__((aligned(16))) float2 s_val1[EB][VL];
+    __shared__ __attribute__((aligned(16))) float2 s_val2[EB][VL];
+    __shared__ __attribute__((aligned(16))) float s_w[EB][TZ + TY + TX];  // 16-byte aligned rows
+
+    const int nvch = B.C / VL;
+    const unsigned bid = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int4 it = items[bid / nvch];
+    const int vch = bid % nvch;
+    const int lane = threadIdx.x;
+    const int h = lane >> 5;     // K half: entries 8h .. 8h + 7
+    const int col = lane & 31;   // A row / B and D column
+    // this lane's A row: tile point (zl, yl, xl) within a row block
+    const int zl = col / (TY * TX), yl = (col / TX) % TY, xl = col % TX;
+
+    const int64_t key = it.x;
+    const int fg_frame = (int)(key / g.ntile_tot);
+    const int64_t tile = key % g.ntile_tot;
+    const int tix = (int)(tile % g.ntiles[2]);
+    const int tiy = (int)((tile / g.ntiles[2]) % g.ntiles[1]);
+    const int tiz = (int)(tile / ((int64_t)g.ntiles[2] * g.ntiles[1]));
+    const int oz = tiz * g.tile[0], oy = tiy * g.tile[1], ox = tix * g.tile[2];
+    const int ez = min(g.tile[0], g.os[0] - oz), ey = min(g.tile[1], g.os[1] - oy);
+    const int ex = min(g.tile[2], g.os[2] - ox);
+    const int rec4 = g.rec / 4;
+    const char* ybase = reinterpret_cast<const char*>(ys);
+
+    pf_f16 acc[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[mb][nb][r] = 0.f;
+    int E = 0;  // the tile's values are scaled by 2^-E
+
+    // one batch (values in val, weights in s_w) into the accumulators
+    auto accumulate = [&](const float2 (&val)[EB][VL]) {
+        // B operand: entry 8h + j, column col of value block nb (value
+        // nb * 16 + col / 2, real or imaginary part)
+        float bv[2][8];
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bv[nb][j] = reinterpret_cast<const float*>(&val[8 * h + j][nb * 16])[col];
+        pf_half8 bhi[2], blo[2];
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            pf_half2 hi[4], lo[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                smfma_split(ldexpf(bv[nb][2 * q], -E), ldexpf(bv[nb][2 * q + 1], -E), hi[q], lo[q]);
+            bhi[nb] = smfma_cat(hi[0], hi[1], hi[2], hi[3]);
+            blo[nb] = smfma_cat(lo[0], lo[1], lo[2], lo[3]);
+        }
+        // A operand, one row block at a time (its operands live only for its
+        // 6 products): entry 8h + j at this lane's point of row block mb
+        float wyx[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) wyx[j] = s_w[8 * h + j][TZ + yl] * s_w[8 * h + j][TZ + TY + xl];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            pf_half2 hi[4], lo[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                smfma_split(s_w[8 * h + 2 * q][mb * ZR + zl] * wyx[2 * q],
+                            s_w[8 * h + 2 * q + 1][mb * ZR + zl] * wyx[2 * q + 1], hi[q], lo[q]);
+            const pf_half8 ahi = smfma_cat(hi[0], hi[1], hi[2], hi[3]);
+            const pf_half8 alo = smfma_cat(lo[0], lo[1], lo[2], lo[3]);
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bhi[nb], acc[mb][nb], 0, 0, 0);
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, blo[nb], acc[mb][nb], 0, 0, 0);
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bhi[nb], acc[mb][nb], 0, 0, 0);
+        }
+    };
+
+    // batch b covers entries [b * EB, min(cnt, b * EB + EB)); its records go
+    // to record slot b % 2, its values to value slot b % 3.  The sample ids
+    // come through scalar loads, so the loop's only vector memory
+    // instructions are the copies, SMFMA_NCOPY per batch: when batch b is
+    // due, exactly the copies of batch b + 1 (if it exists) were issued
+    // after its own.
+    const int cnt = it.z - it.y;
+    if (cnt > 0) {
+        const int nbt = (cnt + EB - 1) / EB;
+        const int32_t* sl = slist + it.y;
+        // batch b's sample ids, lane j < 16 holding entry j's: one scalar
+        // load of 16 ids (batches past the last reread the last one; entries
+        // past the item's end are read from the next item or the list's
+        // padding and never used)
+        auto ids = [&](int b) {
+            const int32_t* bp = sl + (int64_t)min(b, nbt - 1) * EB;
+            int v = 0;
+#pragma unroll
+            for (int j = 0; j < EB; ++j) v = lane == j ? bp[j] : v;
+            return v;
+        };
+        // every slot is named statically at each use, so the compiler can
+        // tell the copies into one slot from reads of another and waits only
+        // where this code does
+        auto issue = [&](int b, int sid, float4 (&rec)[EB][SPIPE_RP], float2 (&val)[EB][VL]) {
+            smfma_issue(rec, val, sid, min(EB, cnt - b * EB), rec4, taprec, ybase, B.C, vch);
+        };
+        // batch b: records in rec (slot b % 2, which batch b + 2's records
+        // then reuse), values in cur (slot b % 3); batch b + 2's values go to
+        // nxt (slot (b + 2) % 3).  One wave per block: no block barrier (it
+        // would also wait for the copies in flight), the compiler fences keep
+        // LDS reads after the waits.
+        auto step = [&](int b, int& sid, float4 (&rec)[EB][SPIPE_RP], float2 (&nxt)[EB][VL],
+                        const float2 (&cur)[EB][VL]) {
+            if (b + 1 < nbt)
+                __builtin_amdgcn_s_waitcnt(0xF70 | SMFMA_NCOPY);  // vmcnt(NCOPY); lgkm/exp not waited on
+            else
+                __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+            asm volatile("" ::: "memory");
+            smfma_weights<TZ, TY, TX>(rec, min(EB, cnt - b * EB), g, oz, oy, ox, wnorm, s_w);
+            asm volatile("" ::: "memory");
+            if (b + 2 < nbt) issue(b + 2, sid, rec, nxt);
+            sid = ids(b + 4);
+            accumulate(cur);
+        };
+        int sE = ids(0), sO = ids(1);  // ids of the next even / odd batch to issue
+        // the tile's largest |value| (per-sample maxima of this value
+        // chunk): its loads run under the first two batches' copies
+        float mx = 0.f;
+        for (int e = lane; e < cnt; e += 64) mx = fmaxf(mx, smax[(int64_t)sl[e] * nvch + vch]);
+        issue(0, sE, s_rec0, s_val0);
+        sE = ids(2);
+        if (nbt > 1) issue(1, sO, s_rec1, s_val1);
+        sO = ids(3);
+        mx = smfma_wave_max(mx);
+        frexpf(mx, &E);  // mx < 2^E
+        E -= 13;         // scaled values below 2^13 (fp16 holds up to 65504)
+        // six steps per pass (slots repeat every 6 batches), one loop exit
+        for (int b = 0; b < nbt; b += 6) {
+            step(b, sE, s_rec0, s_val2, s_val0);
+            if (b + 1 < nbt) step(b + 1, sO, s_rec1, s_val0, s_val1);
+            if (b + 2 < nbt) step(b + 2, sE, s_rec0, s_val1, s_val2);
+            if (b + 3 < nbt) step(b + 3, sO, s_rec1, s_val2, s_val0);
+            if (b + 4 < nbt) step(b + 4, sE, s_rec0, s_val0, s_val1);
+            if (b + 5 < nbt) step(b + 5, sO, s_rec1, s_val1, s_val2);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // no copy outlives the wave's LDS
+
+    // accumulator element r of block (mb, nb) is tile row i = 8 (r / 4) +
+    // 4 h + r % 4 of block mb (z = mb * ZR + i / (TY * TX), y = (i / TX) % TY,
+    // x = i % TX), real column col of block nb.  Rows q = r / 4 hold 4
+    // consecutive x of one (z, y) tile row.
+    const int E0 = E;
+    const int part = col & 1;  // 0: real, 1: imaginary
+    const bool to_grid = it.w < 0;
+    const int Tx = g.tile[2], Ty = g.tile[1], Tz = g.tile[0];
+    const int zmax = to_grid ? ez : Tz, ymax = to_grid ? ey : Ty, xmax = to_grid ? ex : Tx;
+    float* base[2];        // the lane's value at tile point (0, 0, 0)
+    int64_t sz, sy, sx;    // float strides of the tile's z, y, x
+    if (to_grid) {
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const int u = vch * VL + nb * 16 + (col >> 1);
+            const int64_t slab = (int64_t)(B.nfg > 1 ? fg_frame : 0) * B.Cb + u / B.Cv;
+            const int64_t p0 = ((int64_t)oz * g.os[1] + oy) * g.os[2] + ox;
+            base[nb] = reinterpret_cast<float*>(grid + (slab * g.gtot + p0) * B.Cv + u % B.Cv) + part;
+        }
+        sx = 2 * (int64_t)B.Cv;
+        sy = sx * g.os[2];
+        sz = sy * g.os[1];
+    } else {
+        float* d = reinterpret_cast<float*>(partial + ((int64_t)it.w * nvch + vch) * (int64_t)(Tx * Ty * Tz) * VL +
+                                            (col >> 1)) + part;
+        base[0] = d;
+        base[1] = d + 32;  // values 16 .. 31 of the chunk
+        sx = 2 * VL;
+        sy = sx * Tx;
+        sz = sy * Ty;
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i0 = 8 * q + 4 * h;  // first row of the 4 (x = i0 % TX .. + 3)
+            const int lz = mb * ZR + i0 / (TY * TX), ly = (i0 / TX) % TY, lx0 = i0 % TX;
+            if (lz >= zmax || ly >= ymax) continue;
+            const int64_t off = lz * sz + ly * sy + lx0 * sx;
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                float* row = base[nb] + off;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (lx0 + e < xmax) row[e * sx] = ldexpf(acc[mb][nb][4 * q + e], E0) * wunnorm;
+            }
+        }
 }
 
 // -------------------------------------------------------------------------
@@ -1850,13 +2243,43 @@ int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, fl
 // x (64 / VL) x parts of TXL points; VL = 32 holds 64 complex accumulators
 // per lane, the small-batch instances (VL = 16, 8) proportionally fewer.
 using RegLaunch = void (*)(dim3, hipStream_t, const GeoParams&, const BatchParams&, const pf_slist&,
-                           const pf_sitems&, const pf_plan*, const float2*, float2*, float2*);
+                           const pf_sitems&, const pf_plan*, const float2*, float2*, float2*, const float*);
+
+// I0(x) by its power series (the normalisation of the matrix-core spread's
+// weights needs the kernel's peak only roughly)
+static double host_i0(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 200 && term > 1e-17 * sum; ++k) {
+        const double f = 0.5 * x / k;
+        term *= f * f;
+        sum += term;
+    }
+    return sum;
+}
+
+// 1 / (peak of the separable kernel): the weights a tile's A operand holds
+// are at most 1
+static float spread_wnorm(const pf_plan* p) {
+    double peak = 1.0;
+    for (int a = 0; a < 3; ++a)
+        if (p->width[a] > 1) peak *= host_i0((double)p->beta[a]);
+    return (float)(1.0 / peak);
+}
+
+// the matrix-core spread's tiles: 128 points, whole z rows per 32-row block
+template <int TZ, int TY, int TX>
+constexpr bool smfma_tile() { return TZ * TY * TX == 128 && 32 % (TY * TX) == 0 && TX % 2 == 0; }
 
 template <int TZ, int TY, int TXL, int VL, int ZP>
 void launch_reg(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_slist& L,
-                const pf_sitems& I, const pf_plan* p, const float2* ys, float2* out, float2* partial) {
+                const pf_sitems& I, const pf_plan* p, const float2* ys, float2* out, float2* partial,
+                const float* smax) {
     if (B.dma && PF_SPREAD_PIPE) {
-        if constexpr (VL == SPREAD_VL && ZP == 1)
+        if constexpr (PF_SPREAD_MFMA && VL == SPREAD_VL && ZP == 1 && smfma_tile<TZ, TY, 2 * TXL>()) {
+            const float wn = spread_wnorm(p);
+            hipLaunchKernelGGL((k_spread_mfma<TZ, TY, 2 * TXL>), grid, dim3(64), 0, s, g, B, I.sitems, L.slist,
+                               p->taprec, ys, smax, out, partial, wn, 1.0f / wn);
+        } else if constexpr (VL == SPREAD_VL && ZP == 1)
             hipLaunchKernelGGL((k_spread_pipe<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, I.sitems, L.slist,
                                p->taprec, ys, out, partial);
         else
@@ -1937,6 +2360,23 @@ static bool spread_dma(const pf_plan* p, const pf_batch& b, const RegChoice& rc)
            (uint64_t)p->mtot * b.C * sizeof(float2) < (1ull << 32);
 }
 
+// the matrix-core spread runs for this batch (a 128-point register tile with
+// whole 32-value chunks copied global -> LDS)
+static bool spread_mfma(const pf_plan* p, const pf_batch& b) {
+    const RegChoice rc = spread_reg_choice(p, b);
+    if (!PF_SPREAD_MFMA || !PF_SPREAD_PIPE || !rc.fn || rc.vl != SPREAD_VL || !spread_dma(p, b, rc)) return false;
+    const RegTile& t = *rc.tile;
+    return t.x * rc.vl >= 64 && t.z * t.y * t.x == 128 && 32 % (t.y * t.x) == 0 && t.x % 2 == 0;
+}
+
+static size_t spread_tiles_bytes(const pf_plan* p, const pf_batch& b) {
+    const int VC = pf_spread_coil_chunk(p, b);
+    const int V = b.Cb * b.R * b.Cv;
+    const int nvch = (V + VC - 1) / VC;
+    const size_t tpts = (size_t)p->tile[0] * p->tile[1] * p->tile[2];
+    return (size_t)pf_sitems_for(pf_slist_for(p, b), nvch).nslots * nvch * tpts * VC * sizeof(float2);
+}
+
 std::string pf_spread_name(const pf_plan* p, const pf_batch& b) {
     const RegChoice rc = spread_reg_choice(p, b);
     std::string name;
@@ -1947,7 +2387,10 @@ std::string pf_spread_name(const pf_plan* p, const pf_batch& b) {
         const int zp = t.x * rc.vl >= 64 ? 1 : 2;
         const int txl = t.x * rc.vl * zp / 64;
         const std::string tile = std::to_string(t.z) + "," + std::to_string(t.y) + "," + std::to_string(txl);
-        if (spread_dma(p, b, rc) && PF_SPREAD_PIPE)
+        if (spread_mfma(p, b))
+            name = "k_smax+k_spread_mfma<" + std::to_string(t.z) + "," + std::to_string(t.y) + "," +
+                   std::to_string(t.x) + ">";
+        else if (spread_dma(p, b, rc) && PF_SPREAD_PIPE)
             name = (rc.vl == SPREAD_VL && zp == 1) ? "k_spread_pipe<" + tile + ">"
                                                     : "k_spread_pipe_s<" + tile + "," + std::to_string(rc.vl) + "," +
                                                           std::to_string(zp) + ">";
@@ -1980,12 +2423,11 @@ static size_t spread_lds_bytes(const pf_plan* p, int VC) {
     return words * 4 + (size_t)SB * VC * sizeof(float2);
 }
 
+// [partial tiles | per-sample chunk maxima of the matrix-core spread]
 size_t pf_spread_partial_bytes(const pf_plan* p, const pf_batch& b) {
-    const int VC = pf_spread_coil_chunk(p, b);
-    const int V = b.Cb * b.R * b.Cv;
-    const int nvch = (V + VC - 1) / VC;
-    const size_t tpts = (size_t)p->tile[0] * p->tile[1] * p->tile[2];
-    return (size_t)pf_sitems_for(pf_slist_for(p, b), nvch).nslots * nvch * tpts * VC * sizeof(float2);
+    size_t n = spread_tiles_bytes(p, b);
+    if (spread_mfma(p, b)) n = (n + 255) / 256 * 256 + (size_t)p->mtot * (b.C / SPREAD_VL) * sizeof(float);
+    return n;
 }
 
 int pf_spread_reg_tiles(int ndim, int* tiles, int cap) {
@@ -2021,7 +2463,17 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     const RegChoice rc = spread_reg_choice(p, b);
     if (rc.fn) {
         B.dma = spread_dma(p, b, rc) ? 1 : 0;
-        rc.fn(dim3((unsigned)nb), s, g, B, L, I, p, ys, grid, partial);
+        float* smax = nullptr;
+        if (spread_mfma(p, b)) {
+            if (!partial) return pf_fail(PF_ERR_ARG, "spread needs its workspace");
+            smax = reinterpret_cast<float*>(reinterpret_cast<char*>(partial) +
+                                            (spread_tiles_bytes(p, b) + 255) / 256 * 256);
+            const int64_t threads = p->mtot * (b.C / SPREAD_VL) * 16;
+            hipLaunchKernelGGL(k_smax, dim3((unsigned)((threads + TB - 1) / TB)), dim3(TB), 0, s, p->mtot, b.C, ys,
+                               smax);
+            PF_HIP(hipGetLastError());
+        }
+        rc.fn(dim3((unsigned)nb), s, g, B, L, I, p, ys, grid, partial, smax);
         PF_HIP(hipGetLastError());
         if (I.nritems > 0) {
             hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)I.nritems * nvch * nsub)), dim3(TB), 0, s, g,

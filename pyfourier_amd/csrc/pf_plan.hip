@@ -384,7 +384,10 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
     PF_HIP(skeys.alloc(NE * 4));
     PF_HIP(skeys2.alloc(NE * 4));
     PF_HIP(svals.alloc(NE * 4));
-    PF_HIP(hipMalloc(&out.slist, NE * 4));
+    // PF_SLIST_PAD ids past the end: whole 16-entry batches of the last
+    // item can be read as one piece
+    PF_HIP(hipMalloc(&out.slist, (NE + PF_SLIST_PAD) * 4));
+    PF_HIP(hipMemsetAsync(out.slist + NE, 0, PF_SLIST_PAD * 4, stream));
     hipLaunchKernelGGL(k_slist_emit, dim3(nblk(M, TB)), dim3(TB), 0, stream, p->taprec, P, merge,
                        off.as<uint32_t>(), skeys.as<uint32_t>(), svals.as<int32_t>());
     PF_HIP(hipGetLastError());

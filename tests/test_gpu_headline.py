@@ -2,8 +2,8 @@
 
 The 3-D multicoil headline (golden-means radial, 32 coils batched, sigma 2,
 |k|^2 density weights, KB width 6 and the configured width 4) runs the
-LDS-staged gather k_gather_col<W> and the software-pipelined 32-coil spread
-k_spread_pipe<8, 4, 2>.  Here they are checked:
+LDS-staged gather k_gather_col<W> and the matrix-core 32-coil spread
+k_spread_mfma<8, 4, 4>.  Here they are checked:
 
 * on a reduced grid (32^3 image, 64^3 grid) against the float64 restatement
   oracle.restate (src/pyfourier/_subroutines/_grid/_cart2noncart/_3D.py:13-51,
@@ -54,7 +54,7 @@ def test_headline_instances_vs_oracle(cuda_device, width, nspokes):
     plan.to(cuda_device)
     what = _Call(plan, cuda_device, None, None, None).describe(C)
     assert f"gather=k_gather_col<{width}>" in what, what
-    assert "spread=k_spread_pipe<8,4,2>" in what, what
+    assert "spread=k_smax+k_spread_mfma<8,4,4>" in what, what
     P = R.Plan(coord, N, width=width, oversamp=2.0)
 
     y = pf.nufft(torch.from_numpy(xc).to(cuda_device), nufft_plan=plan, weight=torch.from_numpy(w),
@@ -80,7 +80,7 @@ def test_headline_full_size_properties(cuda_device):
     plan = pf.plan_nufft(torch.from_numpy(coord), N, width=6, oversamp=2.0)
     plan.to(cuda_device)
     what = _Call(plan, cuda_device, None, None, None).describe(C)
-    assert "gather=k_gather_col<6>" in what and "spread=k_spread_pipe<8,4,2>" in what, what
+    assert "gather=k_gather_col<6>" in what and "spread=k_smax+k_spread_mfma<8,4,4>" in what, what
     g = torch.Generator(device=cuda_device)
     g.manual_seed(7)
     x = torch.randn((C, N, N, N), dtype=torch.complex64, device=cuda_device, generator=g)
