@@ -1333,7 +1333,7 @@ __device__ __forceinline__ pf_half8 smfma_cat(pf_half2 a, pf_half2 b, pf_half2 c
 #ifndef PF_SMFMA_OCC
 #define PF_SMFMA_OCC 2  // waves per SIMD the register budget is sized for
 #endif
-constexpr int SMFMA_NCOPY = SPIPE_EB * SPIPE_RP / 64 + SPIPE_EB / 4;  // copy instructions per batch
+constexpr int SMFMA_NCOPY = SPIPE_EB * SPIPE_RP / 64 + SPIPE_EB / 4 + 1;  // copy instructions per batch
 
 // max over the wave of non-negative floats (integer order of their bits),
 // by DPP row shifts and row broadcasts; the result is read from lane 63
@@ -1348,12 +1348,14 @@ __device__ __forceinline__ float smfma_wave_max(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
 
-// copies of one batch into one record slot and one value slot: always
-// SMFMA_NCOPY copy instructions (entries past n repeat entry 0), so the wait
-// for a batch can count the copies issued after it; lane j < 16 of sid
-// holds entry j's sample id
+// copies of one batch into one record slot and one value slot, plus the 16
+// sample ids of a later batch (ids_src) into an id slot: always SMFMA_NCOPY
+// copy instructions (entries past n repeat entry 0), so the wait for a batch
+// can count the copies issued after it; lane j < 16 of sid holds entry j's
+// sample id
 __device__ __forceinline__ void smfma_issue(float4 (&s_rec)[SPIPE_EB][SPIPE_RP], float2 (&val)[SPIPE_EB][SPREAD_VL],
-                                            int sid, int n, int rec4, const float* __restrict__ taprec,
+                                            int (&s_id)[SPIPE_EB], const int32_t* __restrict__ ids_src, int sid,
+                                            int n, int rec4, const float* __restrict__ taprec,
                                             const char* __restrict__ ybase, int C, int vch) {
     const int lane = threadIdx.x;
     const int c = lane & (SPIPE_RP - 1);
@@ -1375,6 +1377,11 @@ __device__ __forceinline__ void smfma_issue(float4 (&s_rec)[SPIPE_EB][SPIPE_RP],
         __builtin_amdgcn_global_load_lds(ybase + off, (__attribute__((address_space(3))) void*)&val[q * 4][0], 16, 0,
                                          0);
     }
+    // the ids last: the slot still held this batch's ids (sid), and sid has
+    // been consumed (the shuffles above waited for its read) before the copy
+    // can overwrite it
+    if (lane < SPIPE_EB)
+        __builtin_amdgcn_global_load_lds(ids_src + lane, (__attribute__((address_space(3))) void*)&s_id[0], 4, 0, 0);
 }
 
 // the batch's weight rows: entry e's weights at this tile's TZ z, TY y and TX
@@ -1448,6 +1455,8 @@ __global__ void __launch_bounds__(64, PF_SMFMA_OCC)
     __shared__ __attribute__((aligned(16))) float2 s_val1[EB][VL];
     __shared__ __attribute__((aligned(16))) float2 s_val2[EB][VL];
     __shared__ __attribute__((aligned(16))) float s_w[EB][TZ + TY + TX];  // 16-byte aligned rows
+    __shared__ int s_id0[EB];
+    __shared__ int s_id1[EB];
 
     const int nvch = B.C / VL;
     const unsigned bid = pf_xcd_remap(blockIdx.x, gridDim.x);
@@ -1526,70 +1535,66 @@ __global__ void __launch_bounds__(64, PF_SMFMA_OCC)
     };
 
     // batch b covers entries [b * EB, min(cnt, b * EB + EB)); its records go
-    // to record slot b % 2, its values to value slot b % 3.  The sample ids
-    // come through scalar loads, so the loop's only vector memory
-    // instructions are the copies, SMFMA_NCOPY per batch: when batch b is
-    // due, exactly the copies of batch b + 1 (if it exists) were issued
-    // after its own.
+    // to record slot b % 2, its values to value slot b % 3, and the copies of
+    // batch b bring the sample ids of batch b + 2 into id slot b % 2.  The
+    // loop's only vector memory instructions are these copies, SMFMA_NCOPY
+    // per batch: when batch b is due, exactly the copies of batch b + 1 (if
+    // it exists) were issued after its own.
     const int cnt = it.z - it.y;
     if (cnt > 0) {
         const int nbt = (cnt + EB - 1) / EB;
         const int32_t* sl = slist + it.y;
-        // batch b's sample ids, lane j < 16 holding entry j's: one scalar
-        // load of 16 ids (batches past the last reread the last one; entries
-        // past the item's end are read from the next item or the list's
-        // padding and never used)
-        auto ids = [&](int b) {
-            const int32_t* bp = sl + (int64_t)min(b, nbt - 1) * EB;
-            int v = 0;
-#pragma unroll
-            for (int j = 0; j < EB; ++j) v = lane == j ? bp[j] : v;
-            return v;
-        };
+        // batch b's 16 ids (batches past the last reread the last one;
+        // entries past the item's end come from the next item or the list's
+        // padding and are never used)
+        auto ids_src = [&](int b) { return sl + (int64_t)min(b, nbt - 1) * EB; };
         // every slot is named statically at each use, so the compiler can
         // tell the copies into one slot from reads of another and waits only
         // where this code does
-        auto issue = [&](int b, int sid, float4 (&rec)[EB][SPIPE_RP], float2 (&val)[EB][VL]) {
-            smfma_issue(rec, val, sid, min(EB, cnt - b * EB), rec4, taprec, ybase, B.C, vch);
+        auto issue = [&](int b, int sid, float4 (&rec)[EB][SPIPE_RP], float2 (&val)[EB][VL], int (&ids)[EB]) {
+            smfma_issue(rec, val, ids, ids_src(b + 2), sid, min(EB, cnt - b * EB), rec4, taprec, ybase, B.C, vch);
         };
         // batch b: records in rec (slot b % 2, which batch b + 2's records
-        // then reuse), values in cur (slot b % 3); batch b + 2's values go to
-        // nxt (slot (b + 2) % 3).  One wave per block: no block barrier (it
-        // would also wait for the copies in flight), the compiler fences keep
-        // LDS reads after the waits.
-        auto step = [&](int b, int& sid, float4 (&rec)[EB][SPIPE_RP], float2 (&nxt)[EB][VL],
-                        const float2 (&cur)[EB][VL]) {
+        // then reuse), values in cur (slot b % 3), ids of batch b + 2 in ids
+        // (slot b % 2, reused for those of batch b + 4); batch b + 2's values
+        // go to nxt (slot (b + 2) % 3).  One wave per block: no block barrier
+        // (it would also wait for the copies in flight), the compiler fences
+        // keep LDS reads after the waits.
+        auto step = [&](int b, float4 (&rec)[EB][SPIPE_RP], float2 (&nxt)[EB][VL], const float2 (&cur)[EB][VL],
+                        int (&ids)[EB]) {
             if (b + 1 < nbt)
                 __builtin_amdgcn_s_waitcnt(0xF70 | SMFMA_NCOPY);  // vmcnt(NCOPY); lgkm/exp not waited on
             else
                 __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
             asm volatile("" ::: "memory");
             smfma_weights<TZ, TY, TX>(rec, min(EB, cnt - b * EB), g, oz, oy, ox, wnorm, s_w);
+            if (b + 2 < nbt) {
+                const int sid = ids[lane & (EB - 1)];
+                asm volatile("" ::: "memory");
+                issue(b + 2, sid, rec, nxt, ids);
+            }
             asm volatile("" ::: "memory");
-            if (b + 2 < nbt) issue(b + 2, sid, rec, nxt);
-            sid = ids(b + 4);
             accumulate(cur);
         };
-        int sE = ids(0), sO = ids(1);  // ids of the next even / odd batch to issue
         // the tile's largest |value| (per-sample maxima of this value
         // chunk): its loads run under the first two batches' copies
+        const int sid0 = sl[lane & (EB - 1)];
+        const int sid1 = sl[min(EB + (lane & (EB - 1)), cnt - 1)];
         float mx = 0.f;
         for (int e = lane; e < cnt; e += 64) mx = fmaxf(mx, smax[(int64_t)sl[e] * nvch + vch]);
-        issue(0, sE, s_rec0, s_val0);
-        sE = ids(2);
-        if (nbt > 1) issue(1, sO, s_rec1, s_val1);
-        sO = ids(3);
+        issue(0, sid0, s_rec0, s_val0, s_id0);
+        if (nbt > 1) issue(1, sid1, s_rec1, s_val1, s_id1);
         mx = smfma_wave_max(mx);
         frexpf(mx, &E);  // mx < 2^E
         E -= 13;         // scaled values below 2^13 (fp16 holds up to 65504)
         // six steps per pass (slots repeat every 6 batches), one loop exit
         for (int b = 0; b < nbt; b += 6) {
-            step(b, sE, s_rec0, s_val2, s_val0);
-            if (b + 1 < nbt) step(b + 1, sO, s_rec1, s_val0, s_val1);
-            if (b + 2 < nbt) step(b + 2, sE, s_rec0, s_val1, s_val2);
-            if (b + 3 < nbt) step(b + 3, sO, s_rec1, s_val2, s_val0);
-            if (b + 4 < nbt) step(b + 4, sE, s_rec0, s_val0, s_val1);
-            if (b + 5 < nbt) step(b + 5, sO, s_rec1, s_val1, s_val2);
+            step(b, s_rec0, s_val2, s_val0, s_id0);
+            if (b + 1 < nbt) step(b + 1, s_rec1, s_val0, s_val1, s_id1);
+            if (b + 2 < nbt) step(b + 2, s_rec0, s_val1, s_val2, s_id0);
+            if (b + 3 < nbt) step(b + 3, s_rec1, s_val2, s_val0, s_id1);
+            if (b + 4 < nbt) step(b + 4, s_rec0, s_val0, s_val1, s_id0);
+            if (b + 5 < nbt) step(b + 5, s_rec1, s_val1, s_val2, s_id1);
         }
     }
     __builtin_amdgcn_s_waitcnt(0);  // no copy outlives the wave's LDS
