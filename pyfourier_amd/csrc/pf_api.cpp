@@ -470,8 +470,12 @@ int pf_nufft_adj(pf_plan* p, const pf_batch_desc* d, const void* kspace, void* i
     double factor = 1.0 / p->scale;
     if (norm == PF_NORM_ORTHO) factor /= std::sqrt(ghat(p));
     if (norm == PF_NORM_NONE) factor /= ghat(p);
-    if ((rc = pf_launch_sort(p, b, static_cast<const float2*>(kspace), ys, weight, s)) != PF_OK) return rc;
-    if ((rc = pf_launch_spread(p, b, ys, grid, part, s)) != PF_OK) return rc;
+    // the sort writes the spread's per-sample maxima when the spread uses them
+    bool smax_ready = false;
+    if ((rc = pf_launch_sort(p, b, static_cast<const float2*>(kspace), ys, weight, s, pf_spread_smax(p, b, part),
+                             &smax_ready)) != PF_OK)
+        return rc;
+    if ((rc = pf_launch_spread(p, b, ys, grid, part, s, smax_ready)) != PF_OK) return rc;
     if (pf_fftk_supported(p))
         return pf_fftk_adjoint(p, b, grid, static_cast<float2*>(image), extra, (float)factor, s);
     if ((rc = pf_fft_exec(p, b, grid, 1, s)) != PF_OK) return rc;

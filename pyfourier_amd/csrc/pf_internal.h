@@ -75,14 +75,16 @@ static inline const pf_sitems& pf_sitems_for(const pf_slist& L, int nvch) {
 #ifndef PF_GCOL_MAXS
 #define PF_GCOL_MAXS 256  // lanes per work item (one sample each, one block)
 #endif
-#ifndef PF_GCOL_DZ6
-#define PF_GCOL_DZ6 5
-#endif
 // z planes per window: (DZ + W - 1) * (PF_GCOL + W - 1)^2 points of 16 bytes
-// stay near 70 KB (two blocks per CU)
+// within 80 KB (two blocks per CU).  Deeper windows restage less halo per
+// staged point but hold more samples, and a window with more samples than a
+// block's lanes is staged once per block: measured on 3-D radial at 128^3,
+// width 6 is fastest at the full budget (DZ 6, 76 KB; 5 and 4 slower), width
+// 4 at DZ 7 (58 KB; the full-budget DZ 11 is 12 % slower).
 __host__ __device__ constexpr int pf_gather_dz(int W) {
-    return W == 6 ? PF_GCOL_DZ6 : (W <= 6 ? 11 - W : (W == 7 ? 3 : 2));
+    return W == 6 ? 6 : (W <= 6 ? 11 - W : (W == 7 ? 3 : 2));
 }
+static_assert((pf_gather_dz(6) + 5) * 21 * 21 * 16 <= 80 * 1024, "width-6 window within two blocks per CU");
 struct pf_glist {
     int32_t* ids = nullptr;  // lane slots: sorted-sample index t or -1, PF_GCOL_MAXS per item
     int4* items = nullptr;   // (key, first lane slot, 0, 0)
@@ -163,12 +165,19 @@ int pf_launch_crop_deapod(const pf_plan* p, const pf_batch& b, const float2* gri
                           float factor, hipStream_t s);
 int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, float2* ys,
                      hipStream_t s);
+// smax_ready: the sort already wrote the per-sample maxima the matrix-core
+// spread scales by (pf_spread_smax), else the spread computes them
 int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, float2* grid,
-                     float2* partial, hipStream_t s);
+                     float2* partial, hipStream_t s, bool smax_ready = false);
+// where the matrix-core spread keeps its per-sample maxima inside the spread
+// workspace, or null when the batch does not use it
+float* pf_spread_smax(const pf_plan* p, const pf_batch& b, float2* partial);
 int pf_launch_unsort(const pf_plan* p, const pf_batch& b, const float2* ys, float2* y,
                      const float* weight, float factor, hipStream_t s);
+// smax: also write the per-sample maxima of the matrix-core spread where
+// the sort can (*smax_written says whether it did)
 int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2* ys,
-                   const float* weight, hipStream_t s);
+                   const float* weight, hipStream_t s, float* smax = nullptr, bool* smax_written = nullptr);
 int pf_fft_exec(pf_plan* p, const pf_batch& b, float2* grid, int inverse, hipStream_t s);
 void pf_fft_destroy(pf_fft_cache* c);
 int pf_fft_cache_init(pf_plan* p);  // rocFFT setup + the plan's (empty) cache

@@ -1285,10 +1285,10 @@ __global__ void __launch_bounds__(64, spread_wreg<100 * TZ + TY>() ? 3 : 1)
 // is below 2^-22 relative), so the result matches FP32 arithmetic to rounding
 // level while the tile's arithmetic runs on the matrix cores instead of the
 // vector ALUs.  fp16 range: weights are normalised by the kernel's peak
-// value (w <= 1), values by a power of two per tile and value chunk, taken
-// from the largest |value| among the tile's samples (k_smax's per-sample
-// maxima) so that scaled values stay below 2^13; both are undone when the
-// tile is written.  Deterministic: a fixed instruction sequence per tile.
+// value (w <= 1), values by a power of two per work item and value chunk,
+// taken from the largest |value| among its samples (k_tile_max over the
+// per-sample maxima the sort or k_smax wrote) so that scaled values stay
+// below 2^13; both are undone when the tile is written.  Deterministic: a fixed instruction sequence per tile.
 // -------------------------------------------------------------------------
 // per sorted sample t and 32-value chunk v: max |re|, |im| over the chunk's
 // values, smax[t * nvch + v]; 16 lanes per (t, v), one 16-byte piece each
@@ -1347,6 +1347,23 @@ __device__ __forceinline__ float smfma_wave_max(float v) {
     x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
     return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
+
+// per spread work item and 32-value chunk (index item * nvch + chunk, the
+// spread's block index): the largest of its samples' smax, one wave each
+__global__ void __launch_bounds__(TB) k_tile_max(int nitems, int nvch, const int4* __restrict__ items,
+                                                 const int32_t* __restrict__ slist, const float* __restrict__ smax,
+                                                 float* __restrict__ tmax) {
+    const int64_t wid = ((int64_t)blockIdx.x * TB + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wid >= (int64_t)nitems * nvch) return;  // whole waves
+    const int4 it = items[wid / nvch];
+    const int vch = (int)(wid % nvch);
+    float m = 0.f;
+    for (int e = it.y + lane; e < it.z; e += 64) m = fmaxf(m, smax[(int64_t)slist[e] * nvch + vch]);
+    m = smfma_wave_max(m);
+    if (lane == 0) tmax[wid] = m;
+}
+
 
 // copies of one batch into one record slot and one value slot, plus the 16
 // sample ids of a later batch (ids_src) into an id slot: always SMFMA_NCOPY
@@ -1442,7 +1459,7 @@ __device__ __forceinline__ void smfma_weights(const float4 (&s_rec)[SPIPE_EB][SP
 template <int TZ, int TY, int TX>
 __global__ void __launch_bounds__(64, PF_SMFMA_OCC)
     k_spread_mfma(GeoParams g, BatchParams B, const int4* __restrict__ items, const int32_t* __restrict__ slist,
-                  const float* __restrict__ taprec, const float2* __restrict__ ys, const float* __restrict__ smax,
+                  const float* __restrict__ taprec, const float2* __restrict__ ys, const float* __restrict__ tmax,
                   float2* __restrict__ grid, float2* __restrict__ partial, float wnorm, float wunnorm) {
     static_assert(TZ * TY * TX == 128 && 32 % (TY * TX) == 0 && TX % 2 == 0, "128-point tile, whole z rows per block");
     constexpr int VL = SPREAD_VL;
@@ -1576,17 +1593,13 @@ __global__ void __launch_bounds__(64, PF_SMFMA_OCC)
             asm volatile("" ::: "memory");
             accumulate(cur);
         };
-        // the tile's largest |value| (per-sample maxima of this value
-        // chunk): its loads run under the first two batches' copies
         const int sid0 = sl[lane & (EB - 1)];
         const int sid1 = sl[min(EB + (lane & (EB - 1)), cnt - 1)];
-        float mx = 0.f;
-        for (int e = lane; e < cnt; e += 64) mx = fmaxf(mx, smax[(int64_t)sl[e] * nvch + vch]);
         issue(0, sid0, s_rec0, s_val0, s_id0);
         if (nbt > 1) issue(1, sid1, s_rec1, s_val1, s_id1);
-        mx = smfma_wave_max(mx);
-        frexpf(mx, &E);  // mx < 2^E
-        E -= 13;         // scaled values below 2^13 (fp16 holds up to 65504)
+        // the largest |value| of this item's samples in this chunk
+        frexpf(tmax[bid], &E);  // tmax < 2^E
+        E -= 13;                // scaled values below 2^13 (fp16 holds up to 65504)
         // six steps per pass (slots repeat every 6 batches), one loop exit
         for (int b = 0; b < nbt; b += 6) {
             step(b, s_rec0, s_val2, s_val0, s_id0);
@@ -2001,9 +2014,12 @@ constexpr int QSC = 32;
 constexpr int QN = QSB * QSC / 2 / TB;  // float4 per thread and phase
 
 // user -> sorted, times the density weight
+// With smax (C a multiple of 32, one 32-value chunk per block row): also
+// the chunk's max |re|, |im| per sorted sample, smax[t * C / 32 + chunk], for
+// the matrix-core spread's value scale.
 __global__ void __launch_bounds__(TB) k_sort4(int64_t mtot, int C, const int32_t* __restrict__ iperm,
                                               const float2* __restrict__ y, const float* __restrict__ w,
-                                              float2* __restrict__ ys) {
+                                              float2* __restrict__ ys, float* __restrict__ smax) {
     __shared__ float2 t[QSB][QSC + 1];
     const int64_t j0 = (int64_t)blockIdx.x * QSB;
     const int c0 = blockIdx.y * QSC;
@@ -2039,9 +2055,19 @@ __global__ void __launch_bounds__(TB) k_sort4(int64_t mtot, int C, const int32_t
     for (int k = 0; k < QN; ++k) {
         const int i = threadIdx.x + k * TB;
         const int jl = i / h, q = i - jl * h;
+        float m = 0.f;
         if (jl < nj) {
             const float2 a = t[jl][2 * q], b = t[jl][2 * q + 1];
             *reinterpret_cast<float4*>(ys + (int64_t)row[k] * C + c0 + 2 * q) = make_float4(a.x, a.y, b.x, b.y);
+            m = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(b.x), fabsf(b.y)));
+        }
+        if (smax) {  // h = 16: a row's pieces are one 16-lane DPP row
+            int x = __float_as_int(m);
+            x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+            x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+            x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+            x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+            if (q == 15 && jl < nj) smax[(int64_t)row[k] * (C / QSC) + blockIdx.y] = __int_as_float(x);
         }
     }
 }
@@ -2278,12 +2304,12 @@ constexpr bool smfma_tile() { return TZ * TY * TX == 128 && 32 % (TY * TX) == 0 
 template <int TZ, int TY, int TXL, int VL, int ZP>
 void launch_reg(dim3 grid, hipStream_t s, const GeoParams& g, const BatchParams& B, const pf_slist& L,
                 const pf_sitems& I, const pf_plan* p, const float2* ys, float2* out, float2* partial,
-                const float* smax) {
+                const float* tmax) {
     if (B.dma && PF_SPREAD_PIPE) {
         if constexpr (PF_SPREAD_MFMA && VL == SPREAD_VL && ZP == 1 && smfma_tile<TZ, TY, 2 * TXL>()) {
             const float wn = spread_wnorm(p);
             hipLaunchKernelGGL((k_spread_mfma<TZ, TY, 2 * TXL>), grid, dim3(64), 0, s, g, B, I.sitems, L.slist,
-                               p->taprec, ys, smax, out, partial, wn, 1.0f / wn);
+                               p->taprec, ys, tmax, out, partial, wn, 1.0f / wn);
         } else if constexpr (VL == SPREAD_VL && ZP == 1)
             hipLaunchKernelGGL((k_spread_pipe<TZ, TY, TXL>), grid, dim3(64), 0, s, g, B, I.sitems, L.slist,
                                p->taprec, ys, out, partial);
@@ -2393,7 +2419,7 @@ std::string pf_spread_name(const pf_plan* p, const pf_batch& b) {
         const int txl = t.x * rc.vl * zp / 64;
         const std::string tile = std::to_string(t.z) + "," + std::to_string(t.y) + "," + std::to_string(txl);
         if (spread_mfma(p, b))
-            name = "k_smax+k_spread_mfma<" + std::to_string(t.z) + "," + std::to_string(t.y) + "," +
+            name = "k_tile_max+k_spread_mfma<" + std::to_string(t.z) + "," + std::to_string(t.y) + "," +
                    std::to_string(t.x) + ">";
         else if (spread_dma(p, b, rc) && PF_SPREAD_PIPE)
             name = (rc.vl == SPREAD_VL && zp == 1) ? "k_spread_pipe<" + tile + ">"
@@ -2428,11 +2454,23 @@ static size_t spread_lds_bytes(const pf_plan* p, int VC) {
     return words * 4 + (size_t)SB * VC * sizeof(float2);
 }
 
-// [partial tiles | per-sample chunk maxima of the matrix-core spread]
+static size_t al256(size_t n) { return (n + 255) / 256 * 256; }
+
+// [partial tiles | per-sample chunk maxima | per-item chunk maxima], the
+// maxima for the matrix-core spread only
 size_t pf_spread_partial_bytes(const pf_plan* p, const pf_batch& b) {
     size_t n = spread_tiles_bytes(p, b);
-    if (spread_mfma(p, b)) n = (n + 255) / 256 * 256 + (size_t)p->mtot * (b.C / SPREAD_VL) * sizeof(float);
+    if (spread_mfma(p, b)) {
+        const int nvch = b.C / SPREAD_VL;
+        n = al256(n) + al256((size_t)p->mtot * nvch * sizeof(float)) +
+            (size_t)pf_sitems_for(pf_slist_for(p, b), nvch).nsitems * nvch * sizeof(float);
+    }
     return n;
+}
+
+float* pf_spread_smax(const pf_plan* p, const pf_batch& b, float2* partial) {
+    if (!partial || !spread_mfma(p, b)) return nullptr;
+    return reinterpret_cast<float*>(reinterpret_cast<char*>(partial) + al256(spread_tiles_bytes(p, b)));
 }
 
 int pf_spread_reg_tiles(int ndim, int* tiles, int cap) {
@@ -2450,7 +2488,7 @@ int pf_spread_reg_tiles(int ndim, int* tiles, int cap) {
 }
 
 int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, float2* grid, float2* partial,
-                     hipStream_t s) {
+                     hipStream_t s, bool smax_ready) {
     GeoParams g = geo(p);
     BatchParams B = bparams(b);
     const int VC = pf_spread_coil_chunk(p, b);
@@ -2468,17 +2506,23 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     const RegChoice rc = spread_reg_choice(p, b);
     if (rc.fn) {
         B.dma = spread_dma(p, b, rc) ? 1 : 0;
-        float* smax = nullptr;
+        float* tmax = nullptr;
         if (spread_mfma(p, b)) {
-            if (!partial) return pf_fail(PF_ERR_ARG, "spread needs its workspace");
-            smax = reinterpret_cast<float*>(reinterpret_cast<char*>(partial) +
-                                            (spread_tiles_bytes(p, b) + 255) / 256 * 256);
-            const int64_t threads = p->mtot * (b.C / SPREAD_VL) * 16;
-            hipLaunchKernelGGL(k_smax, dim3((unsigned)((threads + TB - 1) / TB)), dim3(TB), 0, s, p->mtot, b.C, ys,
-                               smax);
+            float* smax = pf_spread_smax(p, b, partial);
+            if (!smax) return pf_fail(PF_ERR_ARG, "spread needs its workspace");
+            tmax = reinterpret_cast<float*>(reinterpret_cast<char*>(smax) + al256((size_t)p->mtot * nvch * sizeof(float)));
+            if (!smax_ready) {
+                const int64_t threads = p->mtot * nvch * 16;
+                hipLaunchKernelGGL(k_smax, dim3((unsigned)((threads + TB - 1) / TB)), dim3(TB), 0, s, p->mtot, b.C, ys,
+                                   smax);
+                PF_HIP(hipGetLastError());
+            }
+            const int64_t waves = (int64_t)I.nsitems * nvch;
+            hipLaunchKernelGGL(k_tile_max, dim3((unsigned)((waves * 64 + TB - 1) / TB)), dim3(TB), 0, s, I.nsitems,
+                               nvch, I.sitems, L.slist, smax, tmax);
             PF_HIP(hipGetLastError());
         }
-        rc.fn(dim3((unsigned)nb), s, g, B, L, I, p, ys, grid, partial, smax);
+        rc.fn(dim3((unsigned)nb), s, g, B, L, I, p, ys, grid, partial, tmax);
         PF_HIP(hipGetLastError());
         if (I.nritems > 0) {
             hipLaunchKernelGGL(k_spread_reduce, dim3((unsigned)((int64_t)I.nritems * nvch * nsub)), dim3(TB), 0, s, g,
@@ -2520,10 +2564,14 @@ static bool permute_vec(const pf_plan* p, const pf_batch& b, const void* a, cons
 }
 
 int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2* ys, const float* weight,
-                   hipStream_t s) {
+                   hipStream_t s, float* smax, bool* smax_written) {
     if ((p->mtot + PSB - 1) / PSB > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many samples");
+    // the maxima come with the 16-byte sort of whole 32-value chunks only
+    if (!permute_vec(p, b, y, ys) || b.C % QSC != 0) smax = nullptr;
+    if (smax_written) *smax_written = smax != nullptr;
     if (permute_vec(p, b, y, ys))
-        hipLaunchKernelGGL(k_sort4, permute_grid4(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, y, weight, ys);
+        hipLaunchKernelGGL(k_sort4, permute_grid4(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, y, weight, ys,
+                           smax);
     else
         hipLaunchKernelGGL(k_sort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, y, weight, ys);
     PF_HIP(hipGetLastError());

@@ -43,7 +43,7 @@ def rel(a, b):
         (6, 24, 64, 128, 48, 2.0),
         (7, 16, 4, 128, 32, 2.0),
         (8, 16, 4, 128, 32, 2.0),
-        (6, 8, 4, 64, 16, 2.0),  # 16^3 grid: smaller than the 21 x 21 x 10 window
+        (6, 8, 4, 64, 16, 2.0),  # 16^3 grid: smaller than the 21 x 21 x 11 window
         (4, 40, 2, 200, 80, 1.25),  # grid 50: columns and windows cut by the edge
     ],
 )

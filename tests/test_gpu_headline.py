@@ -54,7 +54,7 @@ def test_headline_instances_vs_oracle(cuda_device, width, nspokes):
     plan.to(cuda_device)
     what = _Call(plan, cuda_device, None, None, None).describe(C)
     assert f"gather=k_gather_col<{width}>" in what, what
-    assert "spread=k_smax+k_spread_mfma<8,4,4>" in what, what
+    assert "spread=k_tile_max+k_spread_mfma<8,4,4>" in what, what
     P = R.Plan(coord, N, width=width, oversamp=2.0)
 
     y = pf.nufft(torch.from_numpy(xc).to(cuda_device), nufft_plan=plan, weight=torch.from_numpy(w),
@@ -80,7 +80,7 @@ def test_headline_full_size_properties(cuda_device):
     plan = pf.plan_nufft(torch.from_numpy(coord), N, width=6, oversamp=2.0)
     plan.to(cuda_device)
     what = _Call(plan, cuda_device, None, None, None).describe(C)
-    assert "gather=k_gather_col<6>" in what and "spread=k_smax+k_spread_mfma<8,4,4>" in what, what
+    assert "gather=k_gather_col<6>" in what and "spread=k_tile_max+k_spread_mfma<8,4,4>" in what, what
     g = torch.Generator(device=cuda_device)
     g.manual_seed(7)
     x = torch.randn((C, N, N, N), dtype=torch.complex64, device=cuda_device, generator=g)
@@ -108,3 +108,30 @@ def test_headline_full_size_properties(cuda_device):
     e_nudft = rel(y0.reshape(-1)[torch.from_numpy(sel).to(cuda_device)].cpu().numpy(), yt)
     print(f"  W=6 forward vs FP64 NUDFT {e_nudft:.2e}", flush=True)
     assert e_nudft < 1e-4
+
+
+def test_headline_staged_equals_fused(cuda_device):
+    """The stage entry points (pad+FFT, gather, unsort; sort, spread,
+    IFFT+crop) called one by one give the fused operators' result bit for
+    bit.  The stand-alone spread computes the per-sample maxima of its
+    matrix-core value scale itself (k_smax); the fused adjoint takes them from
+    the sort."""
+    from pyfourier_amd._nufft import stage_timer
+
+    print("headline staged vs fused: 32^3 x 32 coils, 512x64, W=6", flush=True)
+    rng = np.random.default_rng(9)
+    N, C = 32, 32
+    coord = demo.golden_means_radial_3d(512, 64)
+    w = torch.from_numpy(_ramp(coord))
+    plan = pf.plan_nufft(torch.from_numpy(coord), N, width=6, oversamp=2.0)
+    plan.to(cuda_device)
+    x = torch.from_numpy(demo.complex_gaussian(rng, (C, N, N, N))).to(cuda_device)
+    k = torch.from_numpy(demo.complex_gaussian(rng, (C,) + coord.shape[:-1])).to(cuda_device)
+    yf = pf.nufft(x, nufft_plan=plan, weight=w, device=cuda_device)
+    xf = pf.nufft_adj(k, nufft_plan=plan, weight=w, device=cuda_device)
+    with stage_timer():
+        ys = pf.nufft(x, nufft_plan=plan, weight=w, device=cuda_device)
+        xs = pf.nufft_adj(k, nufft_plan=plan, weight=w, device=cuda_device)
+    torch.cuda.synchronize(cuda_device)
+    assert torch.equal(yf.view(torch.float32), ys.view(torch.float32))
+    assert torch.equal(xf.view(torch.float32), xs.view(torch.float32))
