@@ -462,7 +462,8 @@ def _traj_tensor(coord, call):
     return None
 
 
-def _prepare(x, coord, plan, shape_arg, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, for_adj):
+def _prepare(x, coord, plan, shape_arg, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, for_adj,
+             units="auto"):
     backend_numpy = not isinstance(x, torch.Tensor)
     x = _to_torch(x)
     idevice = x.device
@@ -475,7 +476,8 @@ def _prepare(x, coord, plan, shape_arg, zmap, L, nbins, dt, T, L_batch_size, wid
             if for_adj:
                 raise ValueError("nufft_adj needs the image shape when no plan is given")
             shape_arg = tuple(x.shape[-c.shape[-1]:])
-        plan = plan_nufft(c.detach().to(torch.float32), shape_arg, width, oversamp, zmap, L, nbins, dt, T, L_batch_size)
+        plan = plan_nufft(c.detach().to(torch.float32), shape_arg, width, oversamp, zmap, L, nbins, dt, T, L_batch_size,
+                          units=units)
     plan.to(dev)
     return x, backend_numpy, idevice, dev, plan
 
@@ -498,15 +500,17 @@ def nufft(
     threadsperblock=128,
     width=4,
     oversamp=1.25,
+    units="auto",
 ):
     """N-dimensional forward NUFFT (drop-in for pyfourier.nufft).
 
     image (..., [ncontrasts | ncoeff], [nz], ny, nx) -> kspace (..., [ncontrasts], nviews, nsamples).
     `device` must name a GPU ("cuda:n" or n); `threadsperblock` is accepted and ignored.
+    `units` (extension): unit of coord when no plan is given, as for plan_nufft.
     """
     code = _norm_code(norm)
     x, backend_numpy, idevice, dev, plan = _prepare(
-        image, coord, nufft_plan, shape, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, False
+        image, coord, nufft_plan, shape, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, False, units
     )
     dtype = x.dtype
     call = _Call(plan, dev, basis, weight, norm)
@@ -540,14 +544,16 @@ def nufft_adj(
     threadsperblock=128,
     width=4,
     oversamp=1.25,
+    units="auto",
 ):
     """N-dimensional adjoint NUFFT (drop-in for pyfourier.nufft_adj).
 
     kspace (..., [ncontrasts], nviews, nsamples) -> image (..., [ncontrasts | ncoeff], [nz], ny, nx).
+    `units` (extension): unit of coord when no plan is given, as for plan_nufft.
     """
     code = _norm_code(norm)
     y, backend_numpy, idevice, dev, plan = _prepare(
-        kspace, coord, nufft_plan, shape, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, True
+        kspace, coord, nufft_plan, shape, zmap, L, nbins, dt, T, L_batch_size, width, oversamp, device, True, units
     )
     dtype = y.dtype
     call = _Call(plan, dev, basis, weight, norm)

@@ -144,3 +144,27 @@ def test_permutation_paths(cuda_device, C, nsp, ns):
     xa = pf.nufft_adj(torch.from_numpy(k).to(cuda_device), nufft_plan=plan, weight=w, device=cuda_device)
     xr = R.nufft_adj(k.reshape(C, 1, -1), P, weight=w)[:, 0]
     assert rel(xa.cpu().numpy(), xr) < PARITY
+
+
+def test_units_keyword_on_the_operators(cuda_device):
+    """A grid-unit calibration region (radius <= pi, which "auto" would read
+    as radians) passed straight to nufft / nufft_adj with units="grid": the
+    operator equals the restatement on the unscaled grid coordinates."""
+    rng = np.random.default_rng(12)
+    N, C = 48, 2
+    coord = (rng.uniform(-3.0, 3.0, (4, 96, 2))).astype(np.float32)  # grid units, |k| < pi
+    x = demo.complex_gaussian(rng, (C, N, N))
+    k = demo.complex_gaussian(rng, (C,) + coord.shape[:-1])
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), torch.from_numpy(coord), width=4, oversamp=2.0,
+                 device=cuda_device, units="grid")
+    xa = pf.nufft_adj(torch.from_numpy(k).to(cuda_device), torch.from_numpy(coord), shape=N, width=4,
+                      oversamp=2.0, device=cuda_device, units="grid")
+    P = R.Plan(coord, N, width=4, oversamp=2.0, normalize=False)
+    yr = R.nufft(x[:, None], P)[:, 0].reshape((C,) + coord.shape[:-1])
+    xr = R.nufft_adj(k.reshape(C, 1, -1), P)[:, 0]
+    assert rel(y.cpu().numpy(), yr) < PARITY
+    assert rel(xa.cpu().numpy(), xr) < PARITY
+    # the default reads the same numbers as radians: a different operator
+    y_auto = pf.nufft(torch.from_numpy(x).to(cuda_device), torch.from_numpy(coord), width=4, oversamp=2.0,
+                      device=cuda_device)
+    assert rel(y_auto.cpu().numpy(), yr) > 0.1
