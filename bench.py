@@ -349,7 +349,7 @@ def bench_uneven_batch(pf, dev, steps):
     spokes = [int(v) for v in rng.integers(16, 257, 64)]
     coords = [demo.golden_angle_radial_2d(n, 256) for n in spokes]
     C = 8
-    batch = pf.plan_nufft_batch(coords, sizes, nbatch=C, width=4, oversamp=2.0, device=dev)
+    batch = pf.plan_nufft_batch(coords, sizes, nbatch=C, width=4, oversamp=2.0, device=dev, autotune=True)
     for buf in batch.images:
         buf.copy_(torch.randn(buf.shape, dtype=torch.complex64, device=dev))
     plans = batch.plans
@@ -367,7 +367,8 @@ def bench_uneven_batch(pf, dev, steps):
     t_batch = _time(batched, dev, steps)
     return {"workload": "64 uneven 2D slices {128,192,256}^2 x 8 coils, radial 16-256 spokes x 256, W=4",
             "samples": int(sum(spokes) * 256), "groups": len(batch.groups), "plain_ms": round(t_plain, 3),
-            "batched_ms": round(t_batch, 3), "speedup": round(t_plain / t_batch, 2)}
+            "batched_ms": round(t_batch, 3), "speedup": round(t_plain / t_batch, 2),
+            "autotuned": [{"tile": list(g.tile) if g.tile else None, "chunk": g.chunk} for g in batch.groups]}
 
 
 def bench_subspace(pf, dev, steps):

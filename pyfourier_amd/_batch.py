@@ -28,18 +28,46 @@ from ._plan import NativePlan, plan_nufft, resolve_device
 
 
 class _Group:
-    """Items of one geometry as the frames of one ragged native plan."""
+    """Items of one geometry as the frames of one ragged native plan.
 
-    def __init__(self, plans, weights, nbatch, device, tile, chunk):
+    autotune: build the ragged plan for every register-spread tile and
+    spread work-item size (as autotune_plan does for one plan), time this
+    group's forward + adjoint with its batch on each, and keep the fastest;
+    `tune_table` then lists the candidates' times."""
+
+    def __init__(self, plans, weights, nbatch, device, tile, chunk, autotune=False, chunks=(0, 1024), reps=3):
         it0 = plans[0].interpolator
         d = it0.ndim
         coords = [p.interpolator.coord.reshape(-1, d) for p in plans]
         counts = [int(c.shape[0]) for c in coords]
         off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
         self.device = device
-        self.native = NativePlan(torch.cat(coords), len(plans), 0, it0.shape, it0.width, it0.oversamp, device,
-                                 tile, chunk, frame_offsets=off.tolist())
+        cat = torch.cat(coords)
+
+        def build(t, c):
+            return NativePlan(cat, len(plans), 0, it0.shape, it0.width, it0.oversamp, device, t, c,
+                              frame_offsets=off.tolist())
+
         total = int(off[-1])
+        self.nbatch = nbatch
+        self.tune_table = []
+        if autotune:
+            from ._tune import spread_tiles
+
+            self.images = torch.randn((nbatch, len(plans)) + tuple(it0.shape), dtype=torch.complex64, device=device)
+            self.kspace = torch.empty((nbatch, total), dtype=torch.complex64, device=device)
+            self.weight = None
+            best = None
+            for t in spread_tiles(d):
+                for c in chunks:
+                    self.native = build(t, c)
+                    ms = self._time(reps)
+                    self.tune_table.append({"tile": t, "chunk": c, "ms": round(ms, 4)})
+                    if best is None or ms < best[0]:
+                        best = (ms, t, c)
+            tile, chunk = best[1], best[2]
+        self.tile, self.chunk = tile, chunk
+        self.native = build(tile, chunk)
         self.images = torch.zeros((nbatch, len(plans)) + tuple(it0.shape), dtype=torch.complex64, device=device)
         self.kspace = torch.zeros((nbatch, total), dtype=torch.complex64, device=device)
         self.weight = None
@@ -49,7 +77,20 @@ class _Group:
             self.weight = torch.cat(parts).contiguous()
         self.image_views = [self.images[:, k:k + 1] for k in range(len(plans))]
         self.kspace_views = [self.kspace[:, int(off[k]):int(off[k + 1])].unsqueeze(1) for k in range(len(plans))]
-        self.nbatch = nbatch
+
+    def _time(self, reps):
+        """ms per forward + adjoint of the current native plan."""
+        self._run(False, 0)
+        self._run(True, _adjoint_code(0))
+        st = torch.cuda.current_stream(self.device)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            self._run(False, 0)
+            self._run(True, _adjoint_code(0))
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
 
     def _run(self, adjoint, code):
         L = _lib.lib()
@@ -82,7 +123,7 @@ class NUFFTBatch:
     """
 
     def __init__(self, coords, shapes, nbatch=1, width=4, oversamp=1.25, weights=None, basis=None, norm=None,
-                 device="cuda", graph=True, group=True, tile=None, chunk=0, units="auto"):
+                 device="cuda", graph=True, group=True, tile=None, chunk=0, units="auto", autotune=False):
         if len(coords) != len(shapes):
             raise ValueError("coords and shapes must have the same length")
         if len(coords) == 0:
@@ -120,7 +161,8 @@ class NUFFTBatch:
                 self.kspace[i] = torch.zeros((nbatch, it.nframes, it.npts), dtype=torch.complex64,
                                              device=self.device)
         for idx in groups.values():
-            g = _Group([self.plans[i] for i in idx], [wts[i] for i in idx], nbatch, self.device, tile, chunk)
+            g = _Group([self.plans[i] for i in idx], [wts[i] for i in idx], nbatch, self.device, tile, chunk,
+                       autotune=autotune and tile is None)
             self.groups.append(g)
             for k, i in enumerate(idx):
                 self.images[i] = g.image_views[k]
@@ -193,7 +235,10 @@ class NUFFTBatch:
 
 
 def plan_nufft_batch(coords, shapes, nbatch=1, width=4, oversamp=1.25, weights=None, basis=None, norm=None,
-                     device="cuda", graph=True, group=True, tile=None, chunk=0, units="auto") -> NUFFTBatch:
-    """Plan an uneven batch of NUFFT problems (see NUFFTBatch)."""
+                     device="cuda", graph=True, group=True, tile=None, chunk=0, units="auto",
+                     autotune=False) -> NUFFTBatch:
+    """Plan an uneven batch of NUFFT problems (see NUFFTBatch).  autotune=True
+    (with tile=None) picks each group's tile and spread work-item size by
+    timing the group's own forward + adjoint on every candidate."""
     return NUFFTBatch(coords, shapes, nbatch, width, oversamp, weights, basis, norm, device, graph, group, tile,
-                      chunk, units)
+                      chunk, units, autotune)

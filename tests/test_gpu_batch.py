@@ -97,3 +97,28 @@ def test_batch_input_buffers_in_place(cuda_device):
         assert torch.allclose(b, 2 * a, rtol=1e-5, atol=1e-5)
     with pytest.raises(ValueError):
         batch.forward(imgs[:-1])
+
+
+def test_batch_autotuned_groups(cuda_device):
+    """autotune=True times every register tile and work-item size on each
+    group's own ragged plan and keeps the fastest; the tuned batch still
+    matches the restatement item by item, forward and adjoint."""
+    rng = np.random.default_rng(3)
+    C = 2
+    coords, shapes = _items(rng, 5)
+    batch = pf.plan_nufft_batch(coords, shapes, nbatch=C, width=4, oversamp=2.0, device=cuda_device, autotune=True)
+    ntiles = len(pf.spread_tiles(2))
+    for g in batch.groups:
+        assert len(g.tune_table) == 2 * ntiles
+        best = min(g.tune_table, key=lambda r: r["ms"])
+        assert (tuple(best["tile"]), best["chunk"]) == (tuple(g.tile), g.chunk)
+    imgs = [torch.from_numpy(demo.complex_gaussian(rng, (C, 1, N, N))).to(cuda_device) for N in shapes]
+    ksps = [torch.from_numpy(demo.complex_gaussian(rng, (C, 1, c.shape[0] * c.shape[1]))).to(cuda_device)
+            for c in coords]
+    fwd = [o.clone() for o in batch.forward(imgs)]
+    adj = [o.clone() for o in batch.adjoint(ksps)]
+    torch.cuda.synchronize()
+    for i in range(len(coords)):
+        P = R.Plan(coords[i], shapes[i], width=4, oversamp=2.0)
+        assert rel(fwd[i].cpu().numpy(), R.nufft(imgs[i].cpu().numpy(), P)) < PARITY, f"item {i} forward"
+        assert rel(adj[i].cpu().numpy(), R.nufft_adj(ksps[i].cpu().numpy(), P)) < PARITY, f"item {i} adjoint"
