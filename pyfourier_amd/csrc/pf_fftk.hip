@@ -523,6 +523,93 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_fftcol256(int N
 }
 
 // ---------------------------------------------------------------------------
+// Column pass for G = 512 (256-point images at oversampling 2), the same
+// register four-step with 512 = 32 x 16: element n = t + 32k of a column is
+// held by thread t (t = 0..31, k = 0..15); thread t takes the 16-point DFT
+// over k, multiplies by exp(-2 pi i t m1 / 512), the block transposes through
+// LDS once, and threads t < 16 (m1 = t) take the 32-point DFT over t (two
+// 16-point DFTs and a radix-2 step), giving output m1 + 16 m2.  Lane =
+// (column b, t), b fastest.  The upper half of the threads idles in the
+// second stage; the pass is bound by its global traffic, not its arithmetic.
+// ---------------------------------------------------------------------------
+#ifndef PF_FFT512
+#define PF_FFT512 1  // register four-step column passes for 512-point axes
+#endif
+constexpr int C512_B = 16;  // columns per block (512 threads)
+
+template <bool FWD, int B>
+__global__ void __launch_bounds__(32 * B) k_fftcol512(int N, int64_t I, const float2* __restrict__ twg,
+                                                    const float2* __restrict__ in, float2* __restrict__ out) {
+    constexpr int G = 512;
+    constexpr int PLANE = 32 * B + 8;  // LDS plane stride (float2) of the transpose
+    __shared__ float2 sh[16 * PLANE];
+    __shared__ float2 tw[G];
+    const int b = threadIdx.x % B, t = threadIdx.x / B;
+    const int64_t nchunk = (I + B - 1) / B;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t o = blk / nchunk;
+    const int64_t i0 = (blk % nchunk) * B;
+    const bool bok = i0 + b < I;
+    for (int k = threadIdx.x; k < G; k += 32 * B) tw[k] = twg[k];
+    const int lo = N - N / 2, hi = G - N / 2;  // stored rows [lo, hi) carry no image data
+    float2 v[32];
+    if (FWD) {
+        const float2* src = in + o * (int64_t)N * I + i0 + b;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int n = t + 32 * k;  // stored row
+            const int p = n < lo ? n + N / 2 : n - G + N / 2;
+            v[k] = (bok && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
+        }
+    } else {
+        const float2* src = in + o * (int64_t)G * I + i0 + b;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float2 x = bok ? src[(int64_t)(t + 32 * k) * I] : make_float2(0.f, 0.f);
+            v[k] = make_float2(x.x, -x.y);
+        }
+    }
+    dft16(v);
+    __syncthreads();  // twiddle table
+#pragma unroll
+    for (int m1 = 1; m1 < 16; ++m1) v[m1] = c_mul(v[m1], tw[t * m1]);
+#pragma unroll
+    for (int m1 = 0; m1 < 16; ++m1) sh[m1 * PLANE + t * B + b] = v[m1];
+    __syncthreads();
+    if (t >= 16 || !bok) return;
+    // thread m1 = t: Y_t'[m1] for t' = 0..31, even t' into v[0..15], odd into v[16..31]
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        v[s] = sh[t * PLANE + (2 * s) * B + b];
+        v[16 + s] = sh[t * PLANE + (2 * s + 1) * B + b];
+    }
+    dft16(v);
+    dft16(v + 16);
+    // X[t + 16 q] = E[q] + W32^q O[q], X[t + 16 (q + 16)] = E[q] - W32^q O[q]
+    if (FWD) {
+        float2* dst = out + o * (int64_t)G * I + i0 + b;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const float2 w = c_mul(v[16 + q], tw[16 * q]);
+            dst[(int64_t)(t + 16 * q) * I] = c_add(v[q], w);
+            dst[(int64_t)(t + 16 * (q + 16)) * I] = c_sub(v[q], w);
+        }
+    } else {
+        float2* dst = out + o * (int64_t)N * I + i0 + b;
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int n = t + 16 * q;  // stored output index -> cropped row p
+            if (n < lo || n >= hi) {
+                const float2 w = c_mul(v[16 + (q & 15)], tw[16 * (q & 15)]);
+                const float2 x = q < 16 ? c_add(v[q & 15], w) : c_sub(v[q & 15], w);
+                const int p = n < lo ? n + N / 2 : n - G + N / 2;
+                dst[(int64_t)p * I] = make_float2(x.x, -x.y);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // X pass for G = 256 and one B0 segment (any subspace K and batch): the same
 // register four-step FFT, 16 columns = (image row, k*Cv + cv) per block.  The
 // image side is x-contiguous, the grid side has the columns innermost, so the
@@ -1026,6 +1113,19 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
 static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, const float2* src, float2* dst,
                       hipStream_t s) {
     const pf_fftk_axis& X = p->fftk->ax[axis];
+    if (X.G == 512 && PF_FFT512) {
+        const int64_t nblk = O * ((I + C512_B - 1) / C512_B);
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+        if (nblk < 1) return PF_OK;
+        if (fwd)
+            hipLaunchKernelGGL((k_fftcol512<true, C512_B>), dim3((unsigned)nblk), dim3(32 * C512_B), 0, s, X.N, I,
+                               X.tw, src, dst);
+        else
+            hipLaunchKernelGGL((k_fftcol512<false, C512_B>), dim3((unsigned)nblk), dim3(32 * C512_B), 0, s, X.N, I,
+                               X.tw, src, dst);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
     if (X.G == 256) {
         const int B = fwd ? C256_BF : C256_BA;
         const int64_t nblk = O * ((I + B - 1) / B);

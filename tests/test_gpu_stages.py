@@ -1,6 +1,7 @@
 """Stage-level GPU parity through the C ABI: the fused deapodise/pad/FFT and
 inverse FFT/crop/deapodise against the oracle's centred-FFT grid, for
-1-D/2-D/3-D grids covering every radix of the fused FFT (8, 5, 4, 3, 2)."""
+1-D/2-D/3-D grids covering every radix of the fused FFT (8, 5, 4, 3, 2) and
+the 256- and 512-point register column passes."""
 
 import ctypes
 
@@ -35,6 +36,10 @@ CASES = [
     ((128, 24, 20), 2.0),  # 256 x 48 x 40: z is a 256-point column pass
     ((12, 20, 128), 2.0),  # 24 x 40 x 256: x is a 256-point row pass
     ((10, 128), 2.0),      # 20 x 256
+    ((256, 40), 2.0),      # 512 x 80: y is a 512-point column pass
+    ((200, 25), 2.56),     # 512 x 64: 200 of the 512 y rows carry data
+    ((256, 12, 10), 2.0),  # 512 x 24 x 20: z is a 512-point column pass
+    ((12, 256, 10), 2.0),  # 24 x 512 x 20: y is a 512-point column pass
 ]
 
 
