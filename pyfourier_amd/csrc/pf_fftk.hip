@@ -728,6 +728,216 @@ __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g
     }
 }
 
+// ---------------------------------------------------------------------------
+// X pass for G = 512 with any number of B0 segments L (<= X512_B) and any
+// subspace K: the register four-step FFT of k_fftcol512 (32 threads per
+// column, 16-point DFTs, one LDS transpose, a 32-point DFT by the lower 16
+// threads), X512_B columns per block.
+//
+// Forward: block columns are X512_B consecutive grid columns (row, v) with
+// v = l*K*Cv + kc, so every output row n is one contiguous run; a column
+// loads its image row (deapodised, times the segment's B0 map) with the 32
+// threads on consecutive x.
+// Adjoint: a block takes all L segments of KC (k, cv) pairs (YG rows when
+// L*K*Cv is smaller than the block), loads with the columns on consecutive
+// lanes, and after the FFT sums the segments (times the conjugate maps)
+// through LDS, so the image rows are written once, x on consecutive lanes.
+// LDS strides: ds_read/write_b64 serve 32 lanes per cycle; the forward's
+// transpose writes have t on consecutive lanes (odd row stride) and its reads
+// columns (plane stride = 16 mod 32), the adjoint's writes columns (row stride
+// 16 with two rows per 32 lanes) and its reads m1 (plane stride = 2 mod 32).
+// ---------------------------------------------------------------------------
+constexpr int X512_B = 16;
+
+template <bool FWD>
+struct X512Strides {
+    static constexpr int ROW = FWD ? X512_B + 1 : X512_B;
+    static constexpr int PLANE = FWD ? 32 * ROW + 16 : 32 * ROW + 2;
+};
+
+// image plane base (x = 0 of the row) of (row, kc); zy = the row's z*Ny + y
+__device__ __forceinline__ int64_t x_image_row(const Geo& g, const Batch& bt, int64_t row, int64_t nrows, int kc,
+                                               const float* apod, float& ayz, int64_t& zy) {
+    int fg, cb;
+    x_row_decode(bt, row, nrows, fg, cb, zy);
+    const int k = kc / bt.Cv, cv = kc % bt.Cv;
+    const int z = (int)(zy / g.N[1]), y = (int)(zy % g.N[1]);
+    ayz = apod[z] * apod[g.N[0] + y];
+    const int ii = bt.nfg > 1 ? fg : (bt.nimg > 1 ? k : 0);
+    const int c = cb * bt.Cv + cv;
+    return ((int64_t)c * bt.nimg + ii) * g.ntot + zy * g.N[2];
+}
+
+__global__ void __launch_bounds__(32 * X512_B) k_fftx512_fwd(Geo g, Batch bt, const float2* __restrict__ twg,
+                                                            const float* __restrict__ apod,
+                                                            const float2* __restrict__ in,
+                                                            float2* __restrict__ out) {
+    constexpr int G = 512, BX = X512_B;
+    constexpr int ROW = X512Strides<true>::ROW, PLANE = X512Strides<true>::PLANE;
+    __shared__ float2 sh[16 * PLANE];
+    __shared__ float2 tw[G];
+    const int N = g.N[2];
+    const int RC = bt.R * bt.Cv;
+    const int KCv = bt.K * bt.Cv;
+    const int64_t nrows = (int64_t)g.N[0] * g.N[1];
+    const int64_t ncols = (int64_t)bt.nfg * bt.Cb * nrows * RC;
+    const int64_t c0 = pf_xcd_remap(blockIdx.x, gridDim.x) * BX;
+    const float* apx = apod + g.N[0] + g.N[1];
+    for (int k = threadIdx.x; k < G; k += 32 * BX) tw[k] = twg[k];
+    const int lo = N - N / 2, hi = G - N / 2;  // stored positions [lo, hi) hold no image data
+    float2 v[32];
+    {
+        const int t = threadIdx.x & 31, b = threadIdx.x >> 5;
+        const int64_t gcol = c0 + b;
+        const bool ok = gcol < ncols;
+        const int64_t row = ok ? gcol / RC : 0;
+        const int vr = ok ? (int)(gcol % RC) : 0;
+        const int l = vr / KCv, kc = vr % KCv;
+        float ayz = 0.f;
+        int64_t zy = 0;
+        const int64_t base = x_image_row(g, bt, row, nrows, kc, apod, ayz, zy);
+        const float2* b0 = bt.b0s ? bt.b0s + (int64_t)l * g.ntot + zy * N : nullptr;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int n = t + 32 * k;
+            const int p = n < lo ? n + N / 2 : n - G + N / 2;
+            float2 x = make_float2(0.f, 0.f);
+            if (ok && (n < lo || n >= hi)) {
+                x = c_scale(in[base + p], ayz * apx[p]);
+                if (b0) x = c_mul(x, b0[p]);
+            }
+            v[k] = x;
+        }
+        dft16(v);
+        __syncthreads();  // twiddle table
+#pragma unroll
+        for (int m1 = 1; m1 < 16; ++m1) v[m1] = c_mul(v[m1], tw[t * m1]);
+#pragma unroll
+        for (int m1 = 0; m1 < 16; ++m1) sh[m1 * PLANE + t * ROW + b] = v[m1];
+    }
+    __syncthreads();
+    if (threadIdx.x >= 16 * BX) return;
+    const int m1 = threadIdx.x / BX, b = threadIdx.x % BX;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        v[s] = sh[m1 * PLANE + (2 * s) * ROW + b];
+        v[16 + s] = sh[m1 * PLANE + (2 * s + 1) * ROW + b];
+    }
+    dft16(v);
+    dft16(v + 16);
+    const int64_t gcol = c0 + b;
+    if (gcol >= ncols) return;
+    float2* dst = out + (gcol / RC) * G * RC + gcol % RC;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const float2 w = c_mul(v[16 + q], tw[16 * q]);
+        dst[(int64_t)(m1 + 16 * q) * RC] = c_add(v[q], w);
+        dst[(int64_t)(m1 + 16 * (q + 16)) * RC] = c_sub(v[q], w);
+    }
+}
+
+__global__ void __launch_bounds__(32 * X512_B) k_fftx512_adj(Geo g, Batch bt, const float2* __restrict__ twg,
+                                                            const float* __restrict__ apod,
+                                                            const float2* __restrict__ in,
+                                                            float2* __restrict__ img, XBlk xb, float factor) {
+    constexpr int G = 512, BX = X512_B;
+    constexpr int ROW = X512Strides<false>::ROW, PLANE = X512Strides<false>::PLANE;
+    static_assert(16 * PLANE >= BX * G, "segment rows must fit the transpose buffer");
+    __shared__ float2 sh[16 * PLANE];
+    __shared__ float2 tw[G];
+    const int N = g.N[2];
+    const int RC = bt.R * bt.Cv;
+    const int KCv = bt.K * bt.Cv;
+    const int L = bt.L;
+    const int VC = L * xb.KC;                  // columns per image row
+    const int NC = xb.YG * xb.KC;              // (row, kc) image rows per block
+    // segment row stride: padded when it fits, so two columns of a half-wave use different banks
+    const int NS = BX * (N + 16) <= 16 * PLANE ? N + 16 : N;
+    const int64_t nrows = (int64_t)g.N[0] * g.N[1];
+    const int64_t rows_tot = (int64_t)bt.nfg * bt.Cb * nrows;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t row0 = (blk / xb.nkc) * xb.YG;
+    const int kc0 = (int)(blk % xb.nkc) * xb.KC;
+    const float* apx = apod + g.N[0] + g.N[1];
+    for (int k = threadIdx.x; k < G; k += 32 * BX) tw[k] = twg[k];
+    const int lo = N - N / 2, hi = G - N / 2;
+    // column b -> (image row, segment, kc)
+    auto column = [&](int b, int64_t& row, int& l, int& kc) {
+        const int yy = b / VC, j = b % VC;
+        l = j / xb.KC;
+        kc = kc0 + j % xb.KC;
+        row = row0 + yy;
+        return yy < xb.YG && row < rows_tot && kc < KCv;
+    };
+    float2 v[32];
+    {
+        const int b = threadIdx.x % BX, t = threadIdx.x / BX;
+        int64_t row;
+        int l, kc;
+        const bool ok = column(b, row, l, kc);
+        const float2* src = in + (ok ? row : 0) * G * RC + l * KCv + kc;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float2 x = ok ? src[(int64_t)(t + 32 * k) * RC] : make_float2(0.f, 0.f);
+            v[k] = make_float2(x.x, -x.y);  // inverse via the forward FFT of the conjugate
+        }
+        dft16(v);
+        __syncthreads();  // twiddle table
+#pragma unroll
+        for (int m1 = 1; m1 < 16; ++m1) v[m1] = c_mul(v[m1], tw[t * m1]);
+#pragma unroll
+        for (int m1 = 0; m1 < 16; ++m1) sh[m1 * PLANE + t * ROW + b] = v[m1];
+    }
+    __syncthreads();
+    const bool act = threadIdx.x < 16 * BX;
+    const int m1 = threadIdx.x & 15, b = (threadIdx.x >> 4) % BX;
+    if (act) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            v[s] = sh[m1 * PLANE + (2 * s) * ROW + b];
+            v[16 + s] = sh[m1 * PLANE + (2 * s + 1) * ROW + b];
+        }
+        dft16(v);
+        dft16(v + 16);
+    }
+    __syncthreads();  // the transpose buffer becomes the segment buffer S[(yy, kcl)][l][p]
+    if (act) {
+        int64_t row;
+        int l, kc;
+        if (column(b, row, l, kc)) {
+            float2* S = sh + ((int64_t)((b / VC) * xb.KC + (kc - kc0)) * L + l) * NS;
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                const int n = m1 + 16 * q;
+                if (n < lo || n >= hi) {
+                    const float2 w = c_mul(v[16 + (q & 15)], tw[16 * (q & 15)]);
+                    const float2 x = q < 16 ? c_add(v[q & 15], w) : c_sub(v[q & 15], w);
+                    S[n < lo ? n + N / 2 : n - G + N / 2] = make_float2(x.x, -x.y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < NC * N; it += 32 * BX) {
+        const int p = it % N, cc = it / N;
+        const int yy = cc / xb.KC, kc = kc0 + cc % xb.KC;
+        const int64_t row = row0 + yy;
+        if (row >= rows_tot || kc >= KCv) continue;
+        float ayz = 0.f;
+        int64_t zy = 0;
+        const int64_t base = x_image_row(g, bt, row, nrows, kc, apod, ayz, zy);
+        const float2* S = sh + (int64_t)cc * L * NS + p;
+        float2 acc = make_float2(0.f, 0.f);
+        if (bt.b0s) {
+            const float2* b0 = bt.b0s + zy * N + p;
+            for (int l = 0; l < L; ++l) acc = c_add(acc, c_mulc(S[l * NS], b0[(int64_t)l * g.ntot]));
+        } else {
+            for (int l = 0; l < L; ++l) acc = c_add(acc, S[l * NS]);
+        }
+        img[base + p] = c_scale(acc, factor * ayz * apx[p]);
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // Cartesian passes (masked centred FFT without basis or B0): the same FFTs
@@ -1081,6 +1291,30 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         else
             hipLaunchKernelGGL((k_fftx256<false, X256_BA>), dim3((unsigned)nblk), dim3(16 * X256_BA), 0, s, g, t,
                                X.tw, p->apod, src, dst, KC, YG, nkc, factor);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    if (X.G == 512 && PF_FFT512 && b.L <= X512_B) {
+        Geo g = geo_of(p);
+        Batch t = batch_of(b);
+        const int64_t rows = (int64_t)b.nfg * b.Cb * p->shape[0] * p->shape[1];
+        if (fwd) {
+            const int64_t nblk = (rows * b.R * b.Cv + X512_B - 1) / X512_B;
+            if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+            if (nblk < 1) return PF_OK;
+            hipLaunchKernelGGL(k_fftx512_fwd, dim3((unsigned)nblk), dim3(32 * X512_B), 0, s, g, t, X.tw, p->apod,
+                               src, dst);
+        } else {
+            XBlk xb;  // all L segments of KC (k, cv) pairs per block, YG rows when they fit
+            xb.KC = std::min(KCv, std::max(1, X512_B / b.L));
+            xb.YG = xb.KC == KCv ? std::max(1, X512_B / (b.L * KCv)) : 1;
+            xb.nkc = (KCv + xb.KC - 1) / xb.KC;
+            const int64_t nblk = ((rows + xb.YG - 1) / xb.YG) * xb.nkc;
+            if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+            if (nblk < 1) return PF_OK;
+            hipLaunchKernelGGL(k_fftx512_adj, dim3((unsigned)nblk), dim3(32 * X512_B), 0, s, g, t, X.tw, p->apod,
+                               src, dst, xb, factor);
+        }
         PF_HIP(hipGetLastError());
         return PF_OK;
     }

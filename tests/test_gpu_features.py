@@ -212,3 +212,32 @@ def test_subspace_and_frames_256_point_rows(cuda_device):
     xf = demo.complex_gaussian(rng, (C, F) + shape)
     yf = pf.nufft(torch.from_numpy(xf).to(cuda_device), nufft_plan=plan, device=cuda_device)
     assert rel(yf.cpu().numpy().reshape(C, F, -1), R.nufft(xf, P)) < PARITY
+
+
+def test_b0_with_frames_and_subspace_512_point_rows(cuda_device):
+    """x of 256 at sigma 2 runs the 512-point register row FFT with B0
+    segments, per frame and with a subspace basis.  3 coils: the forward's
+    16-column blocks span grid rows (15 or 30 values per grid point); with
+    the basis the adjoint splits the 6 (k, coil) pairs over two blocks, each
+    summing all 5 segments."""
+    rng = np.random.default_rng(22)
+    shape = (10, 256)
+    F, K, C, L = 3, 2, 3, 5
+    coord = (rng.uniform(-0.5, 0.5, (F, 2, 150, 2)) * np.array(shape[::-1])).astype(np.float32)
+    coord[0, 0, 0] = [-128, -5]
+    zmap = demo.smooth_field_map(rng, shape, 100.0)
+    t = np.linspace(0, 8e-3, 150).astype(np.float32)
+    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=2.0, zmap=torch.from_numpy(zmap),
+                         T=torch.from_numpy(t), L=L, nbins=40)
+    b0 = (plan.zmap_t_kernel.cpu().numpy().astype(np.complex128),
+          plan.zmap_s_kernel.cpu().numpy().astype(np.complex128))
+    P = R.Plan(coord, shape, width=4, oversamp=2.0)
+    basis = demo.t2_subspace_basis(rng, F, K)
+    for bas in (None, basis):
+        tb = None if bas is None else torch.from_numpy(bas)
+        x = demo.complex_gaussian(rng, (C, F if bas is None else K) + shape)
+        y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, basis=tb, device=cuda_device)
+        assert rel(y.cpu().numpy().reshape(C, F, -1), R.nufft(x, P, basis=bas, b0=b0)) < PARITY
+        ya = demo.complex_gaussian(rng, (C, F) + coord.shape[1:-1])
+        xa = pf.nufft_adj(torch.from_numpy(ya).to(cuda_device), nufft_plan=plan, basis=tb, device=cuda_device)
+        assert rel(xa.cpu().numpy(), R.nufft_adj(ya.reshape(C, F, -1), P, basis=bas, b0=b0)) < PARITY

@@ -1,7 +1,7 @@
 """Stage-level GPU parity through the C ABI: the fused deapodise/pad/FFT and
 inverse FFT/crop/deapodise against the oracle's centred-FFT grid, for
 1-D/2-D/3-D grids covering every radix of the fused FFT (8, 5, 4, 3, 2) and
-the 256- and 512-point register column passes."""
+the 256- and 512-point register column and row passes."""
 
 import ctypes
 
@@ -40,6 +40,10 @@ CASES = [
     ((200, 25), 2.56),     # 512 x 64: 200 of the 512 y rows carry data
     ((256, 12, 10), 2.0),  # 512 x 24 x 20: z is a 512-point column pass
     ((12, 256, 10), 2.0),  # 24 x 512 x 20: y is a 512-point column pass
+    ((256,), 2.0),         # 512: x is a 512-point row pass
+    ((10, 256), 2.0),      # 20 x 512
+    ((8, 200), 2.56),      # 16 x 512: 200 of the 512 x points carry data
+    ((6, 8, 256), 2.0),    # 12 x 16 x 512
 ]
 
 
