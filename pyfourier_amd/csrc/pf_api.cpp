@@ -79,15 +79,21 @@ struct WsLayout {
     size_t grid, samples, fft, spread, fwd_total, adj_total;
 };
 
+// The adjoint of a B0 / basis batch whose R*C values form one vector sorts
+// them expanded (pf_spread_expand): its sorted samples are (Mtot, R*C) and its
+// spread workspace that of the plain R*C-item batch.  Forward and adjoint
+// share one layout.
 WsLayout layout(const pf_plan* p, const pf_batch& b) {
     WsLayout w;
+    pf_batch bx;
+    const bool ex = pf_spread_expand(p, b, &bx);
     w.grid = 0;
     w.samples = align_up(grid_bytes(p, b));
-    w.fft = w.samples + align_up(samples_bytes(p, b));
+    w.fft = w.samples + align_up(ex ? std::max(samples_bytes(p, b), samples_bytes(p, bx)) : samples_bytes(p, b));
     size_t extra = pf_fftk_supported(p) ? pf_fftk_extra_bytes(p, b) : 0;
     w.spread = w.fft + align_up(extra);
     w.fwd_total = w.spread;
-    w.adj_total = w.spread + align_up(pf_spread_partial_bytes(p, b));
+    w.adj_total = w.spread + align_up(pf_spread_partial_bytes(p, ex ? bx : b));
     return w;
 }
 
@@ -262,7 +268,10 @@ int pf_describe_launch(const pf_plan* p, const pf_batch_desc* d, char* buf, size
     if (rc == PF_OK) rc = make_batch(p, d, &ba);
     if (rc != PF_OK) return rc;
     if (!buf || cap == 0) return pf_fail(PF_ERR_ARG, "null output buffer");
-    const std::string text = "gather=" + pf_gather_name(p, bf) + " spread=" + pf_spread_name(p, ba) +
+    pf_batch bx;
+    const std::string spread =
+        pf_spread_expand(p, ba, &bx) ? "k_sort_x+" + pf_spread_name(p, bx) : pf_spread_name(p, ba);
+    const std::string text = "gather=" + pf_gather_name(p, bf) + " spread=" + spread +
                              " fft=" + (pf_fftk_supported(p) ? "fused" : "rocfft");
     const size_t n = std::min(cap - 1, text.size());
     std::copy(text.begin(), text.begin() + n, buf);
@@ -470,12 +479,19 @@ int pf_nufft_adj(pf_plan* p, const pf_batch_desc* d, const void* kspace, void* i
     double factor = 1.0 / p->scale;
     if (norm == PF_NORM_ORTHO) factor /= std::sqrt(ghat(p));
     if (norm == PF_NORM_NONE) factor /= ghat(p);
-    // the sort writes the spread's per-sample maxima when the spread uses them
-    bool smax_ready = false;
-    if ((rc = pf_launch_sort(p, b, static_cast<const float2*>(kspace), ys, weight, s, pf_spread_smax(p, b, part),
-                             &smax_ready)) != PF_OK)
-        return rc;
-    if ((rc = pf_launch_spread(p, b, ys, grid, part, s, smax_ready)) != PF_OK) return rc;
+    pf_batch bx;
+    if (pf_spread_expand(p, b, &bx)) {
+        // B0 segments / basis coefficients applied in the sort: a plain spread of R*C items
+        if ((rc = pf_launch_sort_expand(p, b, static_cast<const float2*>(kspace), ys, weight, s)) != PF_OK) return rc;
+        if ((rc = pf_launch_spread(p, bx, ys, grid, part, s)) != PF_OK) return rc;
+    } else {
+        // the sort writes the spread's per-sample maxima when the spread uses them
+        bool smax_ready = false;
+        if ((rc = pf_launch_sort(p, b, static_cast<const float2*>(kspace), ys, weight, s, pf_spread_smax(p, b, part),
+                                 &smax_ready)) != PF_OK)
+            return rc;
+        if ((rc = pf_launch_spread(p, b, ys, grid, part, s, smax_ready)) != PF_OK) return rc;
+    }
     if (pf_fftk_supported(p))
         return pf_fftk_adjoint(p, b, grid, static_cast<float2*>(image), extra, (float)factor, s);
     if ((rc = pf_fft_exec(p, b, grid, 1, s)) != PF_OK) return rc;

@@ -57,7 +57,9 @@ struct pf_slist {
 // waves of a spread launch above which the coarser split is used: a batch
 // with several value chunks per tile fills the GPU without splitting tiles
 // further, and every extra split costs a partial tile per value chunk
-constexpr int64_t PF_SPREAD_FILL = 8192;
+#ifndef PF_SPREAD_FILL
+#define PF_SPREAD_FILL 8192
+#endif
 
 static inline const pf_sitems& pf_sitems_for(const pf_slist& L, int nvch) {
     return (int64_t)L.split[0].nsitems * nvch >= PF_SPREAD_FILL ? L.split[0] : L.split[1];
@@ -178,6 +180,14 @@ int pf_launch_unsort(const pf_plan* p, const pf_batch& b, const float2* ys, floa
 // the sort can (*smax_written says whether it did)
 int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2* ys,
                    const float* weight, hipStream_t s, float* smax = nullptr, bool* smax_written = nullptr);
+// The fused adjoint's expanded form of a batch with B0 segments or a basis:
+// true (and *bx = the plain batch of R*C items on the same grid memory) when
+// all items form one vector (Cb = 1) and that plain batch takes the
+// pipelined spread.  pf_launch_sort_expand then writes the sorted samples as
+// (Mtot, R*C) with the coefficients applied, for pf_launch_spread(bx).
+bool pf_spread_expand(const pf_plan* p, const pf_batch& b, pf_batch* bx);
+int pf_launch_sort_expand(const pf_plan* p, const pf_batch& b, const float2* y, float2* ys, const float* weight,
+                          hipStream_t s);
 int pf_fft_exec(pf_plan* p, const pf_batch& b, float2* grid, int inverse, hipStream_t s);
 void pf_fft_destroy(pf_fft_cache* c);
 int pf_fft_cache_init(pf_plan* p);  // rocFFT setup + the plan's (empty) cache

@@ -746,6 +746,9 @@ constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 #ifndef PF_SPREAD_MFMA
 #define PF_SPREAD_MFMA 1  // plain 32-coil batches on 128-point 3-D tiles: matrix-core accumulation
 #endif
+#ifndef PF_SPREAD_EXPAND
+#define PF_SPREAD_EXPAND 1  // fused adjoint with B0 / basis: the sort expands the R*C values per sample
+#endif
 #ifndef PF_SPREAD_WREG
 #define PF_SPREAD_WREG 1  // an entry's z/y weights in registers (measured faster)
 #endif
@@ -2072,6 +2075,68 @@ __global__ void __launch_bounds__(TB) k_sort4(int64_t mtot, int C, const int32_t
     }
 }
 
+// user -> sorted with the reduction entries expanded (the fused adjoint with B0
+// segments or a subspace basis, all batch items in one vector): sorted row
+// iperm[j] gets the V = R*C values y[c, j] * w[j] * conj(coef(r)) in the
+// grid's value order r*C + c, coef(r = l*K + k) = Bt[l, t(j)] * basis[f(j), k]
+// as the on-the-fly spread forms them (sample_coef), so the spread sees a
+// plain batch of V items.  A block takes 2^qsh consecutive samples: coil-major
+// loads on consecutive samples, a transpose through LDS, the samples'
+// coefficients and destination rows in LDS, then whole sorted rows written
+// with 16-byte stores (pairs of values of one r; C even).  Divisions by V/2,
+// C and R are (n * ceil(2^32 / d)) >> 32, exact for n * d < 2^32 (the
+// launcher bounds V by 1024 and a block's samples by 32).
+__device__ __forceinline__ int udiv_m(int n, uint64_t m) { return (int)(((uint64_t)(unsigned)n * m) >> 32); }
+
+template <bool PAIR>
+__global__ void __launch_bounds__(TB) k_sort_x(GeoParams g, BatchParams B, int64_t mtot, int qsh, uint64_t m_out,
+                                               uint64_t m_c, uint64_t m_r, const int32_t* __restrict__ iperm,
+                                               const float* __restrict__ taprec, const float2* __restrict__ y,
+                                               const float* __restrict__ w, float2* __restrict__ ys) {
+    extern __shared__ float2 sm[];
+    const int QX = 1 << qsh;
+    const int C = B.C, CP = B.C + 1, R = B.R, V = B.R * B.C;
+    float2* t = sm;                                       // [QX][C + 1] weighted values
+    float2* cf = sm + QX * CP;                            // [QX][R] coefficients
+    int* srow = reinterpret_cast<int*>(cf + QX * R);      // [QX] sorted rows
+    const int64_t j0 = (int64_t)blockIdx.x << qsh;
+    const int nj = (int)min<int64_t>(QX, mtot - j0);
+    for (int i = threadIdx.x; i < QX * C; i += TB) {
+        const int c = i >> qsh, jl = i & (QX - 1);
+        if (jl < nj) {
+            const float a = w ? w[j0 + jl] : 1.0f;
+            const float2 v = y[(int64_t)c * mtot + j0 + jl];
+            t[jl * CP + c] = make_float2(v.x * a, v.y * a);
+        }
+    }
+    for (int i = threadIdx.x; i < nj * R; i += TB) {
+        const int jl = udiv_m(i, m_r), r = i - jl * R;
+        const int64_t j = j0 + jl;
+        const int s = iperm[j];
+        if (r == 0) srow[jl] = s;
+        const int f = reinterpret_cast<const int*>(taprec + (int64_t)s * g.rec)[3];
+        const int64_t tread = B.b0t ? (j % g.npts) % B.nt : 0;
+        cf[jl * R + r] = sample_coef(B, r, f, tread);
+    }
+    __syncthreads();
+    if (PAIR) {
+        const int VH = V >> 1;
+        for (int i = threadIdx.x; i < nj * VH; i += TB) {
+            const int jl = udiv_m(i, m_out), e = 2 * (i - jl * VH);
+            const int r = udiv_m(e, m_c), c = e - r * C;
+            const float2 k = cf[jl * R + r];
+            const float2 a = cmulc(t[jl * CP + c], k), b = cmulc(t[jl * CP + c + 1], k);
+            *reinterpret_cast<float4*>(ys + (int64_t)srow[jl] * V + e) = make_float4(a.x, a.y, b.x, b.y);
+        }
+    } else {
+        for (int i = threadIdx.x; i < nj * V; i += TB) {
+            const int jl = udiv_m(i, m_out), e = i - jl * V;
+            const int r = udiv_m(e, m_c), c = e - r * C;
+            ys[(int64_t)srow[jl] * V + e] = cmulc(t[jl * CP + c], cf[jl * R + r]);
+        }
+    }
+}
+
 // sorted -> user, times factor and the density weight
 __global__ void __launch_bounds__(TB) k_unsort4(int64_t mtot, int C, const int32_t* __restrict__ iperm,
                                                 const float2* __restrict__ ys, const float* __restrict__ w,
@@ -2574,6 +2639,50 @@ int pf_launch_sort(const pf_plan* p, const pf_batch& b, const float2* y, float2*
                            smax);
     else
         hipLaunchKernelGGL(k_sort, permute_grid(p, b), dim3(TB), 0, s, p->mtot, b.C, p->iperm, y, weight, ys);
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+}
+
+// the expanded sort's bounds: values per sample (its divisions are exact for
+// n * d < 2^32 with n < 32 * V) and samples per block
+constexpr int PF_EXPAND_VMAX = 1024;
+constexpr int PF_EXPAND_QSH = 5;  // 32 samples per block
+
+bool pf_spread_expand(const pf_plan* p, const pf_batch& b, pf_batch* bx) {
+    if (!PF_SPREAD_EXPAND || (!b.b0t && !b.basis) || b.Cb != 1 || b.R * b.C > PF_EXPAND_VMAX) return false;
+    pf_batch e = b;
+    e.C = e.Cv = b.R * b.C;
+    e.Cb = 1;
+    e.R = e.L = e.K = 1;
+    e.nt = 1;
+    e.basis = e.b0t = e.b0s = nullptr;
+    const RegChoice rc = spread_reg_choice(p, e);
+    if (!rc.fn || !spread_dma(p, e, rc)) return false;
+    if (bx) *bx = e;
+    return true;
+}
+
+static uint64_t magic_div(int d) { return ((1ull << 32) + (uint64_t)d - 1) / (uint64_t)d; }
+
+int pf_launch_sort_expand(const pf_plan* p, const pf_batch& b, const float2* y, float2* ys, const float* weight,
+                          hipStream_t s) {
+    const int V = b.R * b.C;
+    if (V > PF_EXPAND_VMAX) return pf_fail(PF_ERR_UNSUPPORTED, "too many values per sample to expand");
+    // samples per block: values, coefficients and rows within 32 KB of LDS
+    int qsh = PF_EXPAND_QSH;
+    auto lds = [&](int q) { return (size_t)q * ((b.C + 1 + b.R) * sizeof(float2) + sizeof(int)); };
+    while (qsh > 0 && lds(1 << qsh) > 32768) --qsh;
+    const int64_t nb = (p->mtot + (1 << qsh) - 1) >> qsh;
+    if (nb > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many samples");
+    if (nb < 1) return PF_OK;
+    const bool pair = b.C % 2 == 0;
+    const uint64_t m_out = magic_div(pair ? V / 2 : V), m_c = magic_div(b.C), m_r = magic_div(b.R);
+    if (pair)
+        hipLaunchKernelGGL(k_sort_x<true>, dim3((unsigned)nb), dim3(TB), lds(1 << qsh), s, geo(p), bparams(b), p->mtot,
+                           qsh, m_out, m_c, m_r, p->iperm, p->taprec, y, weight, ys);
+    else
+        hipLaunchKernelGGL(k_sort_x<false>, dim3((unsigned)nb), dim3(TB), lds(1 << qsh), s, geo(p), bparams(b), p->mtot,
+                           qsh, m_out, m_c, m_r, p->iperm, p->taprec, y, weight, ys);
     PF_HIP(hipGetLastError());
     return PF_OK;
 }

@@ -241,3 +241,64 @@ def test_b0_with_frames_and_subspace_512_point_rows(cuda_device):
         ya = demo.complex_gaussian(rng, (C, F) + coord.shape[1:-1])
         xa = pf.nufft_adj(torch.from_numpy(ya).to(cuda_device), nufft_plan=plan, basis=tb, device=cuda_device)
         assert rel(xa.cpu().numpy(), R.nufft_adj(ya.reshape(C, F, -1), P, basis=bas, b0=b0)) < PARITY
+
+
+def _expanded_case(cuda_device, shape, coord, C, L=1, basis=None, nt=None, seed=0):
+    """Adjoint of a B0 / subspace batch whose R*C values form whole 32-value
+    chunks: the sort writes them expanded and the spread runs the plain
+    pipelined (2-D) or matrix-core (3-D) kernel.  Checked against the oracle
+    and against the staged path (sort + on-the-fly spread, k_spread_reg)."""
+    from pyfourier_amd._nufft import _Call, stage_timer
+
+    rng = np.random.default_rng(seed)
+    kw = {}
+    if L > 1:
+        zmap = demo.smooth_field_map(rng, shape, 80.0)
+        kw = dict(zmap=torch.from_numpy(zmap), T=torch.from_numpy(np.linspace(0, 6e-3, nt).astype(np.float32)),
+                  L=L, nbins=40)
+    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=2.0, **kw)
+    P = R.Plan(coord, shape, width=4, oversamp=2.0)
+    b0 = None
+    if L > 1:
+        b0 = (plan.zmap_t_kernel.cpu().numpy().astype(np.complex128),
+              plan.zmap_s_kernel.cpu().numpy().astype(np.complex128))
+    tb = None if basis is None else torch.from_numpy(basis)
+    call = _Call(plan, cuda_device, tb, None, None)
+    what = call.describe(C, call.segments[0])
+    F = coord.shape[0] if basis is not None else 1
+    ya = demo.complex_gaussian(rng, (C,) + coord.shape[:-1])
+    yt = torch.from_numpy(ya).to(cuda_device)
+    xa = pf.nufft_adj(yt, nufft_plan=plan, basis=tb, device=cuda_device)
+    xr = R.nufft_adj(ya.reshape(C, F, -1), P, basis=basis, b0=b0)
+    e = rel(xa.cpu().numpy().reshape(xr.shape), xr)
+    with stage_timer():
+        xs = pf.nufft_adj(yt, nufft_plan=plan, basis=tb, device=cuda_device)
+    torch.cuda.synchronize(cuda_device)
+    es = rel(xa.cpu().numpy(), xs.cpu().numpy())
+    print(f"  {what}: oracle {e:.2e}, staged {es:.2e}", flush=True)
+    assert e < PARITY and es < 1e-6
+    return what
+
+
+def test_expanded_adjoint_b0_2d(cuda_device):
+    shape = (64, 64)
+    coord = demo.variable_density_spiral_2d(8, 512, 64)
+    what = _expanded_case(cuda_device, shape, coord, C=8, L=4, nt=512, seed=31)
+    assert "spread=k_sort_x+k_spread_pipe<" in what, what
+
+
+def test_expanded_adjoint_subspace_2d(cuda_device):
+    rng = np.random.default_rng(32)
+    F, K, N = 6, 4, 48
+    coord = np.stack([demo.golden_angle_radial_2d(2, 96) for _ in range(F)])
+    coord = (coord * rng.uniform(0.8, 1.0, (F, 1, 1, 1))).astype(np.float32)
+    basis = demo.t2_subspace_basis(rng, F, K)
+    what = _expanded_case(cuda_device, (N, N), coord, C=8, basis=basis, seed=33)
+    assert "spread=k_sort_x+k_spread_pipe<" in what, what
+
+
+def test_expanded_adjoint_b0_3d_matrix_core(cuda_device):
+    shape = (24, 24, 24)
+    coord = demo.golden_means_radial_3d(256, 48)
+    what = _expanded_case(cuda_device, shape, coord, C=8, L=4, nt=48, seed=34)
+    assert "spread=k_sort_x+k_tile_max+k_spread_mfma<" in what, what
