@@ -50,8 +50,10 @@ struct pf_slist {
     int32_t* slist = nullptr;    // sorted sample ids grouped by key
     int64_t nslist = 0;
     // [0] heavy tiles split; [1] split further so that the items alone fill
-    // the GPU (the same arrays when that changes nothing)
-    pf_sitems split[2];
+    // the GPU; [2] as [1] with twice the item size, for batches of several
+    // value chunks (each item is then that many waves).  Arrays are shared
+    // where a split changes nothing.
+    pf_sitems split[3];
 };
 
 // waves of a spread launch above which the coarser split is used: a batch
@@ -60,9 +62,13 @@ struct pf_slist {
 #ifndef PF_SPREAD_FILL
 #define PF_SPREAD_FILL 8192
 #endif
+#ifndef PF_SPREAD_MINCH
+#define PF_SPREAD_MINCH 64  // smallest work item of the fine split (entries)
+#endif
 
 static inline const pf_sitems& pf_sitems_for(const pf_slist& L, int nvch) {
-    return (int64_t)L.split[0].nsitems * nvch >= PF_SPREAD_FILL ? L.split[0] : L.split[1];
+    if ((int64_t)L.split[0].nsitems * nvch >= PF_SPREAD_FILL) return L.split[0];
+    return nvch > 1 ? L.split[2] : L.split[1];
 }
 
 // Gather column lists (3-D plans with one kernel width on all axes): the

@@ -753,12 +753,15 @@ constexpr int SPREAD_EB = 32;  // 11.7 KB of LDS per wave: 12 waves per CU fit
 #define PF_SPREAD_WREG 1  // an entry's z/y weights in registers (measured faster)
 #endif
 
-// 3-D tiles (few z/y rows) keep an entry's z/y weights in registers and are
-// compiled for >= 3 waves per SIMD (168 VGPRs); the 2-D tiles (long y
-// columns or x rows) would spill under that cap and read the weights from
-// LDS per row instead.
+// 3-D tiles (few z/y rows) and the 8-row 2-D tiles keep an entry's z/y
+// weights in registers and are compiled for >= 3 waves per SIMD (168 VGPRs;
+// the 32-value 8 x 16 instance spills 3-5 and still measured 3-10 us faster
+// on a 64-value 2-D spread than reading the weights from LDS per row); the
+// other 2-D tiles (longer y columns or x rows) and 1-D read them from LDS.
 template <int ZY>  // 100 * TZ + TY
-constexpr bool spread_wreg() { return PF_SPREAD_WREG && ZY / 100 > 1 && ZY / 100 + ZY % 100 <= 12; }
+constexpr bool spread_wreg() {
+    return PF_SPREAD_WREG && (ZY / 100 > 1 || ZY == 108) && ZY / 100 + ZY % 100 <= 12;
+}
 
 // VL values per wave (32, 16 or 8 for small batches): lane = (value, part),
 // 64 / VL parts; a part is (z part, x part): ZP z parts of TZ / ZP rows (ZP > 1
