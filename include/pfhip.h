@@ -225,7 +225,11 @@ int pf_spread(const pf_plan* plan, const pf_batch_desc* desc, const void* ksp_so
 int pf_spread_workspace_size(const pf_plan* plan, const pf_batch_desc* desc, size_t* bytes);
 
 /* Byte offsets of the parts of the pf_nufft / pf_nufft_adj workspace:
- * [grid | tile-sorted samples | FFT scratch | spread partial tiles]. */
+ * [grid | tile-sorted samples | FFT scratch | spread partial tiles].  The
+ * samples part holds (F*npts, C) for the stage entry points; for a batch
+ * with B0 segments or a basis whose nseg*ncoeff*batch values per grid point
+ * form one vector (batch <= 64) it is sized (F*npts, nseg*ncoeff*batch),
+ * which pf_nufft_adj fills with the coefficient-applied values. */
 int pf_workspace_offsets(const pf_plan* plan, const pf_batch_desc* desc, size_t* grid_off,
                          size_t* samples_off, size_t* fft_off, size_t* spread_off);
 
