@@ -636,6 +636,24 @@ def bench_cartesian(pf, dev, steps):
     return out
 
 
+def bench_voronoi(pf, dev, steps):
+    """Voronoi density compensation on the GPU (pyfourier_amd.voronoi): the
+    reference's own use, one 4096-sample variable-density spiral interleaf
+    rotated into a 48-shot full disk (196,608 points, 48 of them the k-space
+    centre), and 196,608 uniform random points (cell areas only)."""
+    import torch
+
+    from pyfourier_amd import demo
+
+    k = torch.from_numpy(demo.variable_density_spiral_2d(1, 4096, 256)).to(dev)
+    pts = torch.rand((196608, 2), dtype=torch.float64, device=dev, generator=torch.Generator(device=dev).manual_seed(5))
+    n = max(1, min(steps, 5))
+    t_disk = _time(lambda: pf.voronoi.voronoi(k, nshots=48, fullspoke=True), dev, n, warmup=1)
+    t_uni = _time(lambda: pf.voronoi.voronoi_areas(pts, dev), dev, n, warmup=1)
+    return {"workload": "2-D Voronoi DCF: spiral 48 x 4096 full disk; 196,608 uniform points (areas)",
+            "full_disk_ms": round(t_disk, 1), "uniform_areas_ms": round(t_uni, 1)}
+
+
 def bench_grog_sparse(pf, dev, steps):
     """GROG of a 16-slice 2-D radial acquisition (402 spokes x 512 samples,
     32 coils, per-slice operators from 24^2 calibration patches, 11 shift
@@ -775,7 +793,7 @@ def bench_autograd(pf, dev, steps):
 EXTRAS = (("cfg1_latency", bench_cfg1_latency), ("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
           ("autograd_b0", bench_autograd), ("autograd_3d", bench_autograd_3d), ("toeplitz", bench_toeplitz),
           ("cartesian", bench_cartesian),
-          ("grog_sparse", bench_grog_sparse))
+          ("grog_sparse", bench_grog_sparse), ("voronoi", bench_voronoi))
 
 
 def main():

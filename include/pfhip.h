@@ -383,6 +383,28 @@ int pf_grog_index(const float* coord, int ndim, const int* shape, long long npts
 int pf_grog_apply(const void* in, void* out, int batch, int ncoil, long long npts, int ndim, int nsteps,
                   const void* tables, int table_batch, const int* lut, void* stream);
 
+/* ---- Voronoi density compensation ------------------------------------ */
+
+/* Area of every 2-D Voronoi cell, 0 for open cells (points on the convex
+ * hull).  Replaces the tessellation and cell areas of voronoi's _voronoi /
+ * _voronoin / _compute_area, src/pyfourier/_dcomp/voronoi.py:220-258,277-343
+ * (scipy.spatial.Voronoi + the polygon area), for distinct points.
+ * pts:          (npoints, 2) float64, sorted by bucket of a uniform grid of
+ *               nx * ny square buckets of side h with origin (x0, y0)
+ *               (bucket = min(floor((y-y0)/h), ny-1)*nx + min(floor((x-x0)/h), nx-1));
+ * bucket_start: (nx*ny + 1) int32 first point of each bucket, last = npoints;
+ * area:         (npoints) float64, in the sorted order.
+ * open:         (npoints) uint8 or NULL: 1 marks a point known to lie on the
+ *               convex hull (vertex or edge), given area 0 without a scan.
+ * list == NULL: every point, with a 64-vertex working polygon; a cell that
+ *   needs more vertices gets area -1.  list != NULL: the nlist listed
+ *   points again with work_vertices-vertex polygons (even, >= 4) in `work`
+ *   (nlist * work_vertices * 40 bytes, 16-byte aligned); a cell that still
+ *   does not fit sets *status to 1 (int32, cleared by the caller). */
+int pf_voronoi_area_2d(const double* pts, const int* bucket_start, int nx, int ny, double x0, double y0, double h,
+                       long long npoints, const unsigned char* open, const int* list, long long nlist, void* work,
+                       int work_vertices, double* area, int* status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,7 @@ INFN-MRI/pyfourier, src/pyfourier/__init__.py:9-16):
     plan_nufft, nufft, nufft_adj      forward / adjoint NUFFT (+ subspace, B0)
     plan_spgram, sparse_gram          Toeplitz-embedded Gram operator
     pipemenon.pipe_menon_dcf          iterative density compensation (also top level)
+    voronoi.voronoi                   Voronoi density compensation (2-D)
     plan_fft, fftn, ifftn             masked Cartesian FFT (+ subspace, B0)
     plan_gram, gram                   dense Cartesian Gram operator
     fft, ifft                         centred FFT utilities
@@ -29,6 +30,7 @@ from ._sparsegram import GramMatrix, plan_spgram, sparse_gram
 from ._batch import NUFFTBatch, plan_nufft_batch
 from . import pipemenon
 from .pipemenon import pipe_menon_dcf
+from . import voronoi
 from ._tune import autotune_plan, spread_tiles
 from ._fft import FFTPlan, fft, fftn, ifft, ifftn, plan_fft
 from ._gram import gram, plan_gram
@@ -50,6 +52,7 @@ __all__ = [
     "plan_nufft_batch",
     "pipe_menon_dcf",
     "pipemenon",
+    "voronoi",
     "autotune_plan",
     "spread_tiles",
     "FFTPlan",

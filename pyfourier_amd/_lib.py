@@ -114,6 +114,8 @@ _SIG = {
     "pf_ifft_crop": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, ctypes.c_float, _VP, _VP]),
     "pf_sort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, _VP]),
     "pf_unsort_samples": (_I, [_VP, ctypes.POINTER(BatchDesc), _VP, _VP, _VP, ctypes.c_float, _VP]),
+    "pf_voronoi_area_2d": (_I, [_VP, _VP, _I, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_longlong,
+                                _VP, _VP, ctypes.c_longlong, _VP, _I, _VP, _VP, _VP]),
 }
 
 EXPORTED = tuple(_SIG)

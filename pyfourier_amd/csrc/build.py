@@ -22,7 +22,7 @@ ROOT = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "libpfhip.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["pf_plan.hip", "pf_ops.hip", "pf_fftk.hip", "pf_toep.hip", "pf_grad.hip", "pf_cart.hip", "pf_grog.hip", "pf_noalloc.hip", "pf_fft.cpp", "pf_api.cpp"]
+SOURCES = ["pf_plan.hip", "pf_ops.hip", "pf_fftk.hip", "pf_toep.hip", "pf_grad.hip", "pf_cart.hip", "pf_grog.hip", "pf_dcf.hip", "pf_noalloc.hip", "pf_fft.cpp", "pf_api.cpp"]
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1,0 +1,253 @@
+"""Voronoi density compensation: drop-in voronoi.voronoi (the reference
+exports this module as pyfourier.voronoi, src/pyfourier/__init__.py:15,40).
+
+Mirrors src/pyfourier/_dcomp/voronoi.py:16-123 and its helpers.  The
+reference tessellates on the host with scipy.spatial.Voronoi (Qhull) and
+sums each bounded cell's polygon; here every cell is built on the GPU by
+half-plane clipping in double precision (pf_voronoi_area_2d), open cells
+(points on the convex hull) getting 0 as in the reference.  The steps around
+it run in torch on the same device: the full disk synthesised from the first
+shot (nshots > 1), repeated points, the spike clipping, the optional edge
+extrapolation, the normalisation and the k-space-centre scaling.
+
+Differences from the reference, each a defect fixed toward its stated intent:
+
+* repeated points: the reference's lookup of the copies compares unique-row
+  ids with point indices (voronoi.py:199-217); here every copy of a point
+  gets the cell area of that location divided by the number of copies
+  ("the weight is split evenly", voronoi.py:20);
+* complex coordinates are split into (real, imag) along the last axis (the
+  reference stacks them along the first, voronoi.py:369-371);
+* with nshots = 1 the k-space-centre index addresses the flattened weights
+  (the reference indexes the first axis of a multi-dimensional array).
+
+2-D trajectories only: the reference's 3-D branch measures cells with a
+triangle fan over Qhull's unordered region vertices (voronoi.py:302-309),
+which is not a cell volume, and its angular compensation indexes a
+per-shot axis list with the demo's axis letter (demo/_spiral.py:72).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._nufft import _to_torch
+from ._plan import resolve_device
+
+__all__ = ["voronoi", "analytical_dcf", "voronoi_areas"]
+
+_SPILL_VERTICES = 4096  # working polygon of the second pass (cells with more than 64 edges)
+
+
+def analytical_dcf(coord):
+    """|k| of each point (reference voronoi.py:125-141)."""
+    c = _to_torch(coord)
+    return (c.to(torch.float64 if c.dtype == torch.float64 else torch.float32) ** 2).sum(-1) ** 0.5
+
+
+_HULL_DIRECTIONS = 4096
+_MERGE = 1e-13  # points this close (relative to the extent) are one site, as in the kernel
+
+
+def _hull_points(p: torch.Tensor, extent: float) -> torch.Tensor:
+    """Mask of the points on the convex hull (vertices, and points on an edge
+    to within 1e-13 of the extent): their Voronoi cells are the unbounded
+    ones.  Hull vertices: the extreme points of 4096 directions, then
+    QuickHull steps (the point farthest outside an edge of the polygon so
+    far is a vertex too) until no point lies outside.  A point the mask
+    misses is still measured correctly by the kernel (it finds the cell open
+    after a longer scan); the mask holds no interior point."""
+    M = p.shape[0]
+    if M < 3:
+        return torch.ones(M, dtype=torch.bool, device=p.device)
+    ang = torch.arange(_HULL_DIRECTIONS, dtype=torch.float64, device=p.device) * (2 * math.pi / _HULL_DIRECTIONS)
+    U = torch.stack([torch.cos(ang), torch.sin(ang)], 1)  # (directions, 2)
+    step = max(1, (1 << 24) // M)  # directions per product: <= 128 MB of projections
+    pt = p.t().contiguous()
+    # (directions, M) products reduced along their contiguous rows
+    ext = torch.cat([(U[k:k + step] @ pt).argmax(1) for k in range(0, _HULL_DIRECTIONS, step)])
+    H = torch.unique(ext)
+    for _ in range(64):
+        hv = p[H]
+        c = hv.mean(0)
+        th = torch.atan2(hv[:, 1] - c[1], hv[:, 0] - c[0])
+        th, o = torch.sort(th)
+        H = H[o]
+        hv = hv[o]
+        nh = hv.shape[0]
+        if nh < 3:  # collinear point set: every cell is unbounded
+            return torch.ones(M, dtype=torch.bool, device=p.device)
+        # each point against the hull edge its direction from the centre falls on
+        tp = torch.atan2(p[:, 1] - c[1], p[:, 0] - c[0])
+        j = torch.searchsorted(th, tp.contiguous()) % nh
+        a, b = hv[(j - 1) % nh], hv[j]
+        ab = b - a
+        lab = torch.linalg.vector_norm(ab, dim=1)
+        # counter-clockwise edges: the interior is on the left, so > 0 is outside
+        dist = (ab[:, 1] * (p[:, 0] - a[:, 0]) - ab[:, 0] * (p[:, 1] - a[:, 1])) / lab
+        tol = _MERGE * extent
+        # QuickHull step: the point farthest outside an edge is a hull vertex
+        # missed by the directions (an outer angle narrower than their spacing)
+        far = torch.full((nh,), -math.inf, dtype=p.dtype, device=p.device).scatter_reduce(0, j, dist, "amax")
+        new = torch.nonzero((dist > tol) & (dist == far[j])).reshape(-1)
+        if new.numel() == 0:
+            break
+        H = torch.unique(torch.cat([H, new]))
+    open_ = dist.abs() <= tol
+    open_[H] = True
+    return open_
+
+
+def voronoi_areas(points, device=None) -> torch.Tensor:
+    """Voronoi cell area of each of the distinct 2-D points (M, 2), float64,
+    0 for open cells.  GPU only."""
+    p = _to_torch(points)
+    dev = resolve_device(device if device is not None else (p.device if p.device.type == "cuda" else "cuda"))
+    p = p.to(device=dev, dtype=torch.float64).reshape(-1, 2).contiguous()
+    M = p.shape[0]
+    out = torch.zeros(M, dtype=torch.float64, device=dev)
+    if M == 0:
+        return out
+    if not torch.isfinite(p).all():
+        raise ValueError("Voronoi points must be finite")
+    lo = p.min(0).values
+    hi = p.max(0).values
+    ext = (hi - lo).cpu().tolist()
+    span = max(ext[0], ext[1], 1e-300)
+    # square buckets holding about two points each
+    h = math.sqrt(max(ext[0], span * 1e-6) * max(ext[1], span * 1e-6) * 2.0 / M)
+    nx = int(ext[0] / h) + 1
+    ny = int(ext[1] / h) + 1
+    x0, y0 = (float(v) for v in lo.cpu().tolist())
+    bx = torch.clamp(torch.floor((p[:, 0] - x0) / h), 0, nx - 1).to(torch.int64)
+    by = torch.clamp(torch.floor((p[:, 1] - y0) / h), 0, ny - 1).to(torch.int64)
+    bucket = by * nx + bx
+    order = torch.argsort(bucket, stable=True)
+    ps = p[order].contiguous()
+    start = torch.zeros(nx * ny + 1, dtype=torch.int32, device=dev)
+    start[1:] = torch.cumsum(torch.bincount(bucket, minlength=nx * ny), 0).to(torch.int32)
+    area = torch.empty(M, dtype=torch.float64, device=dev)
+    L = _lib.lib()
+    st = _lib.stream_of(dev)
+    with torch.cuda.device(dev):
+        hull = _hull_points(p, h * max(nx, ny))[order].to(torch.uint8).contiguous()
+        _lib.check(L.pf_voronoi_area_2d(_lib.ptr(ps), _lib.ptr(start), nx, ny, x0, y0, h, M, _lib.ptr(hull), None,
+                                        0, None, 0, _lib.ptr(area), None, st))
+        spill = torch.nonzero(area < 0).reshape(-1).to(torch.int32)
+        if spill.numel() > 0:
+            work = torch.empty(spill.numel() * _SPILL_VERTICES * 40, dtype=torch.uint8, device=dev)
+            status = torch.zeros(1, dtype=torch.int32, device=dev)
+            _lib.check(L.pf_voronoi_area_2d(_lib.ptr(ps), _lib.ptr(start), nx, ny, x0, y0, h, M, None,
+                                            _lib.ptr(spill), spill.numel(), _lib.ptr(work), _SPILL_VERTICES,
+                                            _lib.ptr(area), _lib.ptr(status), st))
+            if int(status.item()) != 0:
+                raise RuntimeError(f"a Voronoi cell has more than {_SPILL_VERTICES} edges")
+    out[order] = area
+    return out
+
+
+def _make_tilt(nshots, fullspoke):
+    """Shot angles (reference voronoi.py:374-387)."""
+    inc = 2 * math.pi / nshots * (1.0 if fullspoke else 0.5)
+    return np.mod(np.arange(nshots) * inc, 2 * np.pi)
+
+
+def _rotz(angles):
+    """Rotations about z, float64 (reference _angleaxis2rotmat with u = z,
+    voronoi.py:390-429)."""
+    s, c = np.sin(angles), np.cos(angles)
+    z, o = np.zeros_like(c), np.ones_like(c)
+    return np.stack([np.stack([c, -s, z], -1), np.stack([s, c, z], -1), np.stack([z, z, o], -1)], 1)
+
+
+def _select_nth_desc(w, n):
+    """n-th largest value (reference _select_nth, voronoi.py:360-366)."""
+    return torch.sort(w, descending=True).values[n]
+
+
+def _clip_and_fix(w, pts, fix_edge, ratio):
+    """Spike clipping and the optional edge extrapolation on the distinct
+    points' areas (reference _voronoi after the cells, voronoi.py:236-258)."""
+    fufa = 0.98
+    while True:
+        r = 1 - (1 - ratio) * fufa
+        thld = _select_nth_desc(w, min(round(r * w.numel()), w.numel() - 1))
+        if torch.isfinite(thld):
+            break
+        fufa **= 2
+    w = torch.minimum(w, thld)
+    if fix_edge:
+        rho = (pts.to(torch.float32) ** 2).sum(-1)
+        good = rho > 0.6 * rho.max()
+        x = rho[good].double()
+        y = w[good].double()
+        a, b = float(x.min()), float(x.max())
+        # cubic least squares in the window [-1, 1] (numpy's Polynomial.fit)
+        def vander(t):
+            u = (2 * t - (a + b)) / (b - a) if b > a else t - a
+            return torch.stack([torch.ones_like(u), u, u * u, u * u * u], -1)
+        V = vander(x)
+        coef = torch.linalg.lstsq(V, y.unsqueeze(-1)).solution.squeeze(-1)
+        zero = w == 0
+        if zero.any():
+            w = w.clone()
+            w[zero] = (vander(rho[zero].double()) @ coef).to(w.dtype)
+    return w
+
+
+def _normalize_weight(w):
+    """Reference _normalize_weight (voronoi.py:186-194)."""
+    s = w.sum()
+    w = torch.where(torch.isclose(w, torch.zeros_like(w)), torch.ones_like(w), w)
+    inv = s / w
+    inv = inv / inv.sum()
+    inv = torch.where(torch.isclose(inv, torch.zeros_like(inv)), torch.ones_like(inv), inv)
+    return 1 / inv
+
+
+def voronoi(coord, nshots, fullspoke, rotation_axis_3D=None, fix_edge=False, ratio=0.1, device=None):
+    """Density compensation weights from the Voronoi cells of the trajectory
+    (reference voronoi.py:16-123).
+
+    coord: (..., nshots, npts, 2) real or (..., nshots, npts) complex; with
+    nshots > 1 the first shot is rotated to nshots copies about the k-space
+    centre (a full disk) and the weights of the first are returned.  Returns
+    float32 weights, numpy for numpy input, else a tensor on the GPU used.
+    """
+    c = _to_torch(coord)
+    backend_numpy = not isinstance(coord, torch.Tensor)
+    if device is None:
+        device = c.device if c.device.type == "cuda" else "cuda"
+    dev = resolve_device(device)
+    if torch.is_complex(c):
+        c = torch.stack([c.real, c.imag], -1)
+    ndim = c.shape[-1]
+    if ndim != 2:
+        raise NotImplementedError("Voronoi density compensation is built for 2-D trajectories")
+    # the caller's precision (float64 stays float64, as scipy sees it in the reference)
+    c = c.to(device=dev, dtype=torch.float64 if c.dtype == torch.float64 else torch.float32)
+    if nshots > 1:
+        kshot = c.reshape(-1, *c.shape[-2:])[0].double()  # (npts, 2)
+        k3 = torch.cat([kshot, torch.zeros_like(kshot[:, :1])], -1)  # in the xy plane
+        R = torch.from_numpy(_rotz(_make_tilt(nshots, fullspoke))).to(dev)
+        c = torch.einsum("bij,nj->bni", R, k3)[..., :ndim].to(torch.float32)  # (nshots, npts, 2)
+        k0 = int(torch.argmin((c[0] ** 2).sum(-1) ** 0.5))
+    else:
+        k0 = int(torch.argmin(((c ** 2).sum(-1) ** 0.5).reshape(-1)))
+    ishape = c.shape[:-1]
+    pts = c.reshape(-1, ndim)
+    uniq, inv, counts = torch.unique(pts, dim=0, return_inverse=True, return_counts=True)
+    w_u = voronoi_areas(uniq, dev).to(torch.float32)
+    w_u = _clip_and_fix(w_u, uniq, fix_edge, ratio)
+    w = (w_u / counts.to(torch.float32))[inv]
+    w = torch.nan_to_num(w, nan=0.0, posinf=0.0, neginf=0.0).reshape(ishape)
+    if nshots > 1:
+        w = w[0]
+    w = w.squeeze()
+    w = _normalize_weight(w)
+    w = w * (1 / w.reshape(-1)[k0] / nshots)
+    return w.cpu().numpy() if backend_numpy else w

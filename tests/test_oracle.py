@@ -669,3 +669,53 @@ def test_operator_restatement_rejects_wrong_rank():
         R.nufft_adj(np.zeros((2, P.npts), np.complex64), P)
     with pytest.raises(ValueError):
         R.nufft(np.zeros((2, 8, 8), np.complex64), P)
+
+
+# ---- Voronoi density compensation (oracle.voronoi) ------------------------
+
+def test_voronoi_oracle_square_grid_closed_form():
+    """6 x 6 grid of spacing 0.5: the 16 interior cells are 0.25, the 20 hull
+    cells open.  The pipeline then gives, by hand: clipping at the 4th largest
+    (round(0.118 * 36) = 4) leaves the areas; normalisation maps the open
+    cells to 1 and the others to 0.25 times the hull value, and the k-space
+    centre (the point at the origin, a hull point) scales that to 1."""
+    from oracle import voronoi as V
+
+    g = (np.stack(np.meshgrid(np.arange(6.0), np.arange(6.0)), -1).reshape(-1, 2) * 0.5).astype(np.float32)
+    a = V.cell_areas(g).reshape(6, 6)
+    interior = np.zeros((6, 6), bool)
+    interior[1:-1, 1:-1] = True
+    assert np.allclose(a[interior], 0.25, rtol=1e-12) and np.all(a[~interior] == 0)
+    w = V.voronoi(g, nshots=1, fullspoke=True).reshape(6, 6)
+    assert np.allclose(w[interior], 0.25, rtol=1e-6) and np.allclose(w[~interior], 1.0, rtol=1e-6)
+
+
+def test_voronoi_oracle_hexagonal_lattice():
+    """Interior cells of a triangular lattice of spacing a are regular
+    hexagons of area sqrt(3)/2 a^2."""
+    from oracle import voronoi as V
+
+    a = 0.3
+    pts = [((i + 0.5 * (j % 2)) * a, j * a * math.sqrt(3) / 2) for j in range(9) for i in range(9)]
+    p = np.asarray(pts)
+    ar = V.cell_areas(p).reshape(9, 9)
+    assert np.allclose(ar[2:-2, 2:-2], math.sqrt(3) / 2 * a * a, rtol=1e-10)
+
+
+def test_voronoi_oracle_duplicates_and_full_disk():
+    """Copies of a point share its cell evenly; the full disk is the first
+    shot rotated by 2 pi s / nshots (fullspoke) about the centre."""
+    from oracle import voronoi as V
+
+    g = (np.stack(np.meshgrid(np.arange(6.0), np.arange(6.0)), -1).reshape(-1, 2) * 0.5).astype(np.float32)
+    d = np.concatenate([g, g[14:15]])  # one interior point twice
+    w1 = V.voronoi(g, 1, True)
+    w2 = V.voronoi(d, 1, True)
+    # after normalisation the copies carry half the single point's relative weight
+    r1 = w1[14] / w1[7]
+    r2 = w2[14] / w2[7]
+    assert abs(w2[14] - w2[-1]) < 1e-7 and abs(r2 - r1 / 2) < 1e-5 * r1
+    # one spoke along +x, 4 full spokes: the second shot points along +y
+    spoke = np.stack([np.linspace(-1, 1, 9), np.zeros(9)], -1)[None].astype(np.float32)
+    w = V.voronoi(spoke, nshots=4, fullspoke=True)
+    assert w.shape == (9,) and abs(w[4] - 0.25) < 1e-6  # k-space centre scaled to 1/nshots
