@@ -118,7 +118,9 @@ def voronoi_areas(points, device=None) -> torch.Tensor:
     hi = p.max(0).values
     ext = (hi - lo).cpu().tolist()
     span = max(ext[0], ext[1], 1e-300)
-    # square buckets holding about two points each
+    # square buckets holding about two points each on average (finer buckets
+    # where the k-space centre crowds measured slower: every bucket a ring
+    # passes costs a dependent load of its bounds)
     h = math.sqrt(max(ext[0], span * 1e-6) * max(ext[1], span * 1e-6) * 2.0 / M)
     nx = int(ext[0] / h) + 1
     ny = int(ext[1] / h) + 1
