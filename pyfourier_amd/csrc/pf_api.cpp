@@ -479,21 +479,25 @@ int pf_nufft_adj(pf_plan* p, const pf_batch_desc* d, const void* kspace, void* i
     double factor = 1.0 / p->scale;
     if (norm == PF_NORM_ORTHO) factor /= std::sqrt(ghat(p));
     if (norm == PF_NORM_NONE) factor /= ghat(p);
+    // 3-D plans with a column mask: the spread leaves the empty tiles of
+    // columns no sample covers unwritten, and the inverse FFT takes those
+    // columns as zero without reading them
+    const bool masked = pf_fftk_supported(p) && pf_fftk_masked(p);
     pf_batch bx;
     if (pf_spread_expand(p, b, &bx)) {
         // B0 segments / basis coefficients applied in the sort: a plain spread of R*C items
         if ((rc = pf_launch_sort_expand(p, b, static_cast<const float2*>(kspace), ys, weight, s)) != PF_OK) return rc;
-        if ((rc = pf_launch_spread(p, bx, ys, grid, part, s)) != PF_OK) return rc;
+        if ((rc = pf_launch_spread(p, bx, ys, grid, part, s, false, masked)) != PF_OK) return rc;
     } else {
         // the sort writes the spread's per-sample maxima when the spread uses them
         bool smax_ready = false;
         if ((rc = pf_launch_sort(p, b, static_cast<const float2*>(kspace), ys, weight, s, pf_spread_smax(p, b, part),
                                  &smax_ready)) != PF_OK)
             return rc;
-        if ((rc = pf_launch_spread(p, b, ys, grid, part, s, smax_ready)) != PF_OK) return rc;
+        if ((rc = pf_launch_spread(p, b, ys, grid, part, s, smax_ready, masked)) != PF_OK) return rc;
     }
     if (pf_fftk_supported(p))
-        return pf_fftk_adjoint(p, b, grid, static_cast<float2*>(image), extra, (float)factor, s);
+        return pf_fftk_adjoint(p, b, grid, static_cast<float2*>(image), extra, (float)factor, s, masked);
     if ((rc = pf_fft_exec(p, b, grid, 1, s)) != PF_OK) return rc;
     return pf_launch_crop_deapod(p, b, grid, static_cast<float2*>(image), (float)factor, s);
 }

@@ -446,6 +446,22 @@ __device__ __forceinline__ void dft16(float2* v) {
     for (int m = 0; m < 16; ++m) v[m] = o[m];
 }
 
+// Dead grid columns of a 3-D plan (no sample's footprint covers the (y, x)
+// column) in the fused adjoint's inverse z pass (inner index
+// i = (gy*Gx + gx)*RC + rc): the spread leaves their empty tiles unwritten,
+// so a dead column is taken as zero without reading it (its zero output is
+// still written).  mode 0: every column read.  Skipping dead columns in the
+// y passes, or their stores in the forward, measured slower (per-row mask
+// reads, rows written with holes) or no faster on the 3-D radial headline.
+struct ColMask {
+    const uint8_t* m;
+    int mode, RC;
+};
+
+__device__ __forceinline__ bool cm_dead_col(const ColMask& cm, int64_t i) {
+    return cm.mode == 1 && !cm.m[i / cm.RC];
+}
+
 // Columns per block of the 256-point column passes: a block reads and writes
 // BC consecutive values of each row, so a larger BC gives longer contiguous
 // runs (BC x 8 bytes) per row at the price of LDS (16 x (16 BC + 8) float2).
@@ -463,7 +479,7 @@ template <bool FWD, int B>
 __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_fftcol256(int N, int64_t I,
                                                                          const float2* __restrict__ twg,
                                                                          const float2* __restrict__ in,
-                                                                         float2* __restrict__ out) {
+                                                                         float2* __restrict__ out, ColMask cm) {
     constexpr int G = 256;
     constexpr int C256_PLANE = 16 * B + 8;  // LDS plane stride (float2) of the transpose
     __shared__ float2 sh[16 * C256_PLANE];
@@ -474,6 +490,7 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_fftcol256(int N
     const int64_t o = blk / nchunk;
     const int64_t i0 = (blk % nchunk) * B;
     const bool bok = i0 + b < I;
+    const bool zero = !bok || cm_dead_col(cm, i0 + b);  // no column, or one known to be zero
     if (threadIdx.x < G) tw[threadIdx.x] = twg[threadIdx.x];
     const int lo = N - N / 2, hi = G - N / 2;  // stored rows [lo, hi) carry no image data
     float2 v[16];
@@ -483,13 +500,13 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_fftcol256(int N
         for (int k = 0; k < 16; ++k) {
             const int n = t + 16 * k;  // stored row
             const int p = n < lo ? n + N / 2 : n - G + N / 2;
-            v[k] = (bok && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
+            v[k] = (!zero && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
         }
     } else {
         const float2* src = in + o * (int64_t)G * I + i0 + b;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const float2 x = bok ? src[(int64_t)(t + 16 * k) * I] : make_float2(0.f, 0.f);
+            const float2 x = !zero ? src[(int64_t)(t + 16 * k) * I] : make_float2(0.f, 0.f);
             v[k] = make_float2(x.x, -x.y);
         }
     }
@@ -539,7 +556,8 @@ constexpr int C512_B = 16;  // columns per block (512 threads)
 
 template <bool FWD, int B>
 __global__ void __launch_bounds__(32 * B) k_fftcol512(int N, int64_t I, const float2* __restrict__ twg,
-                                                    const float2* __restrict__ in, float2* __restrict__ out) {
+                                                    const float2* __restrict__ in, float2* __restrict__ out,
+                                                    ColMask cm) {
     constexpr int G = 512;
     constexpr int PLANE = 32 * B + 8;  // LDS plane stride (float2) of the transpose
     __shared__ float2 sh[16 * PLANE];
@@ -550,6 +568,7 @@ __global__ void __launch_bounds__(32 * B) k_fftcol512(int N, int64_t I, const fl
     const int64_t o = blk / nchunk;
     const int64_t i0 = (blk % nchunk) * B;
     const bool bok = i0 + b < I;
+    const bool zero = !bok || cm_dead_col(cm, i0 + b);  // no column, or one known to be zero
     for (int k = threadIdx.x; k < G; k += 32 * B) tw[k] = twg[k];
     const int lo = N - N / 2, hi = G - N / 2;  // stored rows [lo, hi) carry no image data
     float2 v[32];
@@ -559,13 +578,13 @@ __global__ void __launch_bounds__(32 * B) k_fftcol512(int N, int64_t I, const fl
         for (int k = 0; k < 16; ++k) {
             const int n = t + 32 * k;  // stored row
             const int p = n < lo ? n + N / 2 : n - G + N / 2;
-            v[k] = (bok && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
+            v[k] = (!zero && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
         }
     } else {
         const float2* src = in + o * (int64_t)G * I + i0 + b;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const float2 x = bok ? src[(int64_t)(t + 32 * k) * I] : make_float2(0.f, 0.f);
+            const float2 x = !zero ? src[(int64_t)(t + 32 * k) * I] : make_float2(0.f, 0.f);
             v[k] = make_float2(x.x, -x.y);
         }
     }
@@ -1345,7 +1364,7 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
 }
 
 static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, const float2* src, float2* dst,
-                      hipStream_t s) {
+                      hipStream_t s, ColMask cm = ColMask{nullptr, 0, 1}) {
     const pf_fftk_axis& X = p->fftk->ax[axis];
     if (X.G == 512 && PF_FFT512) {
         const int64_t nblk = O * ((I + C512_B - 1) / C512_B);
@@ -1353,10 +1372,10 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
         if (nblk < 1) return PF_OK;
         if (fwd)
             hipLaunchKernelGGL((k_fftcol512<true, C512_B>), dim3((unsigned)nblk), dim3(32 * C512_B), 0, s, X.N, I,
-                               X.tw, src, dst);
+                               X.tw, src, dst, cm);
         else
             hipLaunchKernelGGL((k_fftcol512<false, C512_B>), dim3((unsigned)nblk), dim3(32 * C512_B), 0, s, X.N, I,
-                               X.tw, src, dst);
+                               X.tw, src, dst, cm);
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
@@ -1367,13 +1386,14 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
         if (nblk < 1) return PF_OK;
         if (fwd)
             hipLaunchKernelGGL((k_fftcol256<true, C256_BF>), dim3((unsigned)nblk), dim3(16 * C256_BF), 0, s, X.N, I,
-                               X.tw, src, dst);
+                               X.tw, src, dst, cm);
         else
             hipLaunchKernelGGL((k_fftcol256<false, C256_BA>), dim3((unsigned)nblk), dim3(16 * C256_BA), 0, s, X.N, I,
-                               X.tw, src, dst);
+                               X.tw, src, dst, cm);
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
+    if (cm.mode != 0) return pf_fail(PF_ERR_UNSUPPORTED, "column mask on an LDS FFT pass");
     AxisFFT A = axis_desc(X);
     const int B = cols_for(X.G);
     int bshift = 0;
@@ -1388,6 +1408,14 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
         hipLaunchKernelGGL(k_fftcol<false>, dim3((unsigned)nblk), dim3(FT), lds, s, A, I, B, bshift, src, dst);
     PF_HIP(hipGetLastError());
     return PF_OK;
+}
+
+// the inverse z pass of a 3-D plan with a column mask runs a register
+// kernel, so the fused adjoint may leave the dead columns unwritten
+bool pf_fftk_masked(const pf_plan* p) {
+    if (!PF_COLMASK || p->ndim != 3 || !p->colmask || !p->fftk) return false;
+    const int G = p->fftk->ax[0].G;
+    return G == 256 || (G == 512 && PF_FFT512);
 }
 
 // image -> grid (deapodise, pad, FFT); `extra` holds pf_fftk_extra_bytes
@@ -1409,9 +1437,11 @@ int pf_fftk_forward(pf_plan* p, const pf_batch& b, const float2* img, float2* gr
     return launch_col(p, 0, true, FC, (int64_t)p->os[1] * p->os[2] * RC, t2, grid, s);
 }
 
-// grid -> image (inverse FFT, crop, deapodise, times factor); the grid is consumed
+// grid -> image (inverse FFT, crop, deapodise, times factor); the grid is
+// consumed; masked: the z pass takes the dead (y, x) columns as zero
+// without reading them (the fused adjoint's spread leaves them unwritten)
 int pf_fftk_adjoint(pf_plan* p, const pf_batch& b, float2* grid, float2* img, float2* extra, float factor,
-                    hipStream_t s) {
+                    hipStream_t s, bool masked) {
     int rc = pf_fftk_ready(p);
     if (rc != PF_OK) return rc;
     const int64_t RC = (int64_t)b.R * b.Cv;
@@ -1423,7 +1453,8 @@ int pf_fftk_adjoint(pf_plan* p, const pf_batch& b, float2* grid, float2* img, fl
     }
     float2* t2 = extra;
     float2* t1 = grid;
-    if ((rc = launch_col(p, 0, false, FC, (int64_t)p->os[1] * p->os[2] * RC, grid, t2, s)) != PF_OK) return rc;
+    const ColMask cz{p->colmask, masked && pf_fftk_masked(p) ? 1 : 0, (int)RC};
+    if ((rc = launch_col(p, 0, false, FC, (int64_t)p->os[1] * p->os[2] * RC, grid, t2, s, cz)) != PF_OK) return rc;
     if ((rc = launch_col(p, 1, false, FC * p->shape[0], (int64_t)p->os[2] * RC, t2, t1, s)) != PF_OK) return rc;
     return launch_x(p, b, false, t1, img, factor, s);
 }

@@ -39,6 +39,11 @@ struct pf_fft_cache;
 struct pf_sitems {
     int4* sitems = nullptr;      // (key, e0, e1, slot or -1)
     int nsitems = 0;
+    // the same items without the empty tiles whose (y, x) columns no sample
+    // covers (3-D plans with a column mask; the fused adjoint, whose inverse
+    // FFT never reads those columns); null when there is no mask
+    int4* lsitems = nullptr;
+    int nlsitems = 0;
     int4* ritems = nullptr;      // (key, slot0, nslot, 0)
     int nritems = 0;
     int nslots = 0;
@@ -61,6 +66,9 @@ struct pf_slist {
 // further, and every extra split costs a partial tile per value chunk
 #ifndef PF_SPREAD_FILL
 #define PF_SPREAD_FILL 8192
+#endif
+#ifndef PF_COLMASK
+#define PF_COLMASK 1  // 3-D plans: skip the (y, x) grid columns no sample covers in the fused operators
 #endif
 #ifndef PF_SPREAD_MINCH
 #define PF_SPREAD_MINCH 64  // smallest work item of the fine split (entries)
@@ -123,6 +131,11 @@ struct pf_plan {
     int32_t* perm = nullptr;     // sorted -> original sample
     int32_t* iperm = nullptr;    // original -> sorted
     float* taprec = nullptr;     // (mtot, rec): int start[3], int frame, weights
+    // 3-D plans: (Gy, Gx) bytes, 1 where some sample's kernel footprint
+    // covers the (y, x) grid column (in any z and frame); the fused adjoint's
+    // spread leaves the empty tiles of the other columns unwritten and its
+    // inverse z pass takes them as zero (null otherwise)
+    uint8_t* colmask = nullptr;
     float* cos_sorted = nullptr; // (mtot, 3) oversampled coordinate (z, y, x)
     float* apod = nullptr;       // deapodisation vectors, Nz + Ny + Nx floats
     int64_t* foff = nullptr;     // ragged plans: frame f owns original samples [foff[f], foff[f+1])
@@ -175,8 +188,10 @@ int pf_launch_gather(const pf_plan* p, const pf_batch& b, const float2* grid, fl
                      hipStream_t s);
 // smax_ready: the sort already wrote the per-sample maxima the matrix-core
 // spread scales by (pf_spread_smax), else the spread computes them
+// live: only the items of pf_sitems::lsitems (the empty tiles of columns no
+// sample covers stay unwritten; for a masked inverse FFT)
 int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, float2* grid,
-                     float2* partial, hipStream_t s, bool smax_ready = false);
+                     float2* partial, hipStream_t s, bool smax_ready = false, bool live = false);
 // where the matrix-core spread keeps its per-sample maxima inside the spread
 // workspace, or null when the batch does not use it
 float* pf_spread_smax(const pf_plan* p, const pf_batch& b, float2* partial);
@@ -214,9 +229,12 @@ int pf_fftk_init(pf_plan* p);
 int pf_fftk_ready(const pf_plan* p);  // PF_OK when pf_fftk_init has run
 void pf_fftk_destroy(struct pf_fftk* f);
 size_t pf_fftk_extra_bytes(const pf_plan* p, const pf_batch& b);
+// masked (the fused adjoint, where pf_fftk_masked(p)): the inverse z pass
+// takes the (y, x) columns no sample covers as zero without reading them
+bool pf_fftk_masked(const pf_plan* p);
 int pf_fftk_forward(pf_plan* p, const pf_batch& b, const float2* img, float2* grid, float2* extra, hipStream_t s);
 int pf_fftk_adjoint(pf_plan* p, const pf_batch& b, float2* grid, float2* img, float2* extra, float factor,
-                    hipStream_t s);
+                    hipStream_t s, bool masked = false);
 // masked centred FFT (oversampling 1) on the layout (planes, Nz, Ny, Nx):
 // src -> dst, in place after the first pass; mask (mframes, Nz*Ny*Nx) or null
 int pf_fftk_cart(pf_plan* p, int64_t planes, int T, const float* mask, int mframes, bool fwd, const float2* src,

@@ -2556,7 +2556,7 @@ int pf_spread_reg_tiles(int ndim, int* tiles, int cap) {
 }
 
 int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, float2* grid, float2* partial,
-                     hipStream_t s, bool smax_ready) {
+                     hipStream_t s, bool smax_ready, bool live) {
     GeoParams g = geo(p);
     BatchParams B = bparams(b);
     const int VC = pf_spread_coil_chunk(p, b);
@@ -2565,7 +2565,11 @@ int pf_launch_spread(const pf_plan* p, const pf_batch& b, const float2* ys, floa
     const size_t lds = spread_lds_bytes(p, VC);
     if (lds > 160 * 1024) return pf_fail(PF_ERR_UNSUPPORTED, "spread tile exceeds LDS");
     const pf_slist& L = pf_slist_for(p, b);
-    const pf_sitems& I = pf_sitems_for(L, nvch);
+    pf_sitems I = pf_sitems_for(L, nvch);
+    if (live && I.lsitems) {  // the same items without the empty tiles of uncovered columns
+        I.sitems = I.lsitems;
+        I.nsitems = I.nlsitems;
+    }
     if (I.nslots > 0 && !partial) return pf_fail(PF_ERR_ARG, "spread needs a partial-tile workspace");
     const int64_t nb = (int64_t)I.nsitems * nvch;
     if (nb > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many spread blocks");

@@ -357,7 +357,8 @@ int pf_plan_params(pf_plan* p, int ndim, const int* shape_zyx, const int* width_
 
 // Spread list: every (key, sample) pair for the tiles a sample's footprint
 // touches, stable-sorted by key; then work items with heavy tiles split.
-static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t stream, pf_slist& out) {
+static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t stream, pf_slist& out,
+                       const std::vector<uint8_t>& cmask) {
     const int64_t M = p->mtot;
     const int TB = 256;
     const uint64_t nkeys = (uint64_t)(merge ? 1 : p->nframes) * p->ntile_tot;
@@ -482,6 +483,26 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
         S.nslots = slot;
         PF_HIP(hipMalloc(&S.sitems, std::max<size_t>(1, items.size()) * sizeof(int4)));
         PF_HIP(hipMemcpy(S.sitems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+        if (!cmask.empty()) {
+            // drop the empty tiles lying wholly in columns no sample covers
+            std::vector<int4> live;
+            live.reserve(items.size());
+            for (const int4& it : items) {
+                bool keep = it.z > it.y;
+                if (!keep) {
+                    const int64_t t = it.x % p->ntile_tot;
+                    const int tx = (int)(t % p->ntiles[2]), ty = (int)((t / p->ntiles[2]) % p->ntiles[1]);
+                    const int y1 = std::min(p->os[1], (ty + 1) * p->tile[1]);
+                    const int x1 = std::min(p->os[2], (tx + 1) * p->tile[2]);
+                    for (int y = ty * p->tile[1]; y < y1 && !keep; ++y)
+                        for (int x = tx * p->tile[2]; x < x1 && !keep; ++x) keep = cmask[(size_t)y * p->os[2] + x] != 0;
+                }
+                if (keep) live.push_back(it);
+            }
+            S.nlsitems = (int)live.size();
+            PF_HIP(hipMalloc(&S.lsitems, std::max<size_t>(1, live.size()) * sizeof(int4)));
+            PF_HIP(hipMemcpy(S.lsitems, live.data(), live.size() * sizeof(int4), hipMemcpyHostToDevice));
+        }
         PF_HIP(hipMalloc(&S.ritems, std::max<size_t>(1, ritems.size()) * sizeof(int4)));
         if (!ritems.empty())
             PF_HIP(hipMemcpy(S.ritems, ritems.data(), ritems.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -490,6 +511,24 @@ static int build_slist(pf_plan* p, const TapParams& P, int merge, hipStream_t st
 }
 
 static int build_glist(pf_plan* p, const TapParams& P, hipStream_t stream);
+
+// 3-D column mask: every sample marks the (y, x) grid columns of its
+// footprint (plain byte stores of 1, so concurrent marks agree)
+__global__ void k_colmask(const float* __restrict__ taprec, int rec, int64_t M, int Gy, int Gx, int Wy, int Wx,
+                          uint8_t* __restrict__ mask) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M * Wy) return;
+    const int64_t s = j / Wy;
+    const int dy = (int)(j % Wy);
+    const int* ri = reinterpret_cast<const int*>(taprec + s * rec);
+    int y = ri[1] + dy;
+    y -= y >= Gy ? Gy : 0;
+    for (int dx = 0; dx < Wx; ++dx) {
+        int x = ri[2] + dx;
+        x -= x >= Gx ? Gx : 0;
+        mask[(int64_t)y * Gx + x] = 1;
+    }
+}
 
 int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
     TapParams P;
@@ -556,11 +595,22 @@ int pf_plan_build(pf_plan* p, const float* coord, hipStream_t stream) {
                        p->taprec, cos0.as<float>(), p->cos_sorted, p->iperm, M, p->rec);
     PF_HIP(hipGetLastError());
 
+    std::vector<uint8_t> cmask;
+    if (p->ndim == 3 && PF_COLMASK) {
+        const int64_t ncol = (int64_t)p->os[1] * p->os[2];
+        PF_HIP(hipMalloc(&p->colmask, ncol));
+        PF_HIP(hipMemsetAsync(p->colmask, 0, ncol, stream));
+        hipLaunchKernelGGL(k_colmask, dim3(nblk(M * p->width[1], TB)), dim3(TB), 0, stream, p->taprec, p->rec, M,
+                           p->os[1], p->os[2], p->width[1], p->width[2], p->colmask);
+        PF_HIP(hipGetLastError());
+        cmask.resize(ncol);
+        PF_HIP(hipMemcpyAsync(cmask.data(), p->colmask, ncol, hipMemcpyDeviceToHost, stream));
+    }
     PF_HIP(hipStreamSynchronize(stream));
-    int rc = build_slist(p, P, 0, stream, p->sl[0]);
+    int rc = build_slist(p, P, 0, stream, p->sl[0], cmask);
     if (rc != PF_OK) return rc;
     if ((rc = build_glist(p, P, stream)) != PF_OK) return rc;
-    if (p->nframes > 1) return build_slist(p, P, 1, stream, p->sl[1]);
+    if (p->nframes > 1) return build_slist(p, P, 1, stream, p->sl[1], cmask);
     p->sl[1] = p->sl[0];
     return PF_OK;
 }
@@ -685,7 +735,7 @@ static int build_glist(pf_plan* p, const TapParams& P, hipStream_t stream) {
 
 void pf_plan_free(pf_plan* p) {
     if (!p) return;
-    void* ptrs[] = {p->perm, p->iperm, p->taprec, p->cos_sorted, p->apod, p->foff, p->gl.ids, p->gl.items};
+    void* ptrs[] = {p->perm, p->iperm, p->taprec, p->cos_sorted, p->apod, p->foff, p->gl.ids, p->gl.items, p->colmask};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     const bool alias = p->sl[1].slist == p->sl[0].slist;
@@ -698,6 +748,7 @@ void pf_plan_free(pf_plan* p) {
             if (dup) continue;
             if (L.split[k].sitems) (void)hipFree(L.split[k].sitems);
             if (L.split[k].ritems) (void)hipFree(L.split[k].ritems);
+            if (L.split[k].lsitems) (void)hipFree(L.split[k].lsitems);
         }
     }
     if (p->fft) pf_fft_destroy(p->fft);
