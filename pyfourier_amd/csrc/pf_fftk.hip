@@ -644,8 +644,12 @@ constexpr int X256_PLANE = 16 * X256_ROW + 4;
 constexpr int X256_BF = 16;
 constexpr int X256_BA = 16;
 
-// BX columns per block (16 or 32): 16 x BX threads
-template <bool FWD, int BX>
+// BX columns per block (16 or 32): 16 x BX threads.  M1F (forward): store
+// with the 16 threads of a column on consecutive lanes, as the inverse loads:
+// where a row has few columns (KC < 8, e.g. the coil-pair grid's 2 values
+// per point) the columns of one row then fill whole contiguous runs, while
+// columns on consecutive lanes write KC x 8-byte pieces of separate rows.
+template <bool FWD, int BX, bool M1F = false>
 __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g, Batch bt,
                                                                          const float2* __restrict__ twg,
                                                                          const float* __restrict__ apod,
@@ -720,8 +724,8 @@ __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g
         for (int m1 = 0; m1 < 16; ++m1) sh[m1 * PLANE + t * ROW + b] = v[m1];
     }
     __syncthreads();
-    const int m1 = FWD ? (threadIdx.x / BX) : (threadIdx.x & 15);
-    const int b = FWD ? (threadIdx.x % BX) : (threadIdx.x >> 4);
+    const int m1 = (FWD && !M1F) ? (threadIdx.x / BX) : (threadIdx.x & 15);
+    const int b = (FWD && !M1F) ? (threadIdx.x % BX) : (threadIdx.x >> 4);
 #pragma unroll
     for (int tp = 0; tp < 16; ++tp) v[tp] = sh[m1 * PLANE + tp * ROW + b];
     dft16(v);  // v[m2] = X[m1 + 16 m2] of column b
@@ -1304,7 +1308,10 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
         Geo g = geo_of(p);
         Batch t = batch_of(b);
-        if (fwd)
+        if (fwd && KC < 8)
+            hipLaunchKernelGGL((k_fftx256<true, X256_BF, true>), dim3((unsigned)nblk), dim3(16 * X256_BF), 0, s, g,
+                               t, X.tw, p->apod, src, dst, KC, YG, nkc, 1.0f);
+        else if (fwd)
             hipLaunchKernelGGL((k_fftx256<true, X256_BF>), dim3((unsigned)nblk), dim3(16 * X256_BF), 0, s, g, t,
                                X.tw, p->apod, src, dst, KC, YG, nkc, 1.0f);
         else
