@@ -237,7 +237,10 @@ int pf_workspace_offsets(const pf_plan* plan, const pf_batch_desc* desc, size_t*
  * For grid sizes 2^a 3^b 5^c (<= 4096 per axis) a pruned FFT transforms only
  * the lines that hold image data; otherwise pf_deapod_pad + pf_fft.
  * Replaces src/pyfourier/_nufft/_nufft.py:234-241.  fft_workspace: the FFT
- * scratch part of the pf_nufft workspace (pf_workspace_offsets). */
+ * scratch part of the pf_nufft workspace (pf_workspace_offsets).  Grid
+ * layout (F, C/Cv, G..., nseg*ncoeff, Cv), F = 1 with a basis: Cv is the
+ * largest divisor of C up to 64, or 2 (coil pairs) for a 3-D plan with one
+ * kernel width of 3..8 on all axes and an even C with no field map or basis. */
 int pf_pad_fft(pf_plan* plan, const pf_batch_desc* desc, const void* image, void* grid,
                void* fft_workspace, void* stream);
 

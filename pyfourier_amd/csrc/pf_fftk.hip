@@ -765,17 +765,22 @@ __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g
 // L*K*Cv is smaller than the block), loads with the columns on consecutive
 // lanes, and after the FFT sums the segments (times the conjugate maps)
 // through LDS, so the image rows are written once, x on consecutive lanes.
+// M1F (forward, rows of at most 2 values per point, e.g. the coil-pair
+// grid): the second stage puts a column's 16 threads m1 on consecutive lanes,
+// so the 2 columns of a row write 256-byte runs instead of the 64-byte pieces
+// of 8 rows that columns on consecutive lanes give.
 // LDS strides: ds_read/write_b64 serve 32 lanes per cycle; the forward's
 // transpose writes have t on consecutive lanes (odd row stride) and its reads
-// columns (plane stride = 16 mod 32), the adjoint's writes columns (row stride
-// 16 with two rows per 32 lanes) and its reads m1 (plane stride = 2 mod 32).
+// columns (plane stride = 16 mod 32) or, M1F, m1 (plane stride = 2 mod 32);
+// the adjoint's writes columns (row stride 16 with two rows per 32 lanes) and
+// its reads m1 (plane stride = 2 mod 32).
 // ---------------------------------------------------------------------------
 constexpr int X512_B = 16;
 
-template <bool FWD>
+template <bool FWD, bool M1F = false>
 struct X512Strides {
     static constexpr int ROW = FWD ? X512_B + 1 : X512_B;
-    static constexpr int PLANE = FWD ? 32 * ROW + 16 : 32 * ROW + 2;
+    static constexpr int PLANE = (FWD && !M1F) ? 32 * ROW + 16 : 32 * ROW + 2;
 };
 
 // image plane base (x = 0 of the row) of (row, kc); zy = the row's z*Ny + y
@@ -791,12 +796,13 @@ __device__ __forceinline__ int64_t x_image_row(const Geo& g, const Batch& bt, in
     return ((int64_t)c * bt.nimg + ii) * g.ntot + zy * g.N[2];
 }
 
+template <bool M1F>
 __global__ void __launch_bounds__(32 * X512_B) k_fftx512_fwd(Geo g, Batch bt, const float2* __restrict__ twg,
                                                             const float* __restrict__ apod,
                                                             const float2* __restrict__ in,
                                                             float2* __restrict__ out) {
     constexpr int G = 512, BX = X512_B;
-    constexpr int ROW = X512Strides<true>::ROW, PLANE = X512Strides<true>::PLANE;
+    constexpr int ROW = X512Strides<true, M1F>::ROW, PLANE = X512Strides<true, M1F>::PLANE;
     __shared__ float2 sh[16 * PLANE];
     __shared__ float2 tw[G];
     const int N = g.N[2];
@@ -840,7 +846,8 @@ __global__ void __launch_bounds__(32 * X512_B) k_fftx512_fwd(Geo g, Batch bt, co
     }
     __syncthreads();
     if (threadIdx.x >= 16 * BX) return;
-    const int m1 = threadIdx.x / BX, b = threadIdx.x % BX;
+    const int m1 = M1F ? (threadIdx.x & 15) : (threadIdx.x / BX);
+    const int b = M1F ? (threadIdx.x >> 4) : (threadIdx.x % BX);
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
         v[s] = sh[m1 * PLANE + (2 * s) * ROW + b];
@@ -1328,8 +1335,12 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
             const int64_t nblk = (rows * b.R * b.Cv + X512_B - 1) / X512_B;
             if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
             if (nblk < 1) return PF_OK;
-            hipLaunchKernelGGL(k_fftx512_fwd, dim3((unsigned)nblk), dim3(32 * X512_B), 0, s, g, t, X.tw, p->apod,
-                               src, dst);
+            if (b.R * b.Cv <= 2)
+                hipLaunchKernelGGL(k_fftx512_fwd<true>, dim3((unsigned)nblk), dim3(32 * X512_B), 0, s, g, t, X.tw,
+                                   p->apod, src, dst);
+            else
+                hipLaunchKernelGGL(k_fftx512_fwd<false>, dim3((unsigned)nblk), dim3(32 * X512_B), 0, s, g, t, X.tw,
+                                   p->apod, src, dst);
         } else {
             XBlk xb;  // all L segments of KC (k, cv) pairs per block, YG rows when they fit
             xb.KC = std::min(KCv, std::max(1, X512_B / b.L));

@@ -45,6 +45,15 @@ CASES = [
     ((8, 200), 2.56),      # 16 x 512: 200 of the 512 x points carry data
     ((6, 8, 256), 2.0),    # 12 x 16 x 512
 ]
+# (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
+# values per grid point (second-stage lanes m1-fastest) and with 4
+PARAMS = [(s, o, 3) for s, o in CASES] + [
+    ((256,), 2.0, 2),
+    ((10, 256), 2.0, 1),
+    ((8, 200), 2.56, 2),
+    ((6, 8, 256), 2.0, 2),
+    ((6, 8, 256), 2.0, 4),
+]
 
 
 def _desc(C):
@@ -53,14 +62,13 @@ def _desc(C):
     return d
 
 
-@pytest.mark.parametrize("shape,oversamp", CASES)
-def test_pad_fft_stage(cuda_device, shape, oversamp):
+@pytest.mark.parametrize("shape,oversamp,C", PARAMS)
+def test_pad_fft_stage(cuda_device, shape, oversamp, C):
     rng = np.random.default_rng(0)
     d = len(shape)
     coord = rng.uniform(-np.pi, np.pi, size=(256, d)).astype(np.float32)
     coord[0] = 0
     coord[0, 0] = -np.pi
-    C = 3
     plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=oversamp)
     plan.to(cuda_device)
     P = R.Plan(coord, shape, width=4, oversamp=oversamp)
@@ -80,15 +88,20 @@ def test_pad_fft_stage(cuda_device, shape, oversamp):
     _lib.check(L.pf_pad_fft(nat.handle, ctypes.byref(desc), _lib.ptr(xd), _lib.ptr(grid), _lib.ptr(fws), st))
     torch.cuda.synchronize()
     G = P.os_shape
-    # grid layout (C / Cv, *G, Cv): Cv is the largest divisor of C up to 64
-    Cv = max(v for v in range(1, min(C, 64) + 1) if C % v == 0)
+    # grid layout (C / Cv, *G, Cv): Cv is the largest divisor of C up to 64,
+    # or 2 (coil pairs) for a 3-D plan with one kernel width and an even C
+    Cv = 2 if d == 3 and C % 2 == 0 else max(v for v in range(1, min(C, 64) + 1) if C % v == 0)
     g = grid[: C * int(np.prod(G))].cpu().numpy().reshape((C // Cv,) + tuple(G) + (Cv,))
     g = np.moveaxis(g, -1, 1).reshape((C,) + tuple(G))
     ref = R._to_grid(x[:, None], P, None)[:, 0]
     ref = np.fft.ifftshift(ref, axes=tuple(range(-d, 0)))  # stored (unshifted) order
     err = rel(g, ref)
     assert err < 2e-6, f"pad_fft {shape} sigma={oversamp} grid={G}: rel err {err:.3e}"
-    # adjoint stage: inverse FFT, crop, deapodise (factor 1)
+    # adjoint stage: inverse FFT, crop, deapodise (factor 1), from the
+    # reference grid in the adjoint's layout (Cv the largest divisor of C up to 64)
+    Cva = max(v for v in range(1, min(C, 64) + 1) if C % v == 0)
+    ga = np.moveaxis(ref.reshape((C // Cva, Cva) + tuple(G)), 1, -1).astype(np.complex64)
+    grid[: C * int(np.prod(G))].copy_(torch.from_numpy(ga.ravel()))
     img = torch.empty((C,) + tuple(shape), dtype=torch.complex64, device=cuda_device)
     _lib.check(L.pf_ifft_crop(nat.handle, ctypes.byref(desc), _lib.ptr(grid), _lib.ptr(img), 1.0,
                               _lib.ptr(fws), st))
