@@ -540,22 +540,110 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_fftcol256(int N
 }
 
 // ---------------------------------------------------------------------------
-// Column pass for G = 512 (256-point images at oversampling 2), the same
-// register four-step with 512 = 32 x 16: element n = t + 32k of a column is
-// held by thread t (t = 0..31, k = 0..15); thread t takes the 16-point DFT
-// over k, multiplies by exp(-2 pi i t m1 / 512), the block transposes through
-// LDS once, and threads t < 16 (m1 = t) take the 32-point DFT over t (two
-// 16-point DFTs and a radix-2 step), giving output m1 + 16 m2.  Lane =
-// (column b, t), b fastest.  The upper half of the threads idles in the
-// second stage; the pass is bound by its global traffic, not its arithmetic.
+// Column pass for G = 512 (256-point images at oversampling 2), the register
+// four-step with 512 = 16 x 32: element n = t + 16k of a column is held by
+// thread t (t = 0..15, k = 0..31), which takes the 32-point DFT over k (two
+// 16-point DFTs and a radix-2 step) and multiplies by exp(-2 pi i t m1 / 512);
+// the block transposes through LDS in two halves (m1 < 16, then m1 >= 16, so
+// a half-size buffer keeps two blocks on a CU), and after each half thread t
+// takes the 16-point DFT over t for m1 = t (+16), giving outputs m1 + 32 m2.
+// Every thread works in both stages.  Lane = (column b, t), b fastest, B
+// consecutive columns per block: each row access is one B x 8-byte run.
 // ---------------------------------------------------------------------------
 #ifndef PF_FFT512
 #define PF_FFT512 1  // register four-step column passes for 512-point axes
 #endif
-constexpr int C512_B = 16;  // columns per block (512 threads)
+constexpr int C512_B = 32;        // columns per block (512 threads)
+constexpr int C512_BS = 16;       // k_fftcol512_t32: columns per block (512 threads)
+constexpr int64_t C512_MIN = 1 << 18;  // columns from which k_fftcol512 runs (measured: 2^15 columns
+                                       // 6% slower than k_fftcol512_t32, 2^18 even, 2^20-2^21 8-10% faster)
 
 template <bool FWD, int B>
-__global__ void __launch_bounds__(32 * B) k_fftcol512(int N, int64_t I, const float2* __restrict__ twg,
+__global__ void __launch_bounds__(16 * B) k_fftcol512(int N, int64_t I, const float2* __restrict__ twg,
+                                                    const float2* __restrict__ in, float2* __restrict__ out,
+                                                    ColMask cm) {
+    constexpr int G = 512;
+    constexpr int PLANE = 16 * B;  // a 32-lane access covers 32 consecutive columns: no bank conflicts
+    __shared__ float2 sh[16 * PLANE];
+    __shared__ float2 tw[G];
+    const int b = threadIdx.x % B, t = threadIdx.x / B;
+    const int64_t nchunk = (I + B - 1) / B;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t o = blk / nchunk;
+    const int64_t i0 = (blk % nchunk) * B;
+    const bool bok = i0 + b < I;
+    const bool zero = !bok || cm_dead_col(cm, i0 + b);  // no column, or one known to be zero
+    for (int k = threadIdx.x; k < G; k += 16 * B) tw[k] = twg[k];
+    const int lo = N - N / 2, hi = G - N / 2;  // stored rows [lo, hi) carry no image data
+    // v[s] = element k = 2s, v[16 + s] = element k = 2s + 1
+    float2 v[32];
+    if (FWD) {
+        const float2* src = in + o * (int64_t)N * I + i0 + b;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const int n = t + 16 * k;  // stored row
+            const int p = n < lo ? n + N / 2 : n - G + N / 2;
+            v[(k & 1) * 16 + (k >> 1)] =
+                (!zero && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
+        }
+    } else {
+        const float2* src = in + o * (int64_t)G * I + i0 + b;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const float2 x = !zero ? src[(int64_t)(t + 16 * k) * I] : make_float2(0.f, 0.f);
+            v[(k & 1) * 16 + (k >> 1)] = make_float2(x.x, -x.y);
+        }
+    }
+    dft16(v);
+    dft16(v + 16);
+    __syncthreads();  // twiddle table
+    // Y[m] = E[m] + W32^m O[m], Y[m + 16] = E[m] - W32^m O[m], times W512^(t m1)
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const float2 w = c_mul(v[16 + m], tw[16 * m]);
+        const float2 e = v[m];
+        v[m] = c_mul(c_add(e, w), tw[t * m]);
+        v[16 + m] = c_mul(c_sub(e, w), tw[t * (m + 16)]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (h) __syncthreads();  // the first half's reads are done
+#pragma unroll
+        for (int m = 0; m < 16; ++m) sh[m * PLANE + t * B + b] = v[16 * h + m];
+        __syncthreads();
+        float2 u[16];  // thread t: m1 = 16 h + t, u[t'] = Y_t'[m1]
+#pragma unroll
+        for (int tp = 0; tp < 16; ++tp) u[tp] = sh[t * PLANE + tp * B + b];
+        dft16(u);  // u[m2] = X[m1 + 32 m2]
+        if (bok) {
+            const int m1 = 16 * h + t;
+            if (FWD) {
+                float2* dst = out + o * (int64_t)G * I + i0 + b;
+#pragma unroll
+                for (int m2 = 0; m2 < 16; ++m2) dst[(int64_t)(m1 + 32 * m2) * I] = u[m2];
+            } else {
+                float2* dst = out + o * (int64_t)N * I + i0 + b;
+#pragma unroll
+                for (int m2 = 0; m2 < 16; ++m2) {
+                    const int n = m1 + 32 * m2;  // stored output index -> cropped row p
+                    if (n < lo || n >= hi) {
+                        const int p = n < lo ? n + N / 2 : n - G + N / 2;
+                        dst[(int64_t)p * I] = make_float2(u[m2].x, -u[m2].y);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// The same transform with 512 = 32 x 16 for passes of fewer columns: 32
+// threads per column hold 16 values each (stride 32) for a 16-point DFT,
+// twiddle W512^(t m1), one LDS transpose, then threads m1 < 16 take the
+// 32-point DFT over t as two 16-point DFTs and a radix-2 step (the upper
+// half idles).  Fewer registers (100 / 140 against 195 / 172 VGPRs), so
+// twice the threads per CU, which pays where a pass has few columns.
+template <bool FWD, int B>
+__global__ void __launch_bounds__(32 * B) k_fftcol512_t32(int N, int64_t I, const float2* __restrict__ twg,
                                                     const float2* __restrict__ in, float2* __restrict__ out,
                                                     ColMask cm) {
     constexpr int G = 512;
@@ -753,9 +841,10 @@ __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g
 
 // ---------------------------------------------------------------------------
 // X pass for G = 512 with any number of B0 segments L (<= X512_B) and any
-// subspace K: the register four-step FFT of k_fftcol512 (32 threads per
-// column, 16-point DFTs, one LDS transpose, a 32-point DFT by the lower 16
-// threads), X512_B columns per block.
+// subspace K: a register four-step FFT with 512 = 32 x 16 (32 threads per
+// column hold 16 values each for a 16-point DFT, twiddle W512^(t m1), one LDS
+// transpose, then threads m1 < 16 take the 32-point DFT over t as two
+// 16-point DFTs and a radix-2 step), X512_B columns per block.
 //
 // Forward: block columns are X512_B consecutive grid columns (row, v) with
 // v = l*K*Cv + kc, so every output row n is one contiguous run; a column
@@ -1385,15 +1474,23 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
                       hipStream_t s, ColMask cm = ColMask{nullptr, 0, 1}) {
     const pf_fftk_axis& X = p->fftk->ax[axis];
     if (X.G == 512 && PF_FFT512) {
-        const int64_t nblk = O * ((I + C512_B - 1) / C512_B);
+        const bool wide = O * I >= C512_MIN;
+        const int B = wide ? C512_B : C512_BS;
+        const int64_t nblk = O * ((I + B - 1) / B);
         if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
         if (nblk < 1) return PF_OK;
-        if (fwd)
-            hipLaunchKernelGGL((k_fftcol512<true, C512_B>), dim3((unsigned)nblk), dim3(32 * C512_B), 0, s, X.N, I,
+        if (wide && fwd)
+            hipLaunchKernelGGL((k_fftcol512<true, C512_B>), dim3((unsigned)nblk), dim3(16 * C512_B), 0, s, X.N, I,
                                X.tw, src, dst, cm);
+        else if (wide)
+            hipLaunchKernelGGL((k_fftcol512<false, C512_B>), dim3((unsigned)nblk), dim3(16 * C512_B), 0, s, X.N, I,
+                               X.tw, src, dst, cm);
+        else if (fwd)
+            hipLaunchKernelGGL((k_fftcol512_t32<true, C512_BS>), dim3((unsigned)nblk), dim3(32 * C512_BS), 0, s,
+                               X.N, I, X.tw, src, dst, cm);
         else
-            hipLaunchKernelGGL((k_fftcol512<false, C512_B>), dim3((unsigned)nblk), dim3(32 * C512_B), 0, s, X.N, I,
-                               X.tw, src, dst, cm);
+            hipLaunchKernelGGL((k_fftcol512_t32<false, C512_BS>), dim3((unsigned)nblk), dim3(32 * C512_BS), 0, s,
+                               X.N, I, X.tw, src, dst, cm);
         PF_HIP(hipGetLastError());
         return PF_OK;
     }

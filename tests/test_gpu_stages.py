@@ -64,6 +64,16 @@ def _desc(C):
 
 @pytest.mark.parametrize("shape,oversamp,C", PARAMS)
 def test_pad_fft_stage(cuda_device, shape, oversamp, C):
+    _check_stages(cuda_device, shape, oversamp, C)
+
+
+def test_pad_fft_stage_wide_512_columns(cuda_device):
+    """A 512-point z pass over 2^18 columns (16 coils x 128 x 128), where the
+    16-thread-per-column kernel takes over from the 32-thread one."""
+    _check_stages(cuda_device, (256, 64, 64), 2.0, 16)
+
+
+def _check_stages(cuda_device, shape, oversamp, C):
     rng = np.random.default_rng(0)
     d = len(shape)
     coord = rng.uniform(-np.pi, np.pi, size=(256, d)).astype(np.float32)
