@@ -76,7 +76,7 @@ def _apodize(x: torch.Tensor, plan: R.Plan) -> torch.Tensor:
         if plan.width[ax] == 1:
             continue
         i = out.shape[ax]
-        os_i = math.ceil(plan.oversamp[ax] * i)
+        os_i = int(np.ceil(np.float32(plan.oversamp[ax]) * np.float32(i)))  # float32 product, as _apod.py:27
         idx = torch.arange(i, dtype=torch.float32)
         a = (plan.beta[ax] ** 2 - (math.pi * plan.width[ax] * (idx - i // 2) / os_i) ** 2) ** 0.5
         a = a / torch.sinh(a)

@@ -238,10 +238,12 @@ def interp_tables(coord_os, width_xyz, beta_xyz, grid_xyz, dtype=np.float32):
 
 
 def apod_vector(n, oversamp, width, beta):
-    """KB deapodisation along one axis of length n: _subroutines/_apod.py:27-41."""
+    """KB deapodisation along one axis of length n: _subroutines/_apod.py:27-41.
+    The grid length is ceil(sigma * n) in float32 (the reference's float32
+    oversampling factor times the axis length, _apod.py:27), as os_shape."""
     if int(width) == 1:
         return np.ones(n, dtype=np.float64)
-    os_n = math.ceil(float(np.float32(oversamp)) * n)
+    os_n = int(np.ceil(np.float32(oversamp) * np.float32(n)))
     idx = np.arange(n, dtype=np.float64)
     with np.errstate(invalid="ignore", divide="ignore"):
         a = np.sqrt(float(beta) ** 2 - (math.pi * int(width) * (idx - n // 2) / os_n) ** 2)

@@ -553,6 +553,9 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_fftcol256(int N
 #ifndef PF_FFT512
 #define PF_FFT512 1  // register four-step column passes for 512-point axes
 #endif
+#ifndef PF_FFTQ
+#define PF_FFTQ 1  // register four-step column passes for 16 Q-point axes
+#endif
 constexpr int C512_B = 32;        // columns per block (512 threads)
 constexpr int C512_BS = 16;       // k_fftcol512_t32: columns per block (512 threads)
 constexpr int64_t C512_MIN = 1 << 18;  // columns from which k_fftcol512 runs (measured: 2^15 columns
@@ -711,6 +714,141 @@ __global__ void __launch_bounds__(32 * B) k_fftcol512_t32(int N, int64_t I, cons
                 const float2 x = q < 16 ? c_add(v[q & 15], w) : c_sub(v[q & 15], w);
                 const int p = n < lo ? n + N / 2 : n - G + N / 2;
                 dst[(int64_t)p * I] = make_float2(x.x, -x.y);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column passes for G = 16 Q, Q in {8, 10, 12, 15, 20, 24} (G = 128, 160, 192,
+// 240, 320, 384: the grids of 128-256 point images at oversampling 1.25 or
+// 2), the same register four-step as the 256-point pass: element n = t + 16k
+// of a column is held by thread t (t = 0..15, k = 0..Q-1), which takes the
+// Q-point DFT over k and multiplies by exp(-2 pi i t m1 / G); the block
+// transposes through LDS (rounds of 16 m1 planes), and thread t takes the
+// 16-point DFT over t for m1 = t (+16), giving outputs m1 + Q m2.  The Q-point
+// DFTs are Good-Thomas products of two coprime small DFTs (2, 3, 4, 5, 8),
+// whose index maps are compile-time register renamings.  Lane = (column b,
+// t), b fastest, B consecutive columns per block.
+// ---------------------------------------------------------------------------
+constexpr int inv_mod(int x, int m) {
+    for (int y = 1; y < m; ++y)
+        if ((x * y) % m == 1) return y;
+    return 1;  // m == 1
+}
+
+// forward Q-point DFT (Q = A R, gcd(A, R) = 1), in place: input index
+// n = (R n1 + A n2) mod Q, output index k = (R (R^-1 mod A) k1 + A (A^-1 mod R) k2) mod Q
+template <int A, int R>
+__device__ __forceinline__ void dft_pfa(float2* v) {
+    constexpr int Q = A * R;
+    constexpr int ra = R * inv_mod(R % A, A), ar = A * inv_mod(A % R, R);
+    float2 u[A][R];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1)
+#pragma unroll
+        for (int n2 = 0; n2 < R; ++n2) u[n1][n2] = v[(R * n1 + A * n2) % Q];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) dftR<R>(u[n1]);
+#pragma unroll
+    for (int k2 = 0; k2 < R; ++k2) {
+        float2 c[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) c[n1] = u[n1][k2];
+        dftR<A>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) u[k1][k2] = c[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < R; ++k2) v[(ra * k1 + ar * k2) % Q] = u[k1][k2];
+}
+
+template <int Q>
+__device__ __forceinline__ void dftQ(float2* v) {
+    if constexpr (Q == 8) dft8(v);
+    else if constexpr (Q == 10) dft_pfa<2, 5>(v);
+    else if constexpr (Q == 12) dft_pfa<4, 3>(v);
+    else if constexpr (Q == 15) dft_pfa<3, 5>(v);
+    else if constexpr (Q == 20) dft_pfa<4, 5>(v);
+    else dft_pfa<8, 3>(v);  // 24
+}
+
+constexpr int CQ_B = 32;  // columns per block (512 threads)
+
+// G with a register column pass (16 Q, Q as above; 256 and 512 have their own)
+static inline bool colq_supported(int G) {
+    return G == 128 || G == 160 || G == 192 || G == 240 || G == 320 || G == 384;
+}
+
+template <bool FWD, int Q, int B>
+__global__ void __launch_bounds__(16 * B, 4) k_fftcolq(int N, int64_t I, const float2* __restrict__ twg,
+                                                  const float2* __restrict__ in, float2* __restrict__ out,
+                                                  ColMask cm) {
+    constexpr int G = 16 * Q;
+    constexpr int NP = Q < 16 ? Q : 16;  // m1 planes per transpose round
+    constexpr int PLANE = 16 * B;        // a 32-lane access covers 32 consecutive columns: no bank conflicts
+    __shared__ float2 sh[NP * PLANE];
+    __shared__ float2 tw[G];
+    const int b = threadIdx.x % B, t = threadIdx.x / B;
+    const int64_t nchunk = (I + B - 1) / B;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t o = blk / nchunk;
+    const int64_t i0 = (blk % nchunk) * B;
+    const bool bok = i0 + b < I;
+    const bool zero = !bok || cm_dead_col(cm, i0 + b);  // no column, or one known to be zero
+    for (int k = threadIdx.x; k < G; k += 16 * B) tw[k] = twg[k];
+    const int lo = N - N / 2, hi = G - N / 2;  // stored rows [lo, hi) carry no image data
+    float2 v[Q];
+    if (FWD) {
+        const float2* src = in + o * (int64_t)N * I + i0 + b;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const int n = t + 16 * k;  // stored row
+            const int p = n < lo ? n + N / 2 : n - G + N / 2;
+            v[k] = (!zero && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
+        }
+    } else {
+        const float2* src = in + o * (int64_t)G * I + i0 + b;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const float2 x = !zero ? src[(int64_t)(t + 16 * k) * I] : make_float2(0.f, 0.f);
+            v[k] = make_float2(x.x, -x.y);
+        }
+    }
+    dftQ<Q>(v);
+    __syncthreads();  // twiddle table
+#pragma unroll
+    for (int m = 1; m < Q; ++m) v[m] = c_mul(v[m], tw[t * m]);
+#pragma unroll
+    for (int h = 0; h < (Q + 15) / 16; ++h) {
+        if (h) __syncthreads();  // the previous round's reads are done
+#pragma unroll
+        for (int m = 16 * h; m < Q && m < 16 * h + 16; ++m) sh[(m - 16 * h) * PLANE + t * B + b] = v[m];
+        __syncthreads();
+        const int m1 = 16 * h + t;
+        if (m1 < Q) {
+            float2 u[16];  // u[t'] = twiddled Q-point output m1 of thread t'
+#pragma unroll
+            for (int tp = 0; tp < 16; ++tp) u[tp] = sh[t * PLANE + tp * B + b];
+            dft16(u);  // u[m2] = X[m1 + Q m2]
+            if (bok) {
+                if (FWD) {
+                    float2* dst = out + o * (int64_t)G * I + i0 + b;
+#pragma unroll
+                    for (int m2 = 0; m2 < 16; ++m2) dst[(int64_t)(m1 + Q * m2) * I] = u[m2];
+                } else {
+                    float2* dst = out + o * (int64_t)N * I + i0 + b;
+#pragma unroll
+                    for (int m2 = 0; m2 < 16; ++m2) {
+                        const int n = m1 + Q * m2;  // stored output index -> cropped row p
+                        if (n < lo || n >= hi) {
+                            const int p = n < lo ? n + N / 2 : n - G + N / 2;
+                            dst[(int64_t)p * I] = make_float2(u[m2].x, -u[m2].y);
+                        }
+                    }
+                }
             }
         }
     }
@@ -1508,6 +1646,33 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
+    if (PF_FFTQ && colq_supported(X.G)) {
+        const int64_t nblk = O * ((I + CQ_B - 1) / CQ_B);
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+        if (nblk < 1) return PF_OK;
+        const dim3 grid((unsigned)nblk), blk(16 * CQ_B);
+        const int Q = X.G / 16;
+#define PF_COLQ_CASE(QV)                                                                                     \
+    case QV:                                                                                                 \
+        if (fwd)                                                                                             \
+            hipLaunchKernelGGL((k_fftcolq<true, QV, CQ_B>), grid, blk, 0, s, X.N, I, X.tw, src, dst, cm);   \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_fftcolq<false, QV, CQ_B>), grid, blk, 0, s, X.N, I, X.tw, src, dst, cm);  \
+        break;
+        switch (Q) {
+            PF_COLQ_CASE(8)
+            PF_COLQ_CASE(10)
+            PF_COLQ_CASE(12)
+            PF_COLQ_CASE(15)
+            PF_COLQ_CASE(20)
+            PF_COLQ_CASE(24)
+            default:
+                return pf_fail(PF_ERR_FFT, "no register column pass for this size");
+        }
+#undef PF_COLQ_CASE
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
     if (cm.mode != 0) return pf_fail(PF_ERR_UNSUPPORTED, "column mask on an LDS FFT pass");
     AxisFFT A = axis_desc(X);
     const int B = cols_for(X.G);
@@ -1530,7 +1695,7 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
 bool pf_fftk_masked(const pf_plan* p) {
     if (!PF_COLMASK || p->ndim != 3 || !p->colmask || !p->fftk) return false;
     const int G = p->fftk->ax[0].G;
-    return G == 256 || (G == 512 && PF_FFT512);
+    return G == 256 || (G == 512 && PF_FFT512) || (PF_FFTQ && colq_supported(G));
 }
 
 // image -> grid (deapodise, pad, FFT); `extra` holds pf_fftk_extra_bytes

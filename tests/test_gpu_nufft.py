@@ -68,6 +68,27 @@ def test_fwd_adj_3d_multicoil_parity(cuda_device):
     assert rel(xa.cpu().numpy(), x_ref) < PARITY
 
 
+@pytest.mark.parametrize("shape,oversamp", [((128, 16, 16), 1.25), ((96, 12, 10), 2.0), ((192, 12, 10), 1.25),
+                                           ((256, 12, 10), 1.25), ((192, 12, 10), 2.0), ((64, 12, 10), 2.0)])
+def test_fwd_adj_3d_register_z_pass(cuda_device, shape, oversamp):
+    """3-D plans whose z axis (160, 192, 240, 320, 384, 128 points) runs the
+    register column pass, masked in the fused adjoint (the radial sphere
+    leaves the (y, x) grid's corner columns empty)."""
+    rng = np.random.default_rng(11)
+    C = 4
+    coord = demo.golden_means_radial_3d(128, 64)
+    x = demo.complex_gaussian(rng, (C,) + shape)
+    P = R.Plan(coord, shape, width=4, oversamp=oversamp)
+    y_ref = R.nufft(x[:, None], P)[:, 0].reshape((C,) + coord.shape[:-1])
+    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=oversamp)
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    assert rel(y.cpu().numpy(), y_ref) < PARITY
+    k = demo.complex_gaussian(rng, (C,) + coord.shape[:-1])
+    x_ref = R.nufft_adj(k.reshape(C, 1, -1), P)[:, 0]
+    xa = pf.nufft_adj(torch.from_numpy(k).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    assert rel(xa.cpu().numpy(), x_ref) < PARITY
+
+
 @pytest.mark.parametrize("C", [1, 2, 8, 16, 32])
 def test_batch_sizes_2d(cuda_device, C):
     rng = np.random.default_rng(2)

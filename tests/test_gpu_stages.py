@@ -44,6 +44,14 @@ CASES = [
     ((10, 256), 2.0),      # 20 x 512
     ((8, 200), 2.56),      # 16 x 512: 200 of the 512 x points carry data
     ((6, 8, 256), 2.0),    # 12 x 16 x 512
+    ((64, 20), 2.0),       # 128 x 40: y is a 16 x 8 register column pass
+    ((128, 20), 1.25),     # 160 x 25: 16 x 10
+    ((100, 10), 1.6),      # 160 x 16: 100 of the 160 y rows carry data
+    ((96, 24), 2.0),       # 192 x 48: 16 x 12
+    ((192, 10, 8), 1.25),  # 240 x 13 x 10: z is a 16 x 15 register column pass
+    ((150, 8), 1.6),       # 240 x 13
+    ((256, 12), 1.25),     # 320 x 15: 16 x 20
+    ((10, 192, 12), 2.0),  # 20 x 384 x 24: y is a 16 x 24 register column pass
 ]
 # (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
 # values per grid point (second-stage lanes m1-fastest) and with 4

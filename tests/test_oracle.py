@@ -70,12 +70,16 @@ def test_interp_tables_brute_force():
     np.testing.assert_allclose(val[0][1], special.i0(b * np.sqrt(1 - u * u)), rtol=6e-7)
 
 
-@pytest.mark.parametrize("W,s,n", [(4, 2.0, 64), (6, 2.0, 48), (4, 1.25, 40)])
+@pytest.mark.parametrize("W,s,n", [(4, 2.0, 64), (6, 2.0, 48), (4, 1.25, 40), (4, 1.6, 10), (4, 1.6, 100)])
 def test_apodization_inverts_kernel_transform(W, s, n):
     """apod(i) * phi_hat((i - n//2)/G) is constant, with phi_hat the Fourier
-    transform of the exact KB kernel computed by quadrature."""
+    transform of the exact KB kernel computed by quadrature and G the grid
+    length the FFT runs on (os_shape: sigma*n in float32; at sigma 1.6 and n
+    10 or 100 a float64 product would round up to 17 or 161)."""
     beta = float(R.kb_beta([W], [s])[0])
-    G = math.ceil(s * n)
+    G = R.os_shape([n], [s])[0]
+    if s == 1.6:
+        assert G == round(s * n) < math.ceil(float(np.float32(s)) * n)
 
     def phi_hat(u):
         f = lambda t: special.i0(beta * math.sqrt(max(0.0, 1 - (2 * t / W) ** 2))) * math.cos(2 * math.pi * u * t)
