@@ -977,6 +977,116 @@ __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g
     }
 }
 
+// X pass for G = 16 Q (the register column passes' sizes) and one B0
+// segment: k_fftx256's lane maps and block shape with the 16 x Q four-step of
+// k_fftcolq (Q values per thread, the transpose in rounds of 16 m1 planes).
+template <bool FWD, int Q, int BX, bool M1F = false>
+__global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2* __restrict__ twg,
+                                                 const float* __restrict__ apod, const float2* __restrict__ in,
+                                                 float2* __restrict__ out, int KC, int YG, int nkc, float factor) {
+    constexpr int G = 16 * Q;
+    constexpr int NP = Q < 16 ? Q : 16;
+    constexpr int ROW = BX + 1, PLANE = 16 * ROW + 4;  // transpose strides (float2), as k_fftx256
+    __shared__ float2 sh[NP * PLANE];
+    __shared__ float2 tw[G];
+    const int N = g.N[2];
+    const int RC = bt.R * bt.Cv;
+    const int KCv = bt.K * bt.Cv;
+    const int64_t nrows = (int64_t)g.N[0] * g.N[1];
+    const int64_t rows_tot = (int64_t)bt.nfg * bt.Cb * nrows;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t row0 = (blk / nkc) * YG;
+    const int kc0 = (int)(blk % nkc) * KC;
+    const float* apz = apod;
+    const float* apy = apod + g.N[0];
+    const float* apx = apod + g.N[0] + g.N[1];
+    for (int k = threadIdx.x; k < G; k += 16 * BX) tw[k] = twg[k];
+    const int lo = N - N / 2, hi = G - N / 2;
+    auto column = [&](int b, int64_t& row, int& kc) {
+        const int yy = b / KC;
+        kc = kc0 + b % KC;
+        row = row0 + yy;
+        return yy < YG && row < rows_tot && kc < KCv;
+    };
+    auto image_row = [&](int64_t row, int kc, float& ayz) -> int64_t {
+        int fg, cb;
+        int64_t zy;
+        x_row_decode(bt, row, nrows, fg, cb, zy);
+        const int k = kc / bt.Cv, cv = kc % bt.Cv;
+        const int z = (int)(zy / g.N[1]), y = (int)(zy % g.N[1]);
+        ayz = apz[z] * apy[y];
+        const int ii = bt.nfg > 1 ? fg : (bt.nimg > 1 ? k : 0);
+        const int c = cb * bt.Cv + cv;
+        return ((int64_t)c * bt.nimg + ii) * g.ntot + zy * N;
+    };
+    float2 v[Q];
+    const int t = FWD ? (threadIdx.x & 15) : (threadIdx.x / BX);
+    {
+        const int b = FWD ? (threadIdx.x >> 4) : (threadIdx.x % BX);
+        int64_t row;
+        int kc;
+        const bool ok = column(b, row, kc);
+        if (FWD) {
+            float ayz = 0.f;
+            const int64_t base = ok ? image_row(row, kc, ayz) : 0;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const int n = t + 16 * k;
+                const int p = n < lo ? n + N / 2 : n - G + N / 2;
+                v[k] = (ok && (n < lo || n >= hi)) ? c_scale(in[base + p], ayz * apx[p]) : make_float2(0.f, 0.f);
+            }
+        } else {
+            const float2* src = in + (ok ? row : 0) * G * RC + kc;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const float2 x = ok ? src[(int64_t)(t + 16 * k) * RC] : make_float2(0.f, 0.f);
+                v[k] = make_float2(x.x, -x.y);
+            }
+        }
+        dftQ<Q>(v);
+        __syncthreads();  // twiddle table
+#pragma unroll
+        for (int m = 1; m < Q; ++m) v[m] = c_mul(v[m], tw[t * m]);
+    }
+    const int sb = FWD ? (threadIdx.x >> 4) : (threadIdx.x % BX);  // stage-1 column of this thread
+    const int j = (FWD && !M1F) ? (threadIdx.x / BX) : (threadIdx.x & 15);
+    const int b = (FWD && !M1F) ? (threadIdx.x % BX) : (threadIdx.x >> 4);
+    int64_t row;
+    int kc;
+    const bool ok = column(b, row, kc);
+#pragma unroll
+    for (int h = 0; h < (Q + 15) / 16; ++h) {
+        if (h) __syncthreads();  // the previous round's reads are done
+#pragma unroll
+        for (int m = 16 * h; m < Q && m < 16 * h + 16; ++m) sh[(m - 16 * h) * PLANE + t * ROW + sb] = v[m];
+        __syncthreads();
+        const int m1 = 16 * h + j;
+        if (m1 < Q && ok) {
+            float2 u[16];
+#pragma unroll
+            for (int tp = 0; tp < 16; ++tp) u[tp] = sh[j * PLANE + tp * ROW + b];
+            dft16(u);  // u[m2] = X[m1 + Q m2] of column b
+            if (FWD) {
+                float2* dst = out + row * G * RC + kc;
+#pragma unroll
+                for (int m2 = 0; m2 < 16; ++m2) dst[(int64_t)(m1 + Q * m2) * RC] = u[m2];
+            } else {
+                float ayz = 0.f;
+                float2* dst = out + image_row(row, kc, ayz);
+                const float sc = factor * ayz;
+#pragma unroll
+                for (int m2 = 0; m2 < 16; ++m2) {
+                    const int n = m1 + Q * m2;
+                    if (n < lo || n >= hi) {
+                        const int p = n < lo ? n + N / 2 : n - G + N / 2;
+                        dst[p] = c_scale(make_float2(u[m2].x, -u[m2].y), sc * apx[p]);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // X pass for G = 512 with any number of B0 segments L (<= X512_B) and any
 // subspace K: a register four-step FFT with 512 = 32 x 16 (32 threads per
@@ -1551,6 +1661,44 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         else
             hipLaunchKernelGGL((k_fftx256<false, X256_BA>), dim3((unsigned)nblk), dim3(16 * X256_BA), 0, s, g, t,
                                X.tw, p->apod, src, dst, KC, YG, nkc, factor);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    if (PF_FFTQ && colq_supported(X.G) && b.L == 1 && !b.b0s) {
+        const int BX = X256_BF;  // 16 columns per block, both directions
+        const int KC = std::min(KCv, BX);
+        const int YG = BX / KC;
+        const int nkc = (KCv + KC - 1) / KC;
+        const int64_t rows = (int64_t)b.nfg * b.Cb * p->shape[0] * p->shape[1];
+        const int64_t nblk = ((rows + YG - 1) / YG) * nkc;
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+        if (nblk < 1) return PF_OK;
+        Geo g = geo_of(p);
+        Batch t = batch_of(b);
+        const dim3 grid((unsigned)nblk), blk(16 * X256_BF);
+#define PF_XQ_CASE(QV)                                                                                           \
+    case QV:                                                                                                     \
+        if (fwd && KC < 8)                                                                                       \
+            hipLaunchKernelGGL((k_fftxq<true, QV, X256_BF, true>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, \
+                               KC, YG, nkc, 1.0f);                                                               \
+        else if (fwd)                                                                                            \
+            hipLaunchKernelGGL((k_fftxq<true, QV, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC,   \
+                               YG, nkc, 1.0f);                                                                   \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_fftxq<false, QV, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC,  \
+                               YG, nkc, factor);                                                                 \
+        break;
+        switch (X.G / 16) {
+            PF_XQ_CASE(8)
+            PF_XQ_CASE(10)
+            PF_XQ_CASE(12)
+            PF_XQ_CASE(15)
+            PF_XQ_CASE(20)
+            PF_XQ_CASE(24)
+            default:
+                return pf_fail(PF_ERR_FFT, "no register row pass for this size");
+        }
+#undef PF_XQ_CASE
         PF_HIP(hipGetLastError());
         return PF_OK;
     }

@@ -52,6 +52,11 @@ CASES = [
     ((150, 8), 1.6),       # 240 x 13
     ((256, 12), 1.25),     # 320 x 15: 16 x 20
     ((10, 192, 12), 2.0),  # 20 x 384 x 24: y is a 16 x 24 register column pass
+    ((12, 64), 2.0),       # 24 x 128: x is a 16 x 8 register row pass
+    ((10, 192), 1.25),     # 13 x 240: 16 x 15
+    ((8, 256), 1.25),      # 10 x 320: 16 x 20
+    ((6, 192), 2.0),       # 12 x 384: 16 x 24
+    ((4, 6, 100), 1.6),    # 7 x 10 x 160: 100 of the 160 x points carry data
 ]
 # (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
 # values per grid point (second-stage lanes m1-fastest) and with 4
@@ -61,6 +66,8 @@ PARAMS = [(s, o, 3) for s, o in CASES] + [
     ((8, 200), 2.56, 2),
     ((6, 8, 256), 2.0, 2),
     ((6, 8, 256), 2.0, 4),
+    ((6, 8, 128), 1.25, 2),  # 160-point row pass storing m1-fastest (coil pairs)
+    ((6, 8, 192), 2.0, 16),
 ]
 
 
