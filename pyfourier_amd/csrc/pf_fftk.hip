@@ -556,6 +556,9 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_fftcol256(int N
 #ifndef PF_FFTQ
 #define PF_FFTQ 1  // register four-step column passes for 16 Q-point axes
 #endif
+#ifndef PF_FFT512X32
+#define PF_FFT512X32 1  // 512-point inverse row passes with one segment as 16 threads x 32 values
+#endif
 constexpr int C512_B = 32;        // columns per block (512 threads)
 constexpr int C512_BS = 16;       // k_fftcol512_t32: columns per block (512 threads)
 constexpr int64_t C512_MIN = 1 << 18;  // columns from which k_fftcol512 runs (measured: 2^15 columns
@@ -765,9 +768,48 @@ __device__ __forceinline__ void dft_pfa(float2* v) {
         for (int k2 = 0; k2 < R; ++k2) v[(ra * k1 + ar * k2) % Q] = u[k1][k2];
 }
 
+// cos and sin of 2 pi m / 32, m = 0..15 (W32^m = c - i s)
+__constant__ float2 c_w32[16] = {
+    {1.0f, 0.0f},
+    {0.98078528040323043f, 0.19509032201612825f},
+    {0.92387953251128674f, 0.38268343236508978f},
+    {0.83146961230254524f, 0.55557023301960218f},
+    {0.70710678118654757f, 0.70710678118654746f},
+    {0.55557023301960229f, 0.83146961230254524f},
+    {0.38268343236508984f, 0.92387953251128674f},
+    {0.19509032201612833f, 0.98078528040323043f},
+    {0.0f, 1.0f},
+    {-0.19509032201612819f, 0.98078528040323043f},
+    {-0.38268343236508973f, 0.92387953251128674f},
+    {-0.55557023301960196f, 0.83146961230254546f},
+    {-0.70710678118654746f, 0.70710678118654757f},
+    {-0.83146961230254535f, 0.55557023301960218f},
+    {-0.92387953251128674f, 0.38268343236508989f},
+    {-0.98078528040323043f, 0.19509032201612861f}};
+
+// forward 32-point DFT in place: two 16-point DFTs over the even and odd
+// elements and a radix-2 step
+__device__ __forceinline__ void dft32(float2* v) {
+    float2 e[16], o[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        e[s] = v[2 * s];
+        o[s] = v[2 * s + 1];
+    }
+    dft16(e);
+    dft16(o);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const float2 w = c_mul(o[m], make_float2(c_w32[m].x, -c_w32[m].y));
+        v[m] = c_add(e[m], w);
+        v[m + 16] = c_sub(e[m], w);
+    }
+}
+
 template <int Q>
 __device__ __forceinline__ void dftQ(float2* v) {
-    if constexpr (Q == 8) dft8(v);
+    if constexpr (Q == 32) dft32(v);
+    else if constexpr (Q == 8) dft8(v);
     else if constexpr (Q == 10) dft_pfa<2, 5>(v);
     else if constexpr (Q == 12) dft_pfa<4, 3>(v);
     else if constexpr (Q == 15) dft_pfa<3, 5>(v);
@@ -977,9 +1019,10 @@ __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g
     }
 }
 
-// X pass for G = 16 Q (the register column passes' sizes) and one B0
-// segment: k_fftx256's lane maps and block shape with the 16 x Q four-step of
-// k_fftcolq (Q values per thread, the transpose in rounds of 16 m1 planes).
+// X pass for G = 16 Q (the register column passes' sizes, and the inverse
+// at Q = 32) and one B0 segment: k_fftx256's lane maps and block shape with
+// the 16 x Q four-step of k_fftcolq (Q values per thread, the transpose in
+// rounds of 16 m1 planes).
 template <bool FWD, int Q, int BX, bool M1F = false>
 __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2* __restrict__ twg,
                                                  const float* __restrict__ apod, const float2* __restrict__ in,
@@ -1664,7 +1707,7 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
-    if (PF_FFTQ && colq_supported(X.G) && b.L == 1 && !b.b0s) {
+    if (PF_FFTQ && (colq_supported(X.G) || (X.G == 512 && PF_FFT512X32 && !fwd)) && b.L == 1 && !b.b0s) {
         const int BX = X256_BF;  // 16 columns per block, both directions
         const int KC = std::min(KCv, BX);
         const int YG = BX / KC;
@@ -1688,6 +1731,12 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
             hipLaunchKernelGGL((k_fftxq<false, QV, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC,  \
                                YG, nkc, factor);                                                                 \
         break;
+        if (X.G == 512) {  // inverse only: the forward's 32-value form measured slower than k_fftx512_fwd
+            hipLaunchKernelGGL((k_fftxq<false, 32, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC, YG,
+                               nkc, factor);
+            PF_HIP(hipGetLastError());
+            return PF_OK;
+        }
         switch (X.G / 16) {
             PF_XQ_CASE(8)
             PF_XQ_CASE(10)
