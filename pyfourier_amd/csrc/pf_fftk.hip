@@ -723,14 +723,15 @@ __global__ void __launch_bounds__(32 * B) k_fftcol512_t32(int N, int64_t I, cons
 }
 
 // ---------------------------------------------------------------------------
-// Column passes for G = 16 Q, Q in {8, 10, 12, 15, 20, 24} (G = 128, 160, 192,
-// 240, 320, 384: the grids of 128-256 point images at oversampling 1.25 or
-// 2), the same register four-step as the 256-point pass: element n = t + 16k
-// of a column is held by thread t (t = 0..15, k = 0..Q-1), which takes the
+// Column passes for G = 16 Q, Q in {8, 10, 12, 15, 20, 24, 30, 40} (G = 128,
+// 160, 192, 240, 320, 384, 480, 640: the grids of 64-512 point images at
+// oversampling 1.25 or 2), the same register four-step as the 256-point
+// pass: element n = t + 16k of a column is held by thread t (t = 0..15,
+// k = 0..Q-1), which takes the
 // Q-point DFT over k and multiplies by exp(-2 pi i t m1 / G); the block
 // transposes through LDS (rounds of 16 m1 planes), and thread t takes the
 // 16-point DFT over t for m1 = t (+16), giving outputs m1 + Q m2.  The Q-point
-// DFTs are Good-Thomas products of two coprime small DFTs (2, 3, 4, 5, 8),
+// DFTs are Good-Thomas products of two coprime small DFTs (2, 3, 4, 5, 6, 8),
 // whose index maps are compile-time register renamings.  Lane = (column b,
 // t), b fastest, B consecutive columns per block.
 // ---------------------------------------------------------------------------
@@ -739,6 +740,9 @@ constexpr int inv_mod(int x, int m) {
         if ((x * y) % m == 1) return y;
     return 1;  // m == 1
 }
+
+template <int R>
+__device__ __forceinline__ void dft_small(float2* v);
 
 // forward Q-point DFT (Q = A R, gcd(A, R) = 1), in place: input index
 // n = (R n1 + A n2) mod Q, output index k = (R (R^-1 mod A) k1 + A (A^-1 mod R) k2) mod Q
@@ -752,13 +756,13 @@ __device__ __forceinline__ void dft_pfa(float2* v) {
 #pragma unroll
         for (int n2 = 0; n2 < R; ++n2) u[n1][n2] = v[(R * n1 + A * n2) % Q];
 #pragma unroll
-    for (int n1 = 0; n1 < A; ++n1) dftR<R>(u[n1]);
+    for (int n1 = 0; n1 < A; ++n1) dft_small<R>(u[n1]);
 #pragma unroll
     for (int k2 = 0; k2 < R; ++k2) {
         float2 c[A];
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) c[n1] = u[n1][k2];
-        dftR<A>(c);
+        dft_small<A>(c);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) u[k1][k2] = c[k1];
     }
@@ -766,6 +770,13 @@ __device__ __forceinline__ void dft_pfa(float2* v) {
     for (int k1 = 0; k1 < A; ++k1)
 #pragma unroll
         for (int k2 = 0; k2 < R; ++k2) v[(ra * k1 + ar * k2) % Q] = u[k1][k2];
+}
+
+// the factors of the Good-Thomas products: 2, 3, 4, 5, 8 and 6 = 2 x 3
+template <int R>
+__device__ __forceinline__ void dft_small(float2* v) {
+    if constexpr (R == 6) dft_pfa<2, 3>(v);
+    else dftR<R>(v);
 }
 
 // cos and sin of 2 pi m / 32, m = 0..15 (W32^m = c - i s)
@@ -814,18 +825,20 @@ __device__ __forceinline__ void dftQ(float2* v) {
     else if constexpr (Q == 12) dft_pfa<4, 3>(v);
     else if constexpr (Q == 15) dft_pfa<3, 5>(v);
     else if constexpr (Q == 20) dft_pfa<4, 5>(v);
-    else dft_pfa<8, 3>(v);  // 24
+    else if constexpr (Q == 24) dft_pfa<8, 3>(v);
+    else if constexpr (Q == 30) dft_pfa<6, 5>(v);
+    else dft_pfa<8, 5>(v);  // 40
 }
 
 constexpr int CQ_B = 32;  // columns per block (512 threads)
 
-// G with a register column pass (16 Q, Q as above; 256 and 512 have their own)
+// G with a register column pass (16 Q, Q as above and 30, 40; 256 and 512 have their own)
 static inline bool colq_supported(int G) {
-    return G == 128 || G == 160 || G == 192 || G == 240 || G == 320 || G == 384;
+    return G == 128 || G == 160 || G == 192 || G == 240 || G == 320 || G == 384 || G == 480 || G == 640;
 }
 
 template <bool FWD, int Q, int B>
-__global__ void __launch_bounds__(16 * B, 4) k_fftcolq(int N, int64_t I, const float2* __restrict__ twg,
+__global__ void __launch_bounds__(16 * B, Q <= 24 ? 4 : 2) k_fftcolq(int N, int64_t I, const float2* __restrict__ twg,
                                                   const float2* __restrict__ in, float2* __restrict__ out,
                                                   ColMask cm) {
     constexpr int G = 16 * Q;
@@ -1744,6 +1757,8 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
             PF_XQ_CASE(15)
             PF_XQ_CASE(20)
             PF_XQ_CASE(24)
+            PF_XQ_CASE(30)
+            PF_XQ_CASE(40)
             default:
                 return pf_fail(PF_ERR_FFT, "no register row pass for this size");
         }
@@ -1863,6 +1878,8 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
             PF_COLQ_CASE(15)
             PF_COLQ_CASE(20)
             PF_COLQ_CASE(24)
+            PF_COLQ_CASE(30)
+            PF_COLQ_CASE(40)
             default:
                 return pf_fail(PF_ERR_FFT, "no register column pass for this size");
         }

@@ -57,6 +57,11 @@ CASES = [
     ((8, 256), 1.25),      # 10 x 320: 16 x 20
     ((6, 192), 2.0),       # 12 x 384: 16 x 24
     ((4, 6, 100), 1.6),    # 7 x 10 x 160: 100 of the 160 x points carry data
+    ((384, 12), 1.25),     # 480 x 15: y is a 16 x 30 register column pass (30 = 6 x 5, 6 = 2 x 3)
+    ((512, 8), 1.25),      # 640 x 10: 16 x 40
+    ((10, 384), 1.25),     # 13 x 480: x is a 16 x 30 register row pass
+    ((6, 512), 1.25),      # 8 x 640: 16 x 40
+    ((384, 10, 8), 1.25),  # 480 x 13 x 10: z
 ]
 # (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
 # values per grid point (second-stage lanes m1-fastest) and with 4
