@@ -190,16 +190,17 @@ def test_small_grids_periodic_wrap(cuda_device, shape, oversamp):
     _check_pair(cuda_device, coord, shape, x, oversamp=oversamp)
 
 
-def test_subspace_and_frames_256_point_rows(cuda_device):
-    """x of 128 at sigma 2 runs the 256-point register row FFT, with a
-    subspace basis (image planes = coefficients) and with per-frame images."""
+@pytest.mark.parametrize("shape,oversamp", [((12, 128), 2.0), ((12, 128), 1.25), ((12, 192), 2.0), ((12, 384), 1.25)])
+def test_subspace_and_frames_register_rows(cuda_device, shape, oversamp):
+    """x axes of 256, 160, 384 and 480 points run the register row FFTs
+    (16 x 16, 16 x 10, 16 x 24, 16 x 30), with a subspace basis (image planes
+    = coefficients) and with per-frame images."""
     rng = np.random.default_rng(21)
-    shape = (12, 128)
     F, K, C = 3, 2, 3
     coord = (rng.uniform(-0.5, 0.5, (F, 2, 150, 2)) * np.array(shape[::-1])).astype(np.float32)
-    coord[0, 0, 0] = [-64, -6]
-    P = R.Plan(coord, shape, width=4, oversamp=2.0)
-    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=2.0)
+    coord[0, 0, 0] = [-shape[1] // 2, -6]
+    P = R.Plan(coord, shape, width=4, oversamp=oversamp)
+    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=oversamp)
     basis = demo.t2_subspace_basis(rng, F, K)
     x = demo.complex_gaussian(rng, (C, K) + shape)
     y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, basis=torch.from_numpy(basis),
@@ -212,6 +213,8 @@ def test_subspace_and_frames_256_point_rows(cuda_device):
     xf = demo.complex_gaussian(rng, (C, F) + shape)
     yf = pf.nufft(torch.from_numpy(xf).to(cuda_device), nufft_plan=plan, device=cuda_device)
     assert rel(yf.cpu().numpy().reshape(C, F, -1), R.nufft(xf, P)) < PARITY
+    xfa = pf.nufft_adj(torch.from_numpy(ya).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    assert rel(xfa.cpu().numpy(), R.nufft_adj(ya.reshape(C, F, -1), P)) < PARITY
 
 
 def test_b0_with_frames_and_subspace_512_point_rows(cuda_device):
