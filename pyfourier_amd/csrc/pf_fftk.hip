@@ -1033,10 +1033,14 @@ __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g
 }
 
 // X pass for G = 16 Q (the register column passes' sizes, and the inverse
-// at Q = 32) and one B0 segment: k_fftx256's lane maps and block shape with
-// the 16 x Q four-step of k_fftcolq (Q values per thread, the transpose in
-// rounds of 16 m1 planes).
-template <bool FWD, int Q, int BX, bool M1F = false>
+// at Q = 32): k_fftx256's lane maps and block shape with the 16 x Q
+// four-step of k_fftcolq (Q values per thread, the transpose in rounds of 16
+// m1 planes).  B0 segments: the forward's columns are (row, v) with
+// v = l*K*Cv + kc, each loading its image row times segment l's map; the
+// inverse (SEG) keeps all L segments of KC (k, cv) pairs of YG rows in a
+// block, writes each segment's cropped row to LDS and sums the segments
+// (times the conjugate maps) before one x-contiguous image write.
+template <bool FWD, int Q, int BX, bool M1F = false, bool SEG = false>
 __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2* __restrict__ twg,
                                                  const float* __restrict__ apod, const float2* __restrict__ in,
                                                  float2* __restrict__ out, int KC, int YG, int nkc, float factor) {
@@ -1045,9 +1049,12 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
     constexpr int ROW = BX + 1, PLANE = 16 * ROW + 4;  // transpose strides (float2), as k_fftx256
     __shared__ float2 sh[NP * PLANE];
     __shared__ float2 tw[G];
+    __shared__ float2 S[SEG ? BX * G : 1];  // SEG: cropped segment rows, S[(yy, kcl)][l][p]
     const int N = g.N[2];
     const int RC = bt.R * bt.Cv;
     const int KCv = bt.K * bt.Cv;
+    const int L = SEG ? bt.L : 1;
+    const int VT = FWD ? bt.L * KCv : KCv;  // forward: columns v = l*KCv + kc
     const int64_t nrows = (int64_t)g.N[0] * g.N[1];
     const int64_t rows_tot = (int64_t)bt.nfg * bt.Cb * nrows;
     const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
@@ -1058,15 +1065,18 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
     const float* apx = apod + g.N[0] + g.N[1];
     for (int k = threadIdx.x; k < G; k += 16 * BX) tw[k] = twg[k];
     const int lo = N - N / 2, hi = G - N / 2;
-    auto column = [&](int b, int64_t& row, int& kc) {
-        const int yy = b / KC;
-        kc = kc0 + b % KC;
+    // column b -> row and v (forward: v = l*KCv + kc; inverse: the pair kc,
+    // SEG: segment l of pair kc)
+    auto column = [&](int b, int64_t& row, int& v) {
+        const int lkc = L * KC;
+        const int yy = b / lkc, r = b % lkc;
+        const int kc = kc0 + r % KC;
+        v = (r / KC) * KCv + kc;
         row = row0 + yy;
-        return yy < YG && row < rows_tot && kc < KCv;
+        return yy < YG && row < rows_tot && kc < VT;
     };
-    auto image_row = [&](int64_t row, int kc, float& ayz) -> int64_t {
+    auto image_row = [&](int64_t row, int kc, float& ayz, int64_t& zy) -> int64_t {
         int fg, cb;
-        int64_t zy;
         x_row_decode(bt, row, nrows, fg, cb, zy);
         const int k = kc / bt.Cv, cv = kc % bt.Cv;
         const int z = (int)(zy / g.N[1]), y = (int)(zy % g.N[1]);
@@ -1080,19 +1090,27 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
     {
         const int b = FWD ? (threadIdx.x >> 4) : (threadIdx.x % BX);
         int64_t row;
-        int kc;
-        const bool ok = column(b, row, kc);
+        int vc;
+        const bool ok = column(b, row, vc);
         if (FWD) {
             float ayz = 0.f;
-            const int64_t base = ok ? image_row(row, kc, ayz) : 0;
+            int64_t zy = 0;
+            const int kc = vc % KCv, l = vc / KCv;
+            const int64_t base = ok ? image_row(row, kc, ayz, zy) : 0;
+            const float2* b0 = (ok && bt.b0s) ? bt.b0s + (int64_t)l * g.ntot + zy * N : nullptr;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
                 const int n = t + 16 * k;
                 const int p = n < lo ? n + N / 2 : n - G + N / 2;
-                v[k] = (ok && (n < lo || n >= hi)) ? c_scale(in[base + p], ayz * apx[p]) : make_float2(0.f, 0.f);
+                float2 x = make_float2(0.f, 0.f);
+                if (ok && (n < lo || n >= hi)) {
+                    x = c_scale(in[base + p], ayz * apx[p]);
+                    if (b0) x = c_mul(x, b0[p]);
+                }
+                v[k] = x;
             }
         } else {
-            const float2* src = in + (ok ? row : 0) * G * RC + kc;
+            const float2* src = in + (ok ? row : 0) * G * RC + vc;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
                 const float2 x = ok ? src[(int64_t)(t + 16 * k) * RC] : make_float2(0.f, 0.f);
@@ -1108,8 +1126,8 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
     const int j = (FWD && !M1F) ? (threadIdx.x / BX) : (threadIdx.x & 15);
     const int b = (FWD && !M1F) ? (threadIdx.x % BX) : (threadIdx.x >> 4);
     int64_t row;
-    int kc;
-    const bool ok = column(b, row, kc);
+    int vc;
+    const bool ok = column(b, row, vc);
 #pragma unroll
     for (int h = 0; h < (Q + 15) / 16; ++h) {
         if (h) __syncthreads();  // the previous round's reads are done
@@ -1123,12 +1141,21 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
             for (int tp = 0; tp < 16; ++tp) u[tp] = sh[j * PLANE + tp * ROW + b];
             dft16(u);  // u[m2] = X[m1 + Q m2] of column b
             if (FWD) {
-                float2* dst = out + row * G * RC + kc;
+                float2* dst = out + row * G * RC + vc;
 #pragma unroll
                 for (int m2 = 0; m2 < 16; ++m2) dst[(int64_t)(m1 + Q * m2) * RC] = u[m2];
+            } else if (SEG) {
+                const int lkc = L * KC, r = b % lkc;  // b = (yy, l, kcl) -> segment row (yy*KC + kcl)*L + l
+                float2* Sr = S + ((int64_t)((b / lkc) * KC + r % KC) * L + r / KC) * N;
+#pragma unroll
+                for (int m2 = 0; m2 < 16; ++m2) {
+                    const int n = m1 + Q * m2;
+                    if (n < lo || n >= hi) Sr[n < lo ? n + N / 2 : n - G + N / 2] = make_float2(u[m2].x, -u[m2].y);
+                }
             } else {
                 float ayz = 0.f;
-                float2* dst = out + image_row(row, kc, ayz);
+                int64_t zy;
+                float2* dst = out + image_row(row, vc, ayz, zy);
                 const float sc = factor * ayz;
 #pragma unroll
                 for (int m2 = 0; m2 < 16; ++m2) {
@@ -1139,6 +1166,28 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
                     }
                 }
             }
+        }
+    }
+    if constexpr (SEG) {
+        __syncthreads();
+        const int NC = YG * KC;  // (row, kc) image rows of this block
+        for (int it = threadIdx.x; it < NC * N; it += 16 * BX) {
+            const int p = it % N, cc = it / N;
+            const int64_t rw = row0 + cc / KC;
+            const int kc = kc0 + cc % KC;
+            if (rw >= rows_tot || kc >= KCv) continue;
+            float ayz = 0.f;
+            int64_t zy;
+            const int64_t base = image_row(rw, kc, ayz, zy);
+            const float2* Sp = S + (int64_t)cc * L * N + p;
+            float2 acc = make_float2(0.f, 0.f);
+            if (bt.b0s) {
+                const float2* b0 = bt.b0s + zy * N + p;
+                for (int l = 0; l < L; ++l) acc = c_add(acc, c_mulc(Sp[l * N], b0[(int64_t)l * g.ntot]));
+            } else {
+                for (int l = 0; l < L; ++l) acc = c_add(acc, Sp[l * N]);
+            }
+            out[base + p] = c_scale(acc, factor * ayz * apx[p]);
         }
     }
 }
@@ -1720,11 +1769,21 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
-    if (PF_FFTQ && (colq_supported(X.G) || (X.G == 512 && PF_FFT512X32 && !fwd)) && b.L == 1 && !b.b0s) {
+    const bool x512inv = X.G == 512 && PF_FFT512X32 && !fwd && b.L == 1 && !b.b0s;
+    if (PF_FFTQ && (colq_supported(X.G) || x512inv) && b.L <= X256_BF) {
         const int BX = X256_BF;  // 16 columns per block, both directions
-        const int KC = std::min(KCv, BX);
-        const int YG = BX / KC;
-        const int nkc = (KCv + KC - 1) / KC;
+        const bool seg = !fwd && (b.L > 1 || b.b0s);  // inverse summing field-map segments
+        int KC, YG, nkc;
+        if (seg) {  // all L segments of KC pairs (of YG rows when they fit) per block
+            KC = std::min(KCv, std::max(1, BX / b.L));
+            YG = KC == KCv ? std::max(1, BX / (b.L * KCv)) : 1;
+            nkc = (KCv + KC - 1) / KC;
+        } else {  // forward columns v = l*KCv + kc over all segments
+            const int VT = fwd ? b.L * KCv : KCv;
+            KC = std::min(VT, BX);
+            YG = BX / KC;
+            nkc = (VT + KC - 1) / KC;
+        }
         const int64_t rows = (int64_t)b.nfg * b.Cb * p->shape[0] * p->shape[1];
         const int64_t nblk = ((rows + YG - 1) / YG) * nkc;
         if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
@@ -1732,24 +1791,27 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         Geo g = geo_of(p);
         Batch t = batch_of(b);
         const dim3 grid((unsigned)nblk), blk(16 * X256_BF);
-#define PF_XQ_CASE(QV)                                                                                           \
-    case QV:                                                                                                     \
-        if (fwd && KC < 8)                                                                                       \
-            hipLaunchKernelGGL((k_fftxq<true, QV, X256_BF, true>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, \
-                               KC, YG, nkc, 1.0f);                                                               \
-        else if (fwd)                                                                                            \
-            hipLaunchKernelGGL((k_fftxq<true, QV, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC,   \
-                               YG, nkc, 1.0f);                                                                   \
-        else                                                                                                     \
-            hipLaunchKernelGGL((k_fftxq<false, QV, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC,  \
-                               YG, nkc, factor);                                                                 \
-        break;
-        if (X.G == 512) {  // inverse only: the forward's 32-value form measured slower than k_fftx512_fwd
+        if (x512inv) {  // inverse only: the forward's 32-value form measured slower than k_fftx512_fwd
             hipLaunchKernelGGL((k_fftxq<false, 32, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC, YG,
                                nkc, factor);
             PF_HIP(hipGetLastError());
             return PF_OK;
         }
+#define PF_XQ_CASE(QV)                                                                                          \
+    case QV:                                                                                                    \
+        if (fwd && KC < 8)                                                                                      \
+            hipLaunchKernelGGL((k_fftxq<true, QV, X256_BF, true>), grid, blk, 0, s, g, t, X.tw, p->apod, src,     \
+                               dst, KC, YG, nkc, 1.0f);                                                         \
+        else if (fwd)                                                                                           \
+            hipLaunchKernelGGL((k_fftxq<true, QV, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC,  \
+                               YG, nkc, 1.0f);                                                                  \
+        else if (seg)                                                                                           \
+            hipLaunchKernelGGL((k_fftxq<false, QV, X256_BF, false, true>), grid, blk, 0, s, g, t, X.tw, p->apod,  \
+                               src, dst, KC, YG, nkc, factor);                                                  \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_fftxq<false, QV, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC, \
+                               YG, nkc, factor);                                                                \
+        break;
         switch (X.G / 16) {
             PF_XQ_CASE(8)
             PF_XQ_CASE(10)

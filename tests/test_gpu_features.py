@@ -217,24 +217,24 @@ def test_subspace_and_frames_register_rows(cuda_device, shape, oversamp):
     assert rel(xfa.cpu().numpy(), R.nufft_adj(ya.reshape(C, F, -1), P)) < PARITY
 
 
-def test_b0_with_frames_and_subspace_512_point_rows(cuda_device):
-    """x of 256 at sigma 2 runs the 512-point register row FFT with B0
-    segments, per frame and with a subspace basis.  3 coils: the forward's
+@pytest.mark.parametrize("shape,oversamp", [((10, 256), 2.0), ((10, 128), 1.25), ((10, 256), 1.25), ((10, 192), 2.0)])
+def test_b0_with_frames_and_subspace_register_rows(cuda_device, shape, oversamp):
+    """x axes of 512, 160, 320 and 384 points run the register row FFTs with
+    B0 segments, per frame and with a subspace basis.  3 coils: the forward's
     16-column blocks span grid rows (15 or 30 values per grid point); with
     the basis the adjoint splits the 6 (k, coil) pairs over two blocks, each
     summing all 5 segments."""
     rng = np.random.default_rng(22)
-    shape = (10, 256)
     F, K, C, L = 3, 2, 3, 5
     coord = (rng.uniform(-0.5, 0.5, (F, 2, 150, 2)) * np.array(shape[::-1])).astype(np.float32)
-    coord[0, 0, 0] = [-128, -5]
+    coord[0, 0, 0] = [-shape[1] // 2, -5]
     zmap = demo.smooth_field_map(rng, shape, 100.0)
     t = np.linspace(0, 8e-3, 150).astype(np.float32)
-    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=2.0, zmap=torch.from_numpy(zmap),
+    plan = pf.plan_nufft(torch.from_numpy(coord), shape, width=4, oversamp=oversamp, zmap=torch.from_numpy(zmap),
                          T=torch.from_numpy(t), L=L, nbins=40)
     b0 = (plan.zmap_t_kernel.cpu().numpy().astype(np.complex128),
           plan.zmap_s_kernel.cpu().numpy().astype(np.complex128))
-    P = R.Plan(coord, shape, width=4, oversamp=2.0)
+    P = R.Plan(coord, shape, width=4, oversamp=oversamp)
     basis = demo.t2_subspace_basis(rng, F, K)
     for bas in (None, basis):
         tb = None if bas is None else torch.from_numpy(bas)
