@@ -14,10 +14,15 @@
 //   the X pass writing the deapodised image (sum over B0 segments with the
 //   conjugate maps) in the user layout.
 //
-// Each block transforms B columns of one line family held in LDS as
-// [G][B+1] complex, with an in-place mixed-radix (8, 5, 4, 3, 2) Stockham
-// FFT (Govindaraju et al., "High performance discrete Fourier transforms on
-// graphics processors", SC 2008); twiddles come from a per-axis table of
+// Lines of G = 16 Q points (Q in {8, 10, 12, 15, 16, 20, 24, 30, 32, 40}:
+// 128-640) run register four-step kernels: each thread holds Q (or 16)
+// values of a line, takes a small DFT in registers, twiddles, and one LDS
+// transpose hands the line to the second small DFT (k_fftcol256/512,
+// k_fftcolq, k_fftx256, k_fftx512_*, k_fftxq).  Other sizes: each block
+// transforms B columns of one line family held in LDS as [G][B+1] complex,
+// with an in-place mixed-radix (8, 5, 4, 3, 2) Stockham FFT (Govindaraju et
+// al., "High performance discrete Fourier transforms on graphics
+// processors", SC 2008).  Twiddles come from a per-axis table of
 // exp(-2 pi i k / G) computed in double on the host.  The inverse uses
 // conj(FFT(conj(x))).
 #include <algorithm>
