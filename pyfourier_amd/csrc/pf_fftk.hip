@@ -14,8 +14,8 @@
 //   the X pass writing the deapodised image (sum over B0 segments with the
 //   conjugate maps) in the user layout.
 //
-// Lines of G = 16 Q points (Q in {8, 10, 12, 15, 16, 20, 24, 30, 32, 40}:
-// 128-640) run register four-step kernels: each thread holds Q (or 16)
+// Lines of G = 16 Q points (Q in {8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 30,
+// 32, 36, 40}: 128-640) run register four-step kernels: each thread holds Q (or 16)
 // values of a line, takes a small DFT in registers, twiddles, and one LDS
 // transpose hands the line to the second small DFT (k_fftcol256/512,
 // k_fftcolq, k_fftx256, k_fftx512_*, k_fftxq).  Other sizes: each block
@@ -728,15 +728,16 @@ __global__ void __launch_bounds__(32 * B) k_fftcol512_t32(int N, int64_t I, cons
 }
 
 // ---------------------------------------------------------------------------
-// Column passes for G = 16 Q, Q in {8, 10, 12, 15, 20, 24, 30, 40} (G = 128,
-// 160, 192, 240, 320, 384, 480, 640: the grids of 64-512 point images at
-// oversampling 1.25 or 2), the same register four-step as the 256-point
+// Column passes for G = 16 Q, Q in {8, 9, 10, 12, 15, 18, 20, 24, 25, 30, 36,
+// 40} (G = 128-640: the grids of 64-512 point images at oversampling 1.25
+// or 2), the same register four-step as the 256-point
 // pass: element n = t + 16k of a column is held by thread t (t = 0..15,
 // k = 0..Q-1), which takes the
 // Q-point DFT over k and multiplies by exp(-2 pi i t m1 / G); the block
 // transposes through LDS (rounds of 16 m1 planes), and thread t takes the
 // 16-point DFT over t for m1 = t (+16), giving outputs m1 + Q m2.  The Q-point
-// DFTs are Good-Thomas products of two coprime small DFTs (2, 3, 4, 5, 6, 8),
+// DFTs are Good-Thomas products of two coprime small DFTs (2, 3, 4, 5, 6, 8,
+// 9) or Cooley-Tukey 3 x 3 and 5 x 5,
 // whose index maps are compile-time register renamings.  Lane = (column b,
 // t), b fastest, B consecutive columns per block.
 // ---------------------------------------------------------------------------
@@ -777,10 +778,51 @@ __device__ __forceinline__ void dft_pfa(float2* v) {
         for (int k2 = 0; k2 < R; ++k2) v[(ra * k1 + ar * k2) % Q] = u[k1][k2];
 }
 
-// the factors of the Good-Thomas products: 2, 3, 4, 5, 8 and 6 = 2 x 3
+// cos and sin of 2 pi j / 9 (j = 0..4) and 2 pi j / 25 (j = 0..16): the
+// twiddles W_Q^(n2 k1) of the Cooley-Tukey 3 x 3 and 5 x 5 DFTs
+__constant__ float2 c_w9[5] = {
+    {1.0f, 0.0f}, {0.76604444311897801f, 0.64278760968653925f}, {0.17364817766693041f, 0.98480775301220802f},
+    {-0.49999999999999978f, 0.86602540378443871f}, {-0.93969262078590832f, 0.34202014332566888f}};
+__constant__ float2 c_w25[17] = {
+    {1.0f, 0.0f}, {0.96858316112863108f, 0.24868988716485479f}, {0.87630668004386358f, 0.48175367410171532f},
+    {0.72896862742141155f, 0.68454710592868862f}, {0.53582679497899655f, 0.84432792550201508f}, {0.30901699437494745f, 0.95105651629515353f},
+    {0.062790519529313527f, 0.99802672842827156f}, {-0.1873813145857246f, 0.98228725072868872f}, {-0.42577929156507272f, 0.90482705246601947f},
+    {-0.63742398974868975f, 0.77051324277578925f}, {-0.80901699437494734f, 0.58778525229247325f}, {-0.92977648588825135f, 0.36812455268467814f},
+    {-0.99211470131447776f, 0.12533323356430454f}, {-0.99211470131447788f, -0.12533323356430429f}, {-0.92977648588825146f, -0.36812455268467792f},
+    {-0.80901699437494778f, -0.58778525229247269f}, {-0.63742398974868952f, -0.77051324277578936f}};
+
+// forward Q-point DFT (Q = A R, A = R = 3 or 5) by Cooley-Tukey: n = R n1 + n2,
+// k = k1 + A k2; A-point DFTs over n1, twiddle W_Q^(n2 k1), R-point DFTs over n2
+template <int A, int R>
+__device__ __forceinline__ void dft_ct(float2* v) {
+    static_assert((A == 3 && R == 3) || (A == 5 && R == 5), "twiddle tables for 9 and 25 only");
+    const float2* w = A == 3 ? c_w9 : c_w25;
+    float2 u[R][A];
+#pragma unroll
+    for (int n2 = 0; n2 < R; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) u[n2][n1] = v[R * n1 + n2];
+        dftR<A>(u[n2]);
+#pragma unroll
+        for (int k1 = 1; k1 < A; ++k1)
+            if (n2 > 0) u[n2][k1] = c_mul(u[n2][k1], make_float2(w[n2 * k1].x, -w[n2 * k1].y));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+        float2 c[R];
+#pragma unroll
+        for (int n2 = 0; n2 < R; ++n2) c[n2] = u[n2][k1];
+        dftR<R>(c);
+#pragma unroll
+        for (int k2 = 0; k2 < R; ++k2) v[k1 + A * k2] = c[k2];
+    }
+}
+
+// the factors of the Good-Thomas products: 2, 3, 4, 5, 8, 6 = 2 x 3 and 9 = 3 x 3
 template <int R>
 __device__ __forceinline__ void dft_small(float2* v) {
     if constexpr (R == 6) dft_pfa<2, 3>(v);
+    else if constexpr (R == 9) dft_ct<3, 3>(v);
     else dftR<R>(v);
 }
 
@@ -832,14 +874,19 @@ __device__ __forceinline__ void dftQ(float2* v) {
     else if constexpr (Q == 20) dft_pfa<4, 5>(v);
     else if constexpr (Q == 24) dft_pfa<8, 3>(v);
     else if constexpr (Q == 30) dft_pfa<6, 5>(v);
-    else dft_pfa<8, 5>(v);  // 40
+    else if constexpr (Q == 40) dft_pfa<8, 5>(v);
+    else if constexpr (Q == 9) dft_ct<3, 3>(v);
+    else if constexpr (Q == 18) dft_pfa<2, 9>(v);
+    else if constexpr (Q == 25) dft_ct<5, 5>(v);
+    else dft_pfa<4, 9>(v);  // 36
 }
 
 constexpr int CQ_B = 32;  // columns per block (512 threads)
 
 // G with a register column pass (16 Q, Q as above and 30, 40; 256 and 512 have their own)
 static inline bool colq_supported(int G) {
-    return G == 128 || G == 160 || G == 192 || G == 240 || G == 320 || G == 384 || G == 480 || G == 640;
+    return G == 128 || G == 144 || G == 160 || G == 192 || G == 240 || G == 288 || G == 320 || G == 384 ||
+           G == 400 || G == 480 || G == 576 || G == 640;
 }
 
 template <bool FWD, int Q, int B>
@@ -1824,7 +1871,11 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
             PF_XQ_CASE(15)
             PF_XQ_CASE(20)
             PF_XQ_CASE(24)
+            PF_XQ_CASE(9)
+            PF_XQ_CASE(18)
+            PF_XQ_CASE(25)
             PF_XQ_CASE(30)
+            PF_XQ_CASE(36)
             PF_XQ_CASE(40)
             default:
                 return pf_fail(PF_ERR_FFT, "no register row pass for this size");
@@ -1945,7 +1996,11 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
             PF_COLQ_CASE(15)
             PF_COLQ_CASE(20)
             PF_COLQ_CASE(24)
+            PF_COLQ_CASE(9)
+            PF_COLQ_CASE(18)
+            PF_COLQ_CASE(25)
             PF_COLQ_CASE(30)
+            PF_COLQ_CASE(36)
             PF_COLQ_CASE(40)
             default:
                 return pf_fail(PF_ERR_FFT, "no register column pass for this size");

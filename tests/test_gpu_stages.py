@@ -62,6 +62,14 @@ CASES = [
     ((10, 384), 1.25),     # 13 x 480: x is a 16 x 30 register row pass
     ((6, 512), 1.25),      # 8 x 640: 16 x 40
     ((384, 10, 8), 1.25),  # 480 x 13 x 10: z
+    ((72, 20), 2.0),       # 144 x 40: 16 x 9 (Cooley-Tukey 3 x 3)
+    ((144, 12), 2.0),      # 288 x 24: 16 x 18 (2 x 9)
+    ((320, 10), 1.25),     # 400 x 13: 16 x 25 (Cooley-Tukey 5 x 5)
+    ((288, 10), 2.0),      # 576 x 20: 16 x 36 (4 x 9)
+    ((10, 72), 2.0),       # 20 x 144: the same as row passes
+    ((8, 144), 2.0),       # 16 x 288
+    ((6, 320), 1.25),      # 8 x 400
+    ((6, 288), 2.0),       # 12 x 576
 ]
 # (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
 # values per grid point (second-stage lanes m1-fastest) and with 4
