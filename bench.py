@@ -801,7 +801,8 @@ def bench_default_params(pf, dev, steps):
     oversampling 1.25 (a 160^3 grid, register 16 x 10-point FFT passes):
     fwd+adj (fused drop-in calls, wall clock), the stage-by-stage times (HIP
     events) and the forward's error against the FP64 NUDFT on 1024 samples
-    of coil 0."""
+    of coil 0.  Then the 2-D B0 workload's forward and adjoint at the
+    field-map defaults (6 segments, width 4, oversampling 1.25)."""
     import torch
 
     from pyfourier_amd._nufft import stage_timer
@@ -824,9 +825,32 @@ def bench_default_params(pf, dev, steps):
     stages = {k: round(float(np.mean(v)), 4) for k, v in timer.summary().items()}
     y0 = pf.nufft(xc_dev[:1], nufft_plan=plan, device=dev)
     err = accuracy(pf, plan, x, xc_dev, y0, plan.interpolator.coord.reshape(-1, 3).numpy(), N, dev)
-    return {"nufft3d_128cube_32coil_w4_os1.25": {
+    out = {"nufft3d_128cube_32coil_w4_os1.25": {
         "grid": list(plan.interpolator.os_shape), "ms_per_fwd_adj": round(ms, 3),
         "fwd_adj_per_s": round(1000.0 / ms, 1), "stages_ms": stages, "fwd_rel_err_vs_nudft": float(f"{err:.2e}")}}
+    del plan, xc_dev, w_dev
+    torch.cuda.empty_cache()
+
+    # the B0 workload's operators (256^2 x 8 coils, 16 spirals x 4096, 10 ms,
+    # +-100 Hz map) at the reference's field-map defaults: 6 segments, width
+    # 4, oversampling 1.25 (320-point rows with segments)
+    from pyfourier_amd import demo
+
+    rng = np.random.default_rng(7)
+    N2, C2 = 256, 8
+    coord2 = demo.variable_density_spiral_2d(16, 4096, N2)
+    zmap = torch.from_numpy(demo.smooth_field_map(rng, (N2, N2), 100.0))
+    t = torch.linspace(0, 10e-3, 4096)
+    x2 = torch.from_numpy(demo.complex_gaussian(rng, (C2, N2, N2))).to(dev)
+    plan2 = pf.plan_nufft(torch.from_numpy(coord2), N2, zmap=zmap, T=t).to(dev)
+    y2 = pf.nufft(x2, nufft_plan=plan2, device=dev)
+    n = max(20, steps * 4)
+    t_f = _time(lambda: pf.nufft(x2, nufft_plan=plan2, device=dev), dev, n, warmup=3)
+    t_a = _time(lambda: pf.nufft_adj(y2, nufft_plan=plan2, device=dev), dev, n, warmup=3)
+    out["b0_2d_256_8coil_spiral16_defaults"] = {
+        "segments": int(plan2.zmap_s_kernel.shape[0]), "grid": list(plan2.interpolator.os_shape),
+        "fwd_ms": round(t_f, 4), "adj_ms": round(t_a, 4)}
+    return out
 
 
 EXTRAS = (("cfg1_latency", bench_cfg1_latency), ("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
