@@ -1540,6 +1540,63 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_cartcol256(int6
     }
 }
 
+// column pass, G = 16 Q (128-640 points): the four-step of k_fftcolq with
+// the centred reads and writes and the mask / factor epilogue of k_cartcol256
+template <bool FWD, int Q, int B>
+__global__ void __launch_bounds__(16 * B) k_cartcolq(int64_t I, const float2* __restrict__ twg, const float2* in,
+                                                   float2* out, CartEpi e) {
+    constexpr int G = 16 * Q;
+    constexpr int NP = Q < 16 ? Q : 16;
+    constexpr int PLANE = 16 * B;  // a 32-lane access covers 32 consecutive slots: no bank conflicts
+    __shared__ float2 sh[NP * PLANE];
+    __shared__ float2 tw[G];
+    const int b = threadIdx.x % B, t = threadIdx.x / B;
+    const int64_t nchunk = (I + B - 1) / B;
+    const int64_t blk = pf_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t o = blk / nchunk;
+    const int64_t i = (blk % nchunk) * B + b;
+    const bool bok = i < I;
+    for (int k = threadIdx.x; k < G; k += 16 * B) tw[k] = twg[k];
+    const float2* src = in + o * G * I + i;
+    float2 v[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int j = crot(t + 16 * k, G);
+        float2 x = bok ? src[(int64_t)j * I] : make_float2(0.f, 0.f);
+        if (!FWD) {
+            if (bok) x = c_scale(x, cmask(e, o, G, j, I, i));
+            x = make_float2(x.x, -x.y);
+        }
+        v[k] = x;
+    }
+    dftQ<Q>(v);
+    __syncthreads();  // twiddle table
+#pragma unroll
+    for (int m = 1; m < Q; ++m) v[m] = c_mul(v[m], tw[t * m]);
+    float2* dst = out + o * G * I + i;
+#pragma unroll
+    for (int h = 0; h < (Q + 15) / 16; ++h) {
+        if (h) __syncthreads();  // the previous round's reads are done
+#pragma unroll
+        for (int m = 16 * h; m < Q && m < 16 * h + 16; ++m) sh[(m - 16 * h) * PLANE + t * B + b] = v[m];
+        __syncthreads();
+        const int m1 = 16 * h + t;
+        if (m1 < Q && bok) {
+            float2 u[16];
+#pragma unroll
+            for (int tp = 0; tp < 16; ++tp) u[tp] = sh[t * PLANE + tp * B + b];
+            dft16(u);  // u[m2] = X[m1 + Q m2]
+#pragma unroll
+            for (int m2 = 0; m2 < 16; ++m2) {
+                const int j = crot(m1 + Q * m2, G);
+                const float2 y = FWD ? u[m2] : make_float2(u[m2].x, -u[m2].y);
+                const float sc = FWD ? e.factor * cmask(e, o, G, j, I, i) : e.factor;
+                dst[(int64_t)j * I] = c_scale(y, sc);
+            }
+        }
+    }
+}
+
 // column pass, any supported G: Stockham in LDS over B columns
 template <bool FWD>
 __global__ void __launch_bounds__(FT, PF_FFT_WPE) k_cartcol(AxisFFT A, int64_t I, int B, int bshift, const float2* in,
@@ -1626,6 +1683,58 @@ __global__ void __launch_bounds__(256, PF_FFT_WPE) k_cartrow256(int64_t nrows, c
         const float2 y = FWD ? v[m2] : make_float2(v[m2].x, -v[m2].y);
         const float sc = FWD ? e.factor * cmask(e, row, 1, 0, G, j) : e.factor;
         out[row * G + j] = c_scale(y, sc);
+    }
+}
+
+// row pass, G = 16 Q: 16 rows per block, k_cartrow256's lane maps with the
+// 16 x Q four-step (transpose rounds of 16 m1 planes)
+template <bool FWD, int Q>
+__global__ void __launch_bounds__(256) k_cartrowq(int64_t nrows, const float2* __restrict__ twg, const float2* in,
+                                                float2* out, CartEpi e) {
+    constexpr int G = 16 * Q;
+    constexpr int NP = Q < 16 ? Q : 16;
+    __shared__ float2 sh[NP * X256_PLANE];
+    __shared__ float2 tw[G];
+    const int64_t r0 = (int64_t)pf_xcd_remap(blockIdx.x, gridDim.x) * 16;
+    for (int k = threadIdx.x; k < G; k += 256) tw[k] = twg[k];
+    const int t = threadIdx.x & 15, b = threadIdx.x >> 4;
+    const int64_t row = r0 + b;
+    const bool ok = row < nrows;
+    float2 v[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int j = crot(t + 16 * k, G);
+        float2 x = ok ? in[row * G + j] : make_float2(0.f, 0.f);
+        if (!FWD) {
+            if (ok) x = c_scale(x, cmask(e, row, 1, 0, G, j));
+            x = make_float2(x.x, -x.y);
+        }
+        v[k] = x;
+    }
+    dftQ<Q>(v);
+    __syncthreads();  // twiddle table
+#pragma unroll
+    for (int m = 1; m < Q; ++m) v[m] = c_mul(v[m], tw[t * m]);
+#pragma unroll
+    for (int h = 0; h < (Q + 15) / 16; ++h) {
+        if (h) __syncthreads();  // the previous round's reads are done
+#pragma unroll
+        for (int m = 16 * h; m < Q && m < 16 * h + 16; ++m) sh[(m - 16 * h) * X256_PLANE + t * X256_ROW + b] = v[m];
+        __syncthreads();
+        const int m1 = 16 * h + t;  // second stage: thread t takes m1 = 16 h + t of row b
+        if (m1 < Q && ok) {
+            float2 u[16];
+#pragma unroll
+            for (int tp = 0; tp < 16; ++tp) u[tp] = sh[t * X256_PLANE + tp * X256_ROW + b];
+            dft16(u);  // u[m2] = X[m1 + Q m2] of row b
+#pragma unroll
+            for (int m2 = 0; m2 < 16; ++m2) {
+                const int j = crot(m1 + Q * m2, G);
+                const float2 y = FWD ? u[m2] : make_float2(u[m2].x, -u[m2].y);
+                const float sc = FWD ? e.factor * cmask(e, row, 1, 0, G, j) : e.factor;
+                out[row * G + j] = c_scale(y, sc);
+            }
+        }
     }
 }
 
@@ -2094,6 +2203,40 @@ static int cart_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, 
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
+    if (PF_FFTQ && (colq_supported(X.G) || X.G == 512)) {
+        constexpr int B = CC256_BF;
+        const int64_t nblk = O * ((I + B - 1) / B);
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+        if (nblk < 1) return PF_OK;
+        const dim3 grid((unsigned)nblk), blk(16 * B);
+#define PF_CCQ_CASE(QV)                                                                                       \
+    case QV:                                                                                                  \
+        if (fwd)                                                                                              \
+            hipLaunchKernelGGL((k_cartcolq<true, QV, B>), grid, blk, 0, s, I, X.tw, src, dst, e);            \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_cartcolq<false, QV, B>), grid, blk, 0, s, I, X.tw, src, dst, e);           \
+        break;
+        switch (X.G / 16) {
+            PF_CCQ_CASE(8)
+            PF_CCQ_CASE(9)
+            PF_CCQ_CASE(10)
+            PF_CCQ_CASE(12)
+            PF_CCQ_CASE(15)
+            PF_CCQ_CASE(18)
+            PF_CCQ_CASE(20)
+            PF_CCQ_CASE(24)
+            PF_CCQ_CASE(25)
+            PF_CCQ_CASE(30)
+            PF_CCQ_CASE(32)
+            PF_CCQ_CASE(36)
+            PF_CCQ_CASE(40)
+            default:
+                return pf_fail(PF_ERR_FFT, "no register Cartesian column pass for this size");
+        }
+#undef PF_CCQ_CASE
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
     AxisFFT A = axis_desc(X);
     const int B = cols_for(X.G);
     int bshift = 0;
@@ -2132,6 +2275,39 @@ static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src
             hipLaunchKernelGGL(k_cartrow256<true>, dim3((unsigned)nblk), dim3(256), 0, s, nrows, X.tw, src, dst, e);
         else
             hipLaunchKernelGGL(k_cartrow256<false>, dim3((unsigned)nblk), dim3(256), 0, s, nrows, X.tw, src, dst, e);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    if (PF_FFTQ && (colq_supported(X.G) || X.G == 512)) {
+        const int64_t nblk = (nrows + 15) / 16;
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+        if (nblk < 1) return PF_OK;
+        const dim3 grid((unsigned)nblk), blk(256);
+#define PF_CRQ_CASE(QV)                                                                                       \
+    case QV:                                                                                                  \
+        if (fwd)                                                                                              \
+            hipLaunchKernelGGL((k_cartrowq<true, QV>), grid, blk, 0, s, nrows, X.tw, src, dst, e);           \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_cartrowq<false, QV>), grid, blk, 0, s, nrows, X.tw, src, dst, e);          \
+        break;
+        switch (X.G / 16) {
+            PF_CRQ_CASE(8)
+            PF_CRQ_CASE(9)
+            PF_CRQ_CASE(10)
+            PF_CRQ_CASE(12)
+            PF_CRQ_CASE(15)
+            PF_CRQ_CASE(18)
+            PF_CRQ_CASE(20)
+            PF_CRQ_CASE(24)
+            PF_CRQ_CASE(25)
+            PF_CRQ_CASE(30)
+            PF_CRQ_CASE(32)
+            PF_CRQ_CASE(36)
+            PF_CRQ_CASE(40)
+            default:
+                return pf_fail(PF_ERR_FFT, "no register Cartesian row pass for this size");
+        }
+#undef PF_CRQ_CASE
         PF_HIP(hipGetLastError());
         return PF_OK;
     }

@@ -51,6 +51,14 @@ CASES = [
     ("rows_256", (256, 256), 2, 1, 1, 0, None),
     ("many_coils", (16, 64), 96, 1, 1, 0, None),
     ("coil_blocks", (8, 40), 130, 2, 2, 0, "ortho"),
+    # 16 x Q-point register passes (column and row)
+    ("q_sizes_2d", (160, 320), 3, 1, 1, 0, "ortho"),
+    ("q_sizes_3d", (144, 24, 240), 2, 1, 1, 0, None),
+    ("q_512", (512, 512), 2, 1, 1, 0, None),
+    ("q_mask_frames", (192, 128), 2, 3, 3, 0, None),
+    ("q_basis", (128, 160), 2, 6, 6, 2, None),
+    ("q_640_400", (640, 400), 1, 1, 1, 0, "ortho"),
+    ("q_288_576", (288, 576), 1, 1, 0, 0, None),
 ]
 
 
