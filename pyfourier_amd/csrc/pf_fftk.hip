@@ -14,8 +14,8 @@
 //   the X pass writing the deapodised image (sum over B0 segments with the
 //   conjugate maps) in the user layout.
 //
-// Lines of G = 16 Q points (Q in {8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 30,
-// 32, 36, 40}: 128-640) run register four-step kernels: each thread holds Q (or 16)
+// Lines of G = 16 Q points (Q in {5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25,
+// 30, 32, 36, 40}: 80-640) run register four-step kernels: each thread holds Q (or 16)
 // values of a line, takes a small DFT in registers, twiddles, and one LDS
 // transpose hands the line to the second small DFT (k_fftcol256/512,
 // k_fftcolq, k_fftx256, k_fftx512_*, k_fftxq).  Other sizes: each block
@@ -728,9 +728,9 @@ __global__ void __launch_bounds__(32 * B) k_fftcol512_t32(int N, int64_t I, cons
 }
 
 // ---------------------------------------------------------------------------
-// Column passes for G = 16 Q, Q in {8, 9, 10, 12, 15, 18, 20, 24, 25, 30, 36,
-// 40} (G = 128-640: the grids of 64-512 point images at oversampling 1.25
-// or 2), the same register four-step as the 256-point
+// Column passes for G = 16 Q, Q in {5, 6, 8, 9, 10, 12, 15, 18, 20, 24, 25,
+// 30, 36, 40} (G = 80-640: the grids of 48-512 point images at oversampling
+// 1.25 or 2), the same register four-step as the 256-point
 // pass: element n = t + 16k of a column is held by thread t (t = 0..15,
 // k = 0..Q-1), which takes the
 // Q-point DFT over k and multiplies by exp(-2 pi i t m1 / G); the block
@@ -867,6 +867,8 @@ __device__ __forceinline__ void dft32(float2* v) {
 template <int Q>
 __device__ __forceinline__ void dftQ(float2* v) {
     if constexpr (Q == 32) dft32(v);
+    else if constexpr (Q == 5) dft5(v);
+    else if constexpr (Q == 6) dft_pfa<2, 3>(v);
     else if constexpr (Q == 8) dft8(v);
     else if constexpr (Q == 10) dft_pfa<2, 5>(v);
     else if constexpr (Q == 12) dft_pfa<4, 3>(v);
@@ -885,8 +887,8 @@ constexpr int CQ_B = 32;  // columns per block (512 threads)
 
 // G with a register column pass (16 Q, Q as above and 30, 40; 256 and 512 have their own)
 static inline bool colq_supported(int G) {
-    return G == 128 || G == 144 || G == 160 || G == 192 || G == 240 || G == 288 || G == 320 || G == 384 ||
-           G == 400 || G == 480 || G == 576 || G == 640;
+    return G == 80 || G == 96 || G == 128 || G == 144 || G == 160 || G == 192 || G == 240 || G == 288 ||
+           G == 320 || G == 384 || G == 400 || G == 480 || G == 576 || G == 640;
 }
 
 template <bool FWD, int Q, int B>
@@ -1974,6 +1976,8 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
                                YG, nkc, factor);                                                                \
         break;
         switch (X.G / 16) {
+            PF_XQ_CASE(5)
+            PF_XQ_CASE(6)
             PF_XQ_CASE(8)
             PF_XQ_CASE(10)
             PF_XQ_CASE(12)
@@ -2099,6 +2103,8 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
             hipLaunchKernelGGL((k_fftcolq<false, QV, CQ_B>), grid, blk, 0, s, X.N, I, X.tw, src, dst, cm);  \
         break;
         switch (Q) {
+            PF_COLQ_CASE(5)
+            PF_COLQ_CASE(6)
             PF_COLQ_CASE(8)
             PF_COLQ_CASE(10)
             PF_COLQ_CASE(12)
@@ -2217,6 +2223,8 @@ static int cart_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, 
             hipLaunchKernelGGL((k_cartcolq<false, QV, B>), grid, blk, 0, s, I, X.tw, src, dst, e);           \
         break;
         switch (X.G / 16) {
+            PF_CCQ_CASE(5)
+            PF_CCQ_CASE(6)
             PF_CCQ_CASE(8)
             PF_CCQ_CASE(9)
             PF_CCQ_CASE(10)
@@ -2291,6 +2299,8 @@ static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src
             hipLaunchKernelGGL((k_cartrowq<false, QV>), grid, blk, 0, s, nrows, X.tw, src, dst, e);          \
         break;
         switch (X.G / 16) {
+            PF_CRQ_CASE(5)
+            PF_CRQ_CASE(6)
             PF_CRQ_CASE(8)
             PF_CRQ_CASE(9)
             PF_CRQ_CASE(10)

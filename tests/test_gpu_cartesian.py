@@ -59,6 +59,7 @@ CASES = [
     ("q_basis", (128, 160), 2, 6, 6, 2, None),
     ("q_640_400", (640, 400), 1, 1, 1, 0, "ortho"),
     ("q_288_576", (288, 576), 1, 1, 0, 0, None),
+    ("q_80_96", (80, 96), 3, 1, 1, 0, "ortho"),
 ]
 
 

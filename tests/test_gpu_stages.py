@@ -70,6 +70,10 @@ CASES = [
     ((8, 144), 2.0),       # 16 x 288
     ((6, 320), 1.25),      # 8 x 400
     ((6, 288), 2.0),       # 12 x 576
+    ((64, 20), 1.25),      # 80 x 25: 16 x 5
+    ((48, 10, 8), 2.0),    # 96 x 20 x 16: z is 16 x 6
+    ((10, 64), 1.25),      # 13 x 80: rows
+    ((12, 48), 2.0),       # 24 x 96
 ]
 # (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
 # values per grid point (second-stage lanes m1-fastest) and with 4
