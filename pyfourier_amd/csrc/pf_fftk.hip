@@ -15,7 +15,8 @@
 //   conjugate maps) in the user layout.
 //
 // Lines of G = 16 Q points (Q in {5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25,
-// 30, 32, 36, 40}: 80-640) run register four-step kernels: each thread holds Q (or 16)
+// 30, 32, 36, 40}: 80-640) and of 120, 200 and 360 points (8 threads x 15,
+// 25, 45) run register four-step kernels: each thread holds Q (or 16)
 // values of a line, takes a small DFT in registers, twiddles, and one LDS
 // transpose hands the line to the second small DFT (k_fftcol256/512,
 // k_fftcolq, k_fftx256, k_fftx512_*, k_fftxq).  Other sizes: each block
@@ -880,10 +881,21 @@ __device__ __forceinline__ void dftQ(float2* v) {
     else if constexpr (Q == 9) dft_ct<3, 3>(v);
     else if constexpr (Q == 18) dft_pfa<2, 9>(v);
     else if constexpr (Q == 25) dft_ct<5, 5>(v);
-    else dft_pfa<4, 9>(v);  // 36
+    else if constexpr (Q == 36) dft_pfa<4, 9>(v);
+    else dft_pfa<9, 5>(v);  // 45
+}
+
+// the second stage's DFT over the T threads of a line
+template <int T>
+__device__ __forceinline__ void dftT(float2* v) {
+    if constexpr (T == 16) dft16(v);
+    else dft8(v);
 }
 
 constexpr int CQ_B = 32;  // columns per block (512 threads)
+
+// G = 8 Q (Q = 15, 25, 45) with register passes of 8 threads per line
+static inline bool colq8_supported(int G) { return G == 120 || G == 200 || G == 360; }
 
 // G with a register column pass (16 Q, Q as above and 30, 40; 256 and 512 have their own)
 static inline bool colq_supported(int G) {
@@ -891,13 +903,13 @@ static inline bool colq_supported(int G) {
            G == 320 || G == 384 || G == 400 || G == 480 || G == 576 || G == 640;
 }
 
-template <bool FWD, int Q, int B>
-__global__ void __launch_bounds__(16 * B, Q <= 24 ? 4 : 2) k_fftcolq(int N, int64_t I, const float2* __restrict__ twg,
+template <bool FWD, int Q, int B, int T = 16>
+__global__ void __launch_bounds__(T * B, Q <= 24 ? 4 : 2) k_fftcolq(int N, int64_t I, const float2* __restrict__ twg,
                                                   const float2* __restrict__ in, float2* __restrict__ out,
                                                   ColMask cm) {
-    constexpr int G = 16 * Q;
-    constexpr int NP = Q < 16 ? Q : 16;  // m1 planes per transpose round
-    constexpr int PLANE = 16 * B;        // a 32-lane access covers 32 consecutive columns: no bank conflicts
+    constexpr int G = T * Q;
+    constexpr int NP = Q < T ? Q : T;  // m1 planes per transpose round
+    constexpr int PLANE = T * B;         // a 32-lane access covers 32 consecutive columns: no bank conflicts
     __shared__ float2 sh[NP * PLANE];
     __shared__ float2 tw[G];
     const int b = threadIdx.x % B, t = threadIdx.x / B;
@@ -907,14 +919,14 @@ __global__ void __launch_bounds__(16 * B, Q <= 24 ? 4 : 2) k_fftcolq(int N, int6
     const int64_t i0 = (blk % nchunk) * B;
     const bool bok = i0 + b < I;
     const bool zero = !bok || cm_dead_col(cm, i0 + b);  // no column, or one known to be zero
-    for (int k = threadIdx.x; k < G; k += 16 * B) tw[k] = twg[k];
+    for (int k = threadIdx.x; k < G; k += T * B) tw[k] = twg[k];
     const int lo = N - N / 2, hi = G - N / 2;  // stored rows [lo, hi) carry no image data
     float2 v[Q];
     if (FWD) {
         const float2* src = in + o * (int64_t)N * I + i0 + b;
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
-            const int n = t + 16 * k;  // stored row
+            const int n = t + T * k;  // stored row
             const int p = n < lo ? n + N / 2 : n - G + N / 2;
             v[k] = (!zero && (n < lo || n >= hi)) ? src[(int64_t)p * I] : make_float2(0.f, 0.f);
         }
@@ -922,7 +934,7 @@ __global__ void __launch_bounds__(16 * B, Q <= 24 ? 4 : 2) k_fftcolq(int N, int6
         const float2* src = in + o * (int64_t)G * I + i0 + b;
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
-            const float2 x = !zero ? src[(int64_t)(t + 16 * k) * I] : make_float2(0.f, 0.f);
+            const float2 x = !zero ? src[(int64_t)(t + T * k) * I] : make_float2(0.f, 0.f);
             v[k] = make_float2(x.x, -x.y);
         }
     }
@@ -931,26 +943,26 @@ __global__ void __launch_bounds__(16 * B, Q <= 24 ? 4 : 2) k_fftcolq(int N, int6
 #pragma unroll
     for (int m = 1; m < Q; ++m) v[m] = c_mul(v[m], tw[t * m]);
 #pragma unroll
-    for (int h = 0; h < (Q + 15) / 16; ++h) {
+    for (int h = 0; h < (Q + T - 1) / T; ++h) {
         if (h) __syncthreads();  // the previous round's reads are done
 #pragma unroll
-        for (int m = 16 * h; m < Q && m < 16 * h + 16; ++m) sh[(m - 16 * h) * PLANE + t * B + b] = v[m];
+        for (int m = T * h; m < Q && m < T * h + T; ++m) sh[(m - T * h) * PLANE + t * B + b] = v[m];
         __syncthreads();
-        const int m1 = 16 * h + t;
+        const int m1 = T * h + t;
         if (m1 < Q) {
-            float2 u[16];  // u[t'] = twiddled Q-point output m1 of thread t'
+            float2 u[T];  // u[t'] = twiddled Q-point output m1 of thread t'
 #pragma unroll
-            for (int tp = 0; tp < 16; ++tp) u[tp] = sh[t * PLANE + tp * B + b];
-            dft16(u);  // u[m2] = X[m1 + Q m2]
+            for (int tp = 0; tp < T; ++tp) u[tp] = sh[t * PLANE + tp * B + b];
+            dftT<T>(u);  // u[m2] = X[m1 + Q m2]
             if (bok) {
                 if (FWD) {
                     float2* dst = out + o * (int64_t)G * I + i0 + b;
 #pragma unroll
-                    for (int m2 = 0; m2 < 16; ++m2) dst[(int64_t)(m1 + Q * m2) * I] = u[m2];
+                    for (int m2 = 0; m2 < T; ++m2) dst[(int64_t)(m1 + Q * m2) * I] = u[m2];
                 } else {
                     float2* dst = out + o * (int64_t)N * I + i0 + b;
 #pragma unroll
-                    for (int m2 = 0; m2 < 16; ++m2) {
+                    for (int m2 = 0; m2 < T; ++m2) {
                         const int n = m1 + Q * m2;  // stored output index -> cropped row p
                         if (n < lo || n >= hi) {
                             const int p = n < lo ? n + N / 2 : n - G + N / 2;
@@ -1094,13 +1106,15 @@ __global__ void __launch_bounds__(16 * BX, PF_FFT_WPE * 16 / BX) k_fftx256(Geo g
 // inverse (SEG) keeps all L segments of KC (k, cv) pairs of YG rows in a
 // block, writes each segment's cropped row to LDS and sums the segments
 // (times the conjugate maps) before one x-contiguous image write.
-template <bool FWD, int Q, int BX, bool M1F = false, bool SEG = false>
-__global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2* __restrict__ twg,
+template <bool FWD, int Q, int BX, bool M1F = false, bool SEG = false, int T = 16>
+__global__ void __launch_bounds__(T * BX) k_fftxq(Geo g, Batch bt, const float2* __restrict__ twg,
                                                  const float* __restrict__ apod, const float2* __restrict__ in,
                                                  float2* __restrict__ out, int KC, int YG, int nkc, float factor) {
-    constexpr int G = 16 * Q;
-    constexpr int NP = Q < 16 ? Q : 16;
-    constexpr int ROW = BX + 1, PLANE = 16 * ROW + 4;  // transpose strides (float2), as k_fftx256
+    constexpr int G = T * Q;
+    constexpr int NP = Q < T ? Q : T;
+    // transpose strides (float2): as k_fftx256 at T = 16; at T = 8 rows of BX + 4 so the 8 x 4
+    // (t, column) lanes of a half-wave hit distinct banks
+    constexpr int ROW = T == 16 ? BX + 1 : BX + 4, PLANE = T * ROW + 4;
     __shared__ float2 sh[NP * PLANE];
     __shared__ float2 tw[G];
     __shared__ float2 S[SEG ? BX * G : 1];  // SEG: cropped segment rows, S[(yy, kcl)][l][p]
@@ -1117,7 +1131,7 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
     const float* apz = apod;
     const float* apy = apod + g.N[0];
     const float* apx = apod + g.N[0] + g.N[1];
-    for (int k = threadIdx.x; k < G; k += 16 * BX) tw[k] = twg[k];
+    for (int k = threadIdx.x; k < G; k += T * BX) tw[k] = twg[k];
     const int lo = N - N / 2, hi = G - N / 2;
     // column b -> row and v (forward: v = l*KCv + kc; inverse: the pair kc,
     // SEG: segment l of pair kc)
@@ -1140,9 +1154,9 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
         return ((int64_t)c * bt.nimg + ii) * g.ntot + zy * N;
     };
     float2 v[Q];
-    const int t = FWD ? (threadIdx.x & 15) : (threadIdx.x / BX);
+    const int t = FWD ? (threadIdx.x % T) : (threadIdx.x / BX);
     {
-        const int b = FWD ? (threadIdx.x >> 4) : (threadIdx.x % BX);
+        const int b = FWD ? (threadIdx.x / T) : (threadIdx.x % BX);
         int64_t row;
         int vc;
         const bool ok = column(b, row, vc);
@@ -1154,7 +1168,7 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
             const float2* b0 = (ok && bt.b0s) ? bt.b0s + (int64_t)l * g.ntot + zy * N : nullptr;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
-                const int n = t + 16 * k;
+                const int n = t + T * k;
                 const int p = n < lo ? n + N / 2 : n - G + N / 2;
                 float2 x = make_float2(0.f, 0.f);
                 if (ok && (n < lo || n >= hi)) {
@@ -1167,7 +1181,7 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
             const float2* src = in + (ok ? row : 0) * G * RC + vc;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
-                const float2 x = ok ? src[(int64_t)(t + 16 * k) * RC] : make_float2(0.f, 0.f);
+                const float2 x = ok ? src[(int64_t)(t + T * k) * RC] : make_float2(0.f, 0.f);
                 v[k] = make_float2(x.x, -x.y);
             }
         }
@@ -1176,33 +1190,33 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
 #pragma unroll
         for (int m = 1; m < Q; ++m) v[m] = c_mul(v[m], tw[t * m]);
     }
-    const int sb = FWD ? (threadIdx.x >> 4) : (threadIdx.x % BX);  // stage-1 column of this thread
-    const int j = (FWD && !M1F) ? (threadIdx.x / BX) : (threadIdx.x & 15);
-    const int b = (FWD && !M1F) ? (threadIdx.x % BX) : (threadIdx.x >> 4);
+    const int sb = FWD ? (threadIdx.x / T) : (threadIdx.x % BX);  // stage-1 column of this thread
+    const int j = (FWD && !M1F) ? (threadIdx.x / BX) : (threadIdx.x % T);
+    const int b = (FWD && !M1F) ? (threadIdx.x % BX) : (threadIdx.x / T);
     int64_t row;
     int vc;
     const bool ok = column(b, row, vc);
 #pragma unroll
-    for (int h = 0; h < (Q + 15) / 16; ++h) {
+    for (int h = 0; h < (Q + T - 1) / T; ++h) {
         if (h) __syncthreads();  // the previous round's reads are done
 #pragma unroll
-        for (int m = 16 * h; m < Q && m < 16 * h + 16; ++m) sh[(m - 16 * h) * PLANE + t * ROW + sb] = v[m];
+        for (int m = T * h; m < Q && m < T * h + T; ++m) sh[(m - T * h) * PLANE + t * ROW + sb] = v[m];
         __syncthreads();
-        const int m1 = 16 * h + j;
+        const int m1 = T * h + j;
         if (m1 < Q && ok) {
-            float2 u[16];
+            float2 u[T];
 #pragma unroll
-            for (int tp = 0; tp < 16; ++tp) u[tp] = sh[j * PLANE + tp * ROW + b];
-            dft16(u);  // u[m2] = X[m1 + Q m2] of column b
+            for (int tp = 0; tp < T; ++tp) u[tp] = sh[j * PLANE + tp * ROW + b];
+            dftT<T>(u);  // u[m2] = X[m1 + Q m2] of column b
             if (FWD) {
                 float2* dst = out + row * G * RC + vc;
 #pragma unroll
-                for (int m2 = 0; m2 < 16; ++m2) dst[(int64_t)(m1 + Q * m2) * RC] = u[m2];
+                for (int m2 = 0; m2 < T; ++m2) dst[(int64_t)(m1 + Q * m2) * RC] = u[m2];
             } else if (SEG) {
                 const int lkc = L * KC, r = b % lkc;  // b = (yy, l, kcl) -> segment row (yy*KC + kcl)*L + l
                 float2* Sr = S + ((int64_t)((b / lkc) * KC + r % KC) * L + r / KC) * N;
 #pragma unroll
-                for (int m2 = 0; m2 < 16; ++m2) {
+                for (int m2 = 0; m2 < T; ++m2) {
                     const int n = m1 + Q * m2;
                     if (n < lo || n >= hi) Sr[n < lo ? n + N / 2 : n - G + N / 2] = make_float2(u[m2].x, -u[m2].y);
                 }
@@ -1212,7 +1226,7 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
                 float2* dst = out + image_row(row, vc, ayz, zy);
                 const float sc = factor * ayz;
 #pragma unroll
-                for (int m2 = 0; m2 < 16; ++m2) {
+                for (int m2 = 0; m2 < T; ++m2) {
                     const int n = m1 + Q * m2;
                     if (n < lo || n >= hi) {
                         const int p = n < lo ? n + N / 2 : n - G + N / 2;
@@ -1225,7 +1239,7 @@ __global__ void __launch_bounds__(16 * BX) k_fftxq(Geo g, Batch bt, const float2
     if constexpr (SEG) {
         __syncthreads();
         const int NC = YG * KC;  // (row, kc) image rows of this block
-        for (int it = threadIdx.x; it < NC * N; it += 16 * BX) {
+        for (int it = threadIdx.x; it < NC * N; it += T * BX) {
             const int p = it % N, cc = it / N;
             const int64_t rw = row0 + cc / KC;
             const int kc = kc0 + cc % KC;
@@ -1544,12 +1558,12 @@ __global__ void __launch_bounds__(16 * B, PF_FFT_WPE * 16 / B) k_cartcol256(int6
 
 // column pass, G = 16 Q (128-640 points): the four-step of k_fftcolq with
 // the centred reads and writes and the mask / factor epilogue of k_cartcol256
-template <bool FWD, int Q, int B>
-__global__ void __launch_bounds__(16 * B) k_cartcolq(int64_t I, const float2* __restrict__ twg, const float2* in,
+template <bool FWD, int Q, int B, int T = 16>
+__global__ void __launch_bounds__(T * B) k_cartcolq(int64_t I, const float2* __restrict__ twg, const float2* in,
                                                    float2* out, CartEpi e) {
-    constexpr int G = 16 * Q;
-    constexpr int NP = Q < 16 ? Q : 16;
-    constexpr int PLANE = 16 * B;  // a 32-lane access covers 32 consecutive slots: no bank conflicts
+    constexpr int G = T * Q;
+    constexpr int NP = Q < T ? Q : T;
+    constexpr int PLANE = T * B;   // a 32-lane access covers 32 consecutive slots: no bank conflicts
     __shared__ float2 sh[NP * PLANE];
     __shared__ float2 tw[G];
     const int b = threadIdx.x % B, t = threadIdx.x / B;
@@ -1558,12 +1572,12 @@ __global__ void __launch_bounds__(16 * B) k_cartcolq(int64_t I, const float2* __
     const int64_t o = blk / nchunk;
     const int64_t i = (blk % nchunk) * B + b;
     const bool bok = i < I;
-    for (int k = threadIdx.x; k < G; k += 16 * B) tw[k] = twg[k];
+    for (int k = threadIdx.x; k < G; k += T * B) tw[k] = twg[k];
     const float2* src = in + o * G * I + i;
     float2 v[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const int j = crot(t + 16 * k, G);
+        const int j = crot(t + T * k, G);
         float2 x = bok ? src[(int64_t)j * I] : make_float2(0.f, 0.f);
         if (!FWD) {
             if (bok) x = c_scale(x, cmask(e, o, G, j, I, i));
@@ -1577,19 +1591,19 @@ __global__ void __launch_bounds__(16 * B) k_cartcolq(int64_t I, const float2* __
     for (int m = 1; m < Q; ++m) v[m] = c_mul(v[m], tw[t * m]);
     float2* dst = out + o * G * I + i;
 #pragma unroll
-    for (int h = 0; h < (Q + 15) / 16; ++h) {
+    for (int h = 0; h < (Q + T - 1) / T; ++h) {
         if (h) __syncthreads();  // the previous round's reads are done
 #pragma unroll
-        for (int m = 16 * h; m < Q && m < 16 * h + 16; ++m) sh[(m - 16 * h) * PLANE + t * B + b] = v[m];
+        for (int m = T * h; m < Q && m < T * h + T; ++m) sh[(m - T * h) * PLANE + t * B + b] = v[m];
         __syncthreads();
-        const int m1 = 16 * h + t;
+        const int m1 = T * h + t;
         if (m1 < Q && bok) {
-            float2 u[16];
+            float2 u[T];
 #pragma unroll
-            for (int tp = 0; tp < 16; ++tp) u[tp] = sh[t * PLANE + tp * B + b];
-            dft16(u);  // u[m2] = X[m1 + Q m2]
+            for (int tp = 0; tp < T; ++tp) u[tp] = sh[t * PLANE + tp * B + b];
+            dftT<T>(u);  // u[m2] = X[m1 + Q m2]
 #pragma unroll
-            for (int m2 = 0; m2 < 16; ++m2) {
+            for (int m2 = 0; m2 < T; ++m2) {
                 const int j = crot(m1 + Q * m2, G);
                 const float2 y = FWD ? u[m2] : make_float2(u[m2].x, -u[m2].y);
                 const float sc = FWD ? e.factor * cmask(e, o, G, j, I, i) : e.factor;
@@ -1688,24 +1702,27 @@ __global__ void __launch_bounds__(256, PF_FFT_WPE) k_cartrow256(int64_t nrows, c
     }
 }
 
-// row pass, G = 16 Q: 16 rows per block, k_cartrow256's lane maps with the
-// 16 x Q four-step (transpose rounds of 16 m1 planes)
-template <bool FWD, int Q>
+// row pass, G = T Q (T = 16 or 8 threads per row): 256 / T rows per block,
+// k_cartrow256's lane maps with the T x Q four-step (transpose rounds of T m1
+// planes)
+template <bool FWD, int Q, int T = 16>
 __global__ void __launch_bounds__(256) k_cartrowq(int64_t nrows, const float2* __restrict__ twg, const float2* in,
                                                 float2* out, CartEpi e) {
-    constexpr int G = 16 * Q;
-    constexpr int NP = Q < 16 ? Q : 16;
-    __shared__ float2 sh[NP * X256_PLANE];
+    constexpr int G = T * Q;
+    constexpr int NP = Q < T ? Q : T;
+    constexpr int RB = 256 / T;                                 // rows per block
+    constexpr int ROW = T == 16 ? RB + 1 : RB + 4, PLANE = T * ROW + 4;  // as k_fftxq
+    __shared__ float2 sh[NP * PLANE];
     __shared__ float2 tw[G];
-    const int64_t r0 = (int64_t)pf_xcd_remap(blockIdx.x, gridDim.x) * 16;
+    const int64_t r0 = (int64_t)pf_xcd_remap(blockIdx.x, gridDim.x) * RB;
     for (int k = threadIdx.x; k < G; k += 256) tw[k] = twg[k];
-    const int t = threadIdx.x & 15, b = threadIdx.x >> 4;
+    const int t = threadIdx.x % T, b = threadIdx.x / T;
     const int64_t row = r0 + b;
     const bool ok = row < nrows;
     float2 v[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const int j = crot(t + 16 * k, G);
+        const int j = crot(t + T * k, G);
         float2 x = ok ? in[row * G + j] : make_float2(0.f, 0.f);
         if (!FWD) {
             if (ok) x = c_scale(x, cmask(e, row, 1, 0, G, j));
@@ -1718,19 +1735,19 @@ __global__ void __launch_bounds__(256) k_cartrowq(int64_t nrows, const float2* _
 #pragma unroll
     for (int m = 1; m < Q; ++m) v[m] = c_mul(v[m], tw[t * m]);
 #pragma unroll
-    for (int h = 0; h < (Q + 15) / 16; ++h) {
+    for (int h = 0; h < (Q + T - 1) / T; ++h) {
         if (h) __syncthreads();  // the previous round's reads are done
 #pragma unroll
-        for (int m = 16 * h; m < Q && m < 16 * h + 16; ++m) sh[(m - 16 * h) * X256_PLANE + t * X256_ROW + b] = v[m];
+        for (int m = T * h; m < Q && m < T * h + T; ++m) sh[(m - T * h) * PLANE + t * ROW + b] = v[m];
         __syncthreads();
-        const int m1 = 16 * h + t;  // second stage: thread t takes m1 = 16 h + t of row b
+        const int m1 = T * h + t;  // second stage: thread t takes m1 = 16 h + t of row b
         if (m1 < Q && ok) {
-            float2 u[16];
+            float2 u[T];
 #pragma unroll
-            for (int tp = 0; tp < 16; ++tp) u[tp] = sh[t * X256_PLANE + tp * X256_ROW + b];
-            dft16(u);  // u[m2] = X[m1 + Q m2] of row b
+            for (int tp = 0; tp < T; ++tp) u[tp] = sh[t * PLANE + tp * ROW + b];
+            dftT<T>(u);  // u[m2] = X[m1 + Q m2] of row b
 #pragma unroll
-            for (int m2 = 0; m2 < 16; ++m2) {
+            for (int m2 = 0; m2 < T; ++m2) {
                 const int j = crot(m1 + Q * m2, G);
                 const float2 y = FWD ? u[m2] : make_float2(u[m2].x, -u[m2].y);
                 const float sc = FWD ? e.factor * cmask(e, row, 1, 0, G, j) : e.factor;
@@ -1933,8 +1950,9 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         return PF_OK;
     }
     const bool x512inv = X.G == 512 && PF_FFT512X32 && !fwd && b.L == 1 && !b.b0s;
-    if (PF_FFTQ && (colq_supported(X.G) || x512inv) && b.L <= X256_BF) {
-        const int BX = X256_BF;  // 16 columns per block, both directions
+    const bool t8 = PF_FFTQ && colq8_supported(X.G);  // 8 threads per line: 32 columns per block
+    if (PF_FFTQ && (colq_supported(X.G) || t8 || x512inv) && b.L <= X256_BF) {
+        const int BX = t8 ? 2 * X256_BF : X256_BF;  // columns per block, both directions
         const bool seg = !fwd && (b.L > 1 || b.b0s);  // inverse summing field-map segments
         int KC, YG, nkc;
         if (seg) {  // all L segments of KC pairs (of YG rows when they fit) per block
@@ -1954,6 +1972,33 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         Geo g = geo_of(p);
         Batch t = batch_of(b);
         const dim3 grid((unsigned)nblk), blk(16 * X256_BF);
+        if (t8) {
+#define PF_XQ8_CASE(QV)                                                                                         \
+    case QV:                                                                                                    \
+        if (fwd && KC < 8)                                                                                      \
+            hipLaunchKernelGGL((k_fftxq<true, QV, 2 * X256_BF, true, false, 8>), grid, blk, 0, s, g, t, X.tw,    \
+                               p->apod, src, dst, KC, YG, nkc, 1.0f);                                           \
+        else if (fwd)                                                                                           \
+            hipLaunchKernelGGL((k_fftxq<true, QV, 2 * X256_BF, false, false, 8>), grid, blk, 0, s, g, t, X.tw,   \
+                               p->apod, src, dst, KC, YG, nkc, 1.0f);                                           \
+        else if (seg)                                                                                           \
+            hipLaunchKernelGGL((k_fftxq<false, QV, 2 * X256_BF, false, true, 8>), grid, blk, 0, s, g, t, X.tw,   \
+                               p->apod, src, dst, KC, YG, nkc, factor);                                         \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_fftxq<false, QV, 2 * X256_BF, false, false, 8>), grid, blk, 0, s, g, t, X.tw,  \
+                               p->apod, src, dst, KC, YG, nkc, factor);                                         \
+        break;
+            switch (X.G / 8) {
+                PF_XQ8_CASE(15)
+                PF_XQ8_CASE(25)
+                PF_XQ8_CASE(45)
+                default:
+                    return pf_fail(PF_ERR_FFT, "no register row pass for this size");
+            }
+#undef PF_XQ8_CASE
+            PF_HIP(hipGetLastError());
+            return PF_OK;
+        }
         if (x512inv) {  // inverse only: the forward's 32-value form measured slower than k_fftx512_fwd
             hipLaunchKernelGGL((k_fftxq<false, 32, X256_BF>), grid, blk, 0, s, g, t, X.tw, p->apod, src, dst, KC, YG,
                                nkc, factor);
@@ -2089,6 +2134,29 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
+    if (PF_FFTQ && colq8_supported(X.G)) {  // 8 threads x Q values per column, 256-thread blocks
+        const int64_t nblk = O * ((I + CQ_B - 1) / CQ_B);
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+        if (nblk < 1) return PF_OK;
+        const dim3 grid((unsigned)nblk), blk(8 * CQ_B);
+#define PF_COLQ8_CASE(QV)                                                                                     \
+    case QV:                                                                                                 \
+        if (fwd)                                                                                             \
+            hipLaunchKernelGGL((k_fftcolq<true, QV, CQ_B, 8>), grid, blk, 0, s, X.N, I, X.tw, src, dst, cm); \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_fftcolq<false, QV, CQ_B, 8>), grid, blk, 0, s, X.N, I, X.tw, src, dst, cm); \
+        break;
+        switch (X.G / 8) {
+            PF_COLQ8_CASE(15)
+            PF_COLQ8_CASE(25)
+            PF_COLQ8_CASE(45)
+            default:
+                return pf_fail(PF_ERR_FFT, "no register column pass for this size");
+        }
+#undef PF_COLQ8_CASE
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
     if (PF_FFTQ && colq_supported(X.G)) {
         const int64_t nblk = O * ((I + CQ_B - 1) / CQ_B);
         if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
@@ -2146,7 +2214,7 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
 bool pf_fftk_masked(const pf_plan* p) {
     if (!PF_COLMASK || p->ndim != 3 || !p->colmask || !p->fftk) return false;
     const int G = p->fftk->ax[0].G;
-    return G == 256 || (G == 512 && PF_FFT512) || (PF_FFTQ && colq_supported(G));
+    return G == 256 || (G == 512 && PF_FFT512) || (PF_FFTQ && (colq_supported(G) || colq8_supported(G)));
 }
 
 // image -> grid (deapodise, pad, FFT); `extra` holds pf_fftk_extra_bytes
@@ -2206,6 +2274,30 @@ static int cart_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, 
         else
             hipLaunchKernelGGL((k_cartcol256<false, CC256_BA>), dim3((unsigned)nblk), dim3(16 * CC256_BA), 0, s, I,
                                X.tw, src, dst, e);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    if (PF_FFTQ && colq8_supported(X.G)) {  // 8 threads per line, 32 columns per block
+        constexpr int B = 2 * CC256_BF;
+        const int64_t nblk = O * ((I + B - 1) / B);
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT columns");
+        if (nblk < 1) return PF_OK;
+        const dim3 grid((unsigned)nblk), blk(8 * B);
+#define PF_CCQ8_CASE(QV)                                                                                      \
+    case QV:                                                                                                  \
+        if (fwd)                                                                                              \
+            hipLaunchKernelGGL((k_cartcolq<true, QV, B, 8>), grid, blk, 0, s, I, X.tw, src, dst, e);         \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_cartcolq<false, QV, B, 8>), grid, blk, 0, s, I, X.tw, src, dst, e);        \
+        break;
+        switch (X.G / 8) {
+            PF_CCQ8_CASE(15)
+            PF_CCQ8_CASE(25)
+            PF_CCQ8_CASE(45)
+            default:
+                return pf_fail(PF_ERR_FFT, "no register Cartesian column pass for this size");
+        }
+#undef PF_CCQ8_CASE
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
@@ -2283,6 +2375,29 @@ static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src
             hipLaunchKernelGGL(k_cartrow256<true>, dim3((unsigned)nblk), dim3(256), 0, s, nrows, X.tw, src, dst, e);
         else
             hipLaunchKernelGGL(k_cartrow256<false>, dim3((unsigned)nblk), dim3(256), 0, s, nrows, X.tw, src, dst, e);
+        PF_HIP(hipGetLastError());
+        return PF_OK;
+    }
+    if (PF_FFTQ && colq8_supported(X.G)) {  // 8 threads per line, 32 rows per block
+        const int64_t nblk = (nrows + 31) / 32;
+        if (nblk > 0x7fffffff) return pf_fail(PF_ERR_UNSUPPORTED, "too many FFT rows");
+        if (nblk < 1) return PF_OK;
+        const dim3 grid((unsigned)nblk), blk(256);
+#define PF_CRQ8_CASE(QV)                                                                                      \
+    case QV:                                                                                                  \
+        if (fwd)                                                                                              \
+            hipLaunchKernelGGL((k_cartrowq<true, QV, 8>), grid, blk, 0, s, nrows, X.tw, src, dst, e);        \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_cartrowq<false, QV, 8>), grid, blk, 0, s, nrows, X.tw, src, dst, e);       \
+        break;
+        switch (X.G / 8) {
+            PF_CRQ8_CASE(15)
+            PF_CRQ8_CASE(25)
+            PF_CRQ8_CASE(45)
+            default:
+                return pf_fail(PF_ERR_FFT, "no register Cartesian row pass for this size");
+        }
+#undef PF_CRQ8_CASE
         PF_HIP(hipGetLastError());
         return PF_OK;
     }

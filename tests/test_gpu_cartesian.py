@@ -60,6 +60,8 @@ CASES = [
     ("q_640_400", (640, 400), 1, 1, 1, 0, "ortho"),
     ("q_288_576", (288, 576), 1, 1, 0, 0, None),
     ("q_80_96", (80, 96), 3, 1, 1, 0, "ortho"),
+    ("q8_120_200", (120, 200), 2, 1, 1, 0, "ortho"),
+    ("q8_360_3d", (360, 12, 120), 2, 2, 2, 0, None),
 ]
 
 

@@ -190,7 +190,8 @@ def test_small_grids_periodic_wrap(cuda_device, shape, oversamp):
     _check_pair(cuda_device, coord, shape, x, oversamp=oversamp)
 
 
-@pytest.mark.parametrize("shape,oversamp", [((12, 128), 2.0), ((12, 128), 1.25), ((12, 192), 2.0), ((12, 384), 1.25)])
+@pytest.mark.parametrize("shape,oversamp", [((12, 128), 2.0), ((12, 128), 1.25), ((12, 192), 2.0), ((12, 384), 1.25),
+                                           ((12, 160), 1.25)])
 def test_subspace_and_frames_register_rows(cuda_device, shape, oversamp):
     """x axes of 256, 160, 384 and 480 points run the register row FFTs
     (16 x 16, 16 x 10, 16 x 24, 16 x 30), with a subspace basis (image planes
@@ -217,7 +218,8 @@ def test_subspace_and_frames_register_rows(cuda_device, shape, oversamp):
     assert rel(xfa.cpu().numpy(), R.nufft_adj(ya.reshape(C, F, -1), P)) < PARITY
 
 
-@pytest.mark.parametrize("shape,oversamp", [((10, 256), 2.0), ((10, 128), 1.25), ((10, 256), 1.25), ((10, 192), 2.0)])
+@pytest.mark.parametrize("shape,oversamp", [((10, 256), 2.0), ((10, 128), 1.25), ((10, 256), 1.25), ((10, 192), 2.0),
+                                           ((10, 160), 1.25)])
 def test_b0_with_frames_and_subspace_register_rows(cuda_device, shape, oversamp):
     """x axes of 512, 160, 320 and 384 points run the register row FFTs with
     B0 segments, per frame and with a subspace basis.  3 coils: the forward's

@@ -74,6 +74,13 @@ CASES = [
     ((48, 10, 8), 2.0),    # 96 x 20 x 16: z is 16 x 6
     ((10, 64), 1.25),      # 13 x 80: rows
     ((12, 48), 2.0),       # 24 x 96
+    ((96, 20), 1.25),      # 120 x 25: 8 threads x 15 values per column
+    ((160, 12), 1.25),     # 200 x 15: 8 x 25
+    ((180, 10), 2.0),      # 360 x 20: 8 x 45
+    ((96, 10, 8), 1.25),   # 120 x 13 x 10: z
+    ((10, 96), 1.25),      # 13 x 120: rows
+    ((8, 160), 1.25),      # 10 x 200
+    ((6, 180), 2.0),       # 12 x 360
 ]
 # (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
 # values per grid point (second-stage lanes m1-fastest) and with 4
@@ -85,6 +92,8 @@ PARAMS = [(s, o, 3) for s, o in CASES] + [
     ((6, 8, 256), 2.0, 4),
     ((6, 8, 128), 1.25, 2),  # 160-point row pass storing m1-fastest (coil pairs)
     ((6, 8, 192), 2.0, 16),
+    ((6, 8, 96), 1.25, 2),   # 120-point row pass (8 threads per row) storing m1-fastest
+    ((6, 8, 160), 1.25, 12),
 ]
 
 
