@@ -234,8 +234,10 @@ int pf_workspace_offsets(const pf_plan* plan, const pf_batch_desc* desc, size_t*
                          size_t* samples_off, size_t* fft_off, size_t* spread_off);
 
 /* Fused deapodise + zero-pad + centred FFT: image (C, nimg, N...) -> grid.
- * For grid sizes 2^a 3^b 5^c (<= 4096 per axis) a pruned FFT transforms only
- * the lines that hold image data; otherwise pf_deapod_pad + pf_fft.
+ * For grid sizes 2^a 3^b 5^c, and the 7-smooth sizes 112, 224, 280, 336 and
+ * 448 (<= 4096 per axis), a pruned FFT transforms only the lines that hold
+ * image data; otherwise pf_deapod_pad + pf_fft.  On the 7-smooth sizes a
+ * batch takes at most 16 field-map segments per call.
  * Replaces src/pyfourier/_nufft/_nufft.py:234-241.  fft_workspace: the FFT
  * scratch part of the pf_nufft workspace (pf_workspace_offsets).  Grid
  * layout (F, C/Cv, G..., nseg*ncoeff, Cv), F = 1 with a basis: Cv is the

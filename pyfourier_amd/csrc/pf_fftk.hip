@@ -14,9 +14,9 @@
 //   the X pass writing the deapodised image (sum over B0 segments with the
 //   conjugate maps) in the user layout.
 //
-// Lines of G = 16 Q points (Q in {5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25,
-// 30, 32, 36, 40}: 80-640) and of 120, 200 and 360 points (8 threads x 15,
-// 25, 45) run register four-step kernels: each thread holds Q (or 16)
+// Lines of G = 16 Q points (Q in {5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 18, 20,
+// 21, 24, 25, 28, 30, 32, 36, 40}: 80-640) and of 120, 200, 280 and 360
+// points (8 threads x 15, 25, 35, 45) run register four-step kernels: each thread holds Q (or 16)
 // values of a line, takes a small DFT in registers, twiddles, and one LDS
 // transpose hands the line to the second small DFT (k_fftcol256/512,
 // k_fftcolq, k_fftx256, k_fftx512_*, k_fftxq).  Other sizes: each block
@@ -128,12 +128,39 @@ __device__ __forceinline__ void dft8(float2* v) {
     v[7] = c_sub(e[3], o3);
 }
 
+// forward 7-point DFT: with a_j = x_j + x_(7-j), b_j = x_j - x_(7-j) (j = 1..3),
+// X_k = x_0 + sum_j a_j cos(2 pi j k / 7) - i sum_j b_j sin(2 pi j k / 7)
+// and X_(7-k) the same with the sine terms negated
+__device__ __forceinline__ void dft7(float2* v) {
+    const float c1 = 0.62348980185873359f, c2 = -0.22252093395631434f, c3 = -0.90096886790241903f;
+    const float s1 = 0.7818314824680298f, s2 = 0.97492791218182362f, s3 = 0.43388373911755823f;
+    const float2 x0 = v[0];
+    const float2 a1 = c_add(v[1], v[6]), b1 = c_sub(v[1], v[6]);
+    const float2 a2 = c_add(v[2], v[5]), b2 = c_sub(v[2], v[5]);
+    const float2 a3 = c_add(v[3], v[4]), b3 = c_sub(v[3], v[4]);
+    // k = 1, 2, 3: cosine parts r_k and sine parts q_k (x the pair differences)
+    const float2 r1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x + c3 * a3.x, x0.y + c1 * a1.y + c2 * a2.y + c3 * a3.y);
+    const float2 r2 = make_float2(x0.x + c2 * a1.x + c3 * a2.x + c1 * a3.x, x0.y + c2 * a1.y + c3 * a2.y + c1 * a3.y);
+    const float2 r3 = make_float2(x0.x + c3 * a1.x + c1 * a2.x + c2 * a3.x, x0.y + c3 * a1.y + c1 * a2.y + c2 * a3.y);
+    const float2 q1 = make_float2(s1 * b1.x + s2 * b2.x + s3 * b3.x, s1 * b1.y + s2 * b2.y + s3 * b3.y);
+    const float2 q2 = make_float2(s2 * b1.x - s3 * b2.x - s1 * b3.x, s2 * b1.y - s3 * b2.y - s1 * b3.y);
+    const float2 q3 = make_float2(s3 * b1.x - s1 * b2.x + s2 * b3.x, s3 * b1.y - s1 * b2.y + s2 * b3.y);
+    v[0] = make_float2(x0.x + a1.x + a2.x + a3.x, x0.y + a1.y + a2.y + a3.y);
+    v[1] = c_add(r1, c_negi(q1));
+    v[6] = c_add(r1, c_posi(q1));
+    v[2] = c_add(r2, c_negi(q2));
+    v[5] = c_add(r2, c_posi(q2));
+    v[3] = c_add(r3, c_negi(q3));
+    v[4] = c_add(r3, c_posi(q3));
+}
+
 template <int R>
 __device__ __forceinline__ void dftR(float2* v) {
     if constexpr (R == 2) dft2(v);
     else if constexpr (R == 3) dft3(v);
     else if constexpr (R == 4) dft4(v);
     else if constexpr (R == 5) dft5(v);
+    else if constexpr (R == 7) dft7(v);
     else dft8(v);
 }
 
@@ -819,7 +846,7 @@ __device__ __forceinline__ void dft_ct(float2* v) {
     }
 }
 
-// the factors of the Good-Thomas products: 2, 3, 4, 5, 8, 6 = 2 x 3 and 9 = 3 x 3
+// the factors of the Good-Thomas products: 2, 3, 4, 5, 7, 8, 6 = 2 x 3 and 9 = 3 x 3
 template <int R>
 __device__ __forceinline__ void dft_small(float2* v) {
     if constexpr (R == 6) dft_pfa<2, 3>(v);
@@ -882,7 +909,12 @@ __device__ __forceinline__ void dftQ(float2* v) {
     else if constexpr (Q == 18) dft_pfa<2, 9>(v);
     else if constexpr (Q == 25) dft_ct<5, 5>(v);
     else if constexpr (Q == 36) dft_pfa<4, 9>(v);
-    else dft_pfa<9, 5>(v);  // 45
+    else if constexpr (Q == 45) dft_pfa<9, 5>(v);
+    else if constexpr (Q == 7) dft7(v);
+    else if constexpr (Q == 14) dft_pfa<2, 7>(v);
+    else if constexpr (Q == 21) dft_pfa<3, 7>(v);
+    else if constexpr (Q == 28) dft_pfa<4, 7>(v);
+    else dft_pfa<5, 7>(v);  // 35
 }
 
 // the second stage's DFT over the T threads of a line
@@ -894,13 +926,14 @@ __device__ __forceinline__ void dftT(float2* v) {
 
 constexpr int CQ_B = 32;  // columns per block (512 threads)
 
-// G = 8 Q (Q = 15, 25, 45) with register passes of 8 threads per line
-static inline bool colq8_supported(int G) { return G == 120 || G == 200 || G == 360; }
+// G = 8 Q (Q = 15, 25, 35, 45) with register passes of 8 threads per line
+static inline bool colq8_supported(int G) { return G == 120 || G == 200 || G == 280 || G == 360; }
 
 // G with a register column pass (16 Q, Q as above and 30, 40; 256 and 512 have their own)
 static inline bool colq_supported(int G) {
-    return G == 80 || G == 96 || G == 128 || G == 144 || G == 160 || G == 192 || G == 240 || G == 288 ||
-           G == 320 || G == 384 || G == 400 || G == 480 || G == 576 || G == 640;
+    return G == 80 || G == 96 || G == 112 || G == 128 || G == 144 || G == 160 || G == 192 || G == 224 ||
+           G == 240 || G == 288 || G == 320 || G == 336 || G == 384 || G == 400 || G == 448 || G == 480 ||
+           G == 576 || G == 640;
 }
 
 template <bool FWD, int Q, int B, int T = 16>
@@ -1817,10 +1850,21 @@ static bool factorize(int G, std::vector<int>& rad) {
     return g == 1 && (int)rad.size() <= 16;
 }
 
-bool pf_fftk_supported(const pf_plan* p) {
+// the error of a pass that needs the Stockham kernels on an axis without a radix set
+constexpr const char* kNoStockham =
+    "this grid size (a factor 7) runs only the register FFT passes, which take at most 16 field-map segments: "
+    "use a segment batch size of at most 16";
+
+// an axis the fused passes can transform: a Stockham radix set (8, 5, 4, 3, 2),
+// or a register four-step size (the 7-smooth ones have no Stockham kernel)
+static bool fftk_axis_ok(int G) {
     std::vector<int> rad;
+    return factorize(G, rad) || (PF_FFTQ && (colq_supported(G) || colq8_supported(G)));
+}
+
+bool pf_fftk_supported(const pf_plan* p) {
     for (int a = 3 - p->ndim; a < 3; ++a) {
-        if (p->os[a] > GBMAX || !factorize(p->os[a], rad)) return false;
+        if (p->os[a] > GBMAX || !fftk_axis_ok(p->os[a])) return false;
         if (p->shape[a] > p->os[a]) return false;
     }
     return true;
@@ -1834,8 +1878,11 @@ int pf_fftk_init(pf_plan* p) {
         X.G = p->os[a];
         X.N = p->shape[a];
         if (!factorize(X.G, X.rad)) {
-            pf_fftk_destroy(f);
-            return pf_fail(PF_ERR_UNSUPPORTED, "grid size has prime factors other than 2, 3, 5");
+            X.rad.clear();  // register passes only (a factor 7)
+            if (!fftk_axis_ok(X.G)) {
+                pf_fftk_destroy(f);
+                return pf_fail(PF_ERR_UNSUPPORTED, "grid size has prime factors other than 2, 3, 5");
+            }
         }
         std::vector<float2> h(X.G);
         for (int k = 0; k < X.G; ++k) {
@@ -1991,6 +2038,7 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
             switch (X.G / 8) {
                 PF_XQ8_CASE(15)
                 PF_XQ8_CASE(25)
+                PF_XQ8_CASE(35)
                 PF_XQ8_CASE(45)
                 default:
                     return pf_fail(PF_ERR_FFT, "no register row pass for this size");
@@ -2023,15 +2071,19 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         switch (X.G / 16) {
             PF_XQ_CASE(5)
             PF_XQ_CASE(6)
+            PF_XQ_CASE(7)
             PF_XQ_CASE(8)
             PF_XQ_CASE(10)
             PF_XQ_CASE(12)
+            PF_XQ_CASE(14)
             PF_XQ_CASE(15)
             PF_XQ_CASE(20)
+            PF_XQ_CASE(21)
             PF_XQ_CASE(24)
             PF_XQ_CASE(9)
             PF_XQ_CASE(18)
             PF_XQ_CASE(25)
+            PF_XQ_CASE(28)
             PF_XQ_CASE(30)
             PF_XQ_CASE(36)
             PF_XQ_CASE(40)
@@ -2070,6 +2122,8 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
+    if (X.rad.empty())
+        return pf_fail(PF_ERR_UNSUPPORTED, kNoStockham);
     AxisFFT A = axis_desc(X);
     if ((int64_t)b.L * X.G > GBMAX)
         return pf_fail(PF_ERR_UNSUPPORTED, "too many B0 segments for the fused FFT at this grid size");
@@ -2149,6 +2203,7 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
         switch (X.G / 8) {
             PF_COLQ8_CASE(15)
             PF_COLQ8_CASE(25)
+            PF_COLQ8_CASE(35)
             PF_COLQ8_CASE(45)
             default:
                 return pf_fail(PF_ERR_FFT, "no register column pass for this size");
@@ -2173,15 +2228,19 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
         switch (Q) {
             PF_COLQ_CASE(5)
             PF_COLQ_CASE(6)
+            PF_COLQ_CASE(7)
             PF_COLQ_CASE(8)
             PF_COLQ_CASE(10)
             PF_COLQ_CASE(12)
+            PF_COLQ_CASE(14)
             PF_COLQ_CASE(15)
             PF_COLQ_CASE(20)
+            PF_COLQ_CASE(21)
             PF_COLQ_CASE(24)
             PF_COLQ_CASE(9)
             PF_COLQ_CASE(18)
             PF_COLQ_CASE(25)
+            PF_COLQ_CASE(28)
             PF_COLQ_CASE(30)
             PF_COLQ_CASE(36)
             PF_COLQ_CASE(40)
@@ -2193,6 +2252,8 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
         return PF_OK;
     }
     if (cm.mode != 0) return pf_fail(PF_ERR_UNSUPPORTED, "column mask on an LDS FFT pass");
+    if (X.rad.empty())
+        return pf_fail(PF_ERR_UNSUPPORTED, kNoStockham);
     AxisFFT A = axis_desc(X);
     const int B = cols_for(X.G);
     int bshift = 0;
@@ -2293,6 +2354,7 @@ static int cart_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, 
         switch (X.G / 8) {
             PF_CCQ8_CASE(15)
             PF_CCQ8_CASE(25)
+            PF_CCQ8_CASE(35)
             PF_CCQ8_CASE(45)
             default:
                 return pf_fail(PF_ERR_FFT, "no register Cartesian column pass for this size");
@@ -2317,15 +2379,19 @@ static int cart_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, 
         switch (X.G / 16) {
             PF_CCQ_CASE(5)
             PF_CCQ_CASE(6)
+            PF_CCQ_CASE(7)
             PF_CCQ_CASE(8)
             PF_CCQ_CASE(9)
             PF_CCQ_CASE(10)
             PF_CCQ_CASE(12)
+            PF_CCQ_CASE(14)
             PF_CCQ_CASE(15)
             PF_CCQ_CASE(18)
             PF_CCQ_CASE(20)
+            PF_CCQ_CASE(21)
             PF_CCQ_CASE(24)
             PF_CCQ_CASE(25)
+            PF_CCQ_CASE(28)
             PF_CCQ_CASE(30)
             PF_CCQ_CASE(32)
             PF_CCQ_CASE(36)
@@ -2337,6 +2403,8 @@ static int cart_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, 
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
+    if (X.rad.empty())
+        return pf_fail(PF_ERR_UNSUPPORTED, kNoStockham);
     AxisFFT A = axis_desc(X);
     const int B = cols_for(X.G);
     int bshift = 0;
@@ -2393,6 +2461,7 @@ static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src
         switch (X.G / 8) {
             PF_CRQ8_CASE(15)
             PF_CRQ8_CASE(25)
+            PF_CRQ8_CASE(35)
             PF_CRQ8_CASE(45)
             default:
                 return pf_fail(PF_ERR_FFT, "no register Cartesian row pass for this size");
@@ -2416,15 +2485,19 @@ static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src
         switch (X.G / 16) {
             PF_CRQ_CASE(5)
             PF_CRQ_CASE(6)
+            PF_CRQ_CASE(7)
             PF_CRQ_CASE(8)
             PF_CRQ_CASE(9)
             PF_CRQ_CASE(10)
             PF_CRQ_CASE(12)
+            PF_CRQ_CASE(14)
             PF_CRQ_CASE(15)
             PF_CRQ_CASE(18)
             PF_CRQ_CASE(20)
+            PF_CRQ_CASE(21)
             PF_CRQ_CASE(24)
             PF_CRQ_CASE(25)
+            PF_CRQ_CASE(28)
             PF_CRQ_CASE(30)
             PF_CRQ_CASE(32)
             PF_CRQ_CASE(36)
@@ -2436,6 +2509,8 @@ static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src
         PF_HIP(hipGetLastError());
         return PF_OK;
     }
+    if (X.rad.empty())
+        return pf_fail(PF_ERR_UNSUPPORTED, kNoStockham);
     AxisFFT A = axis_desc(X);
     const int B = cols_for(X.G);
     const int64_t nblk = (nrows + B - 1) / B;

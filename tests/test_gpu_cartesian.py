@@ -62,6 +62,9 @@ CASES = [
     ("q_80_96", (80, 96), 3, 1, 1, 0, "ortho"),
     ("q8_120_200", (120, 200), 2, 1, 1, 0, "ortho"),
     ("q8_360_3d", (360, 12, 120), 2, 2, 2, 0, None),
+    ("q7_112_224", (112, 224), 2, 1, 1, 0, "ortho"),
+    ("q7_336_280_448", (336, 280, 16), 1, 1, 1, 0, None),
+    ("q7_448", (448, 20), 2, 1, 0, 0, None),
 ]
 
 

@@ -153,6 +153,31 @@ def test_b0_segments(cuda_device):
     assert rel(y2.cpu().numpy(), y.cpu().numpy()) < 1e-6
 
 
+def test_b0_segments_on_a_7_smooth_grid(cuda_device):
+    """224-point rows (2^5 x 7) have register FFT passes only: up to 16
+    segments per call run fused; 20 segments raise a clear error unless the
+    segments are batched (zmap_batch_size), which matches the oracle."""
+    rng = np.random.default_rng(9)
+    N, C, L = 112, 2, 20
+    coord = demo.variable_density_spiral_2d(4, 512, N)
+    zmap = demo.smooth_field_map(rng, (N, N), 100.0)
+    t = np.linspace(0, 8e-3, 512).astype(np.float32)
+    plan = pf.plan_nufft(torch.from_numpy(coord), N, width=4, oversamp=2.0, zmap=torch.from_numpy(zmap),
+                         T=torch.from_numpy(t), L=L, nbins=40)
+    assert plan.interpolator.os_shape == (224, 224)
+    b0 = (plan.zmap_t_kernel.cpu().numpy().astype(np.complex128), plan.zmap_s_kernel.cpu().numpy().astype(np.complex128))
+    P = R.Plan(coord, N, width=4, oversamp=2.0)
+    x = demo.complex_gaussian(rng, (C, N, N))
+    with pytest.raises(RuntimeError, match="segment batch size"):
+        pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    plan.zmap_batch_size = 10
+    y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    assert rel(y.cpu().numpy().reshape(C, 1, -1), R.nufft(x[:, None], P, b0=b0)) < PARITY
+    ya = demo.complex_gaussian(rng, tuple(y.shape))
+    xa = pf.nufft_adj(torch.from_numpy(ya).to(cuda_device), nufft_plan=plan, device=cuda_device)
+    assert rel(xa.cpu().numpy().reshape(C, 1, N, N), R.nufft_adj(ya.reshape(C, 1, -1), P, b0=b0)) < PARITY
+
+
 def test_weights_real_and_numpy(cuda_device):
     rng = np.random.default_rng(8)
     N = 40

@@ -81,6 +81,14 @@ CASES = [
     ((10, 96), 1.25),      # 13 x 120: rows
     ((8, 160), 1.25),      # 10 x 200
     ((6, 180), 2.0),       # 12 x 360
+    ((56, 20), 2.0),       # 112 x 40: 16 x 7 (7-point DFT)
+    ((112, 12), 2.0),      # 224 x 24: 16 x 14 (2 x 7)
+    ((168, 10), 2.0),      # 336 x 20: 16 x 21 (3 x 7)
+    ((224, 8), 2.0),       # 448 x 16: 16 x 28 (4 x 7)
+    ((224, 10), 1.25),     # 280 x 13: 8 x 35 (5 x 7)
+    ((10, 112), 2.0),      # 20 x 224: rows
+    ((6, 224), 1.25),      # 8 x 280
+    ((6, 224), 2.0),       # 12 x 448
 ]
 # (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
 # values per grid point (second-stage lanes m1-fastest) and with 4
