@@ -15,11 +15,12 @@
 //   conjugate maps) in the user layout.
 //
 // Lines of G = 16 Q points (Q in {5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 18, 20,
-// 21, 24, 25, 28, 30, 32, 36, 40}: 80-640) and of 120, 200, 280 and 360
-// points (8 threads x 15, 25, 35, 45) run register four-step kernels: each thread holds Q (or 16)
-// values of a line, takes a small DFT in registers, twiddles, and one LDS
-// transpose hands the line to the second small DFT (k_fftcol256/512,
-// k_fftcolq, k_fftx256, k_fftx512_*, k_fftxq).  Other sizes: each block
+// 21, 24, 25, 28, 30, 32, 36, 40}: 80-640) and of G = 8 Q points (Q in {5,
+// 7, 9, 15, 21, 25, 35, 45}: 40-360) run register four-step kernels: each
+// thread holds Q values of a line, takes a small DFT in registers, twiddles,
+// and one LDS transpose hands the line to the second small DFT over the 16
+// or 8 threads (k_fftcol256/512, k_fftcolq, k_fftx256, k_fftx512_*,
+// k_fftxq).  Other sizes: each block
 // transforms B columns of one line family held in LDS as [G][B+1] complex,
 // with an in-place mixed-radix (8, 5, 4, 3, 2) Stockham FFT (Govindaraju et
 // al., "High performance discrete Fourier transforms on graphics
@@ -926,8 +927,10 @@ __device__ __forceinline__ void dftT(float2* v) {
 
 constexpr int CQ_B = 32;  // columns per block (512 threads)
 
-// G = 8 Q (Q = 15, 25, 35, 45) with register passes of 8 threads per line
-static inline bool colq8_supported(int G) { return G == 120 || G == 200 || G == 280 || G == 360; }
+// G = 8 Q (Q = 5, 7, 9, 15, 21, 25, 35, 45) with register passes of 8 threads per line
+static inline bool colq8_supported(int G) {
+    return G == 40 || G == 56 || G == 72 || G == 120 || G == 168 || G == 200 || G == 280 || G == 360;
+}
 
 // G with a register column pass (16 Q, Q as above and 30, 40; 256 and 512 have their own)
 static inline bool colq_supported(int G) {
@@ -2036,7 +2039,11 @@ static int launch_x(const pf_plan* p, const pf_batch& b, bool fwd, const float2*
                                p->apod, src, dst, KC, YG, nkc, factor);                                         \
         break;
             switch (X.G / 8) {
+                PF_XQ8_CASE(5)
+                PF_XQ8_CASE(7)
+                PF_XQ8_CASE(9)
                 PF_XQ8_CASE(15)
+                PF_XQ8_CASE(21)
                 PF_XQ8_CASE(25)
                 PF_XQ8_CASE(35)
                 PF_XQ8_CASE(45)
@@ -2201,7 +2208,11 @@ static int launch_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I
             hipLaunchKernelGGL((k_fftcolq<false, QV, CQ_B, 8>), grid, blk, 0, s, X.N, I, X.tw, src, dst, cm); \
         break;
         switch (X.G / 8) {
+            PF_COLQ8_CASE(5)
+            PF_COLQ8_CASE(7)
+            PF_COLQ8_CASE(9)
             PF_COLQ8_CASE(15)
+            PF_COLQ8_CASE(21)
             PF_COLQ8_CASE(25)
             PF_COLQ8_CASE(35)
             PF_COLQ8_CASE(45)
@@ -2352,7 +2363,11 @@ static int cart_col(const pf_plan* p, int axis, bool fwd, int64_t O, int64_t I, 
             hipLaunchKernelGGL((k_cartcolq<false, QV, B, 8>), grid, blk, 0, s, I, X.tw, src, dst, e);        \
         break;
         switch (X.G / 8) {
+            PF_CCQ8_CASE(5)
+            PF_CCQ8_CASE(7)
+            PF_CCQ8_CASE(9)
             PF_CCQ8_CASE(15)
+            PF_CCQ8_CASE(21)
             PF_CCQ8_CASE(25)
             PF_CCQ8_CASE(35)
             PF_CCQ8_CASE(45)
@@ -2459,7 +2474,11 @@ static int cart_row(const pf_plan* p, bool fwd, int64_t nrows, const float2* src
             hipLaunchKernelGGL((k_cartrowq<false, QV, 8>), grid, blk, 0, s, nrows, X.tw, src, dst, e);       \
         break;
         switch (X.G / 8) {
+            PF_CRQ8_CASE(5)
+            PF_CRQ8_CASE(7)
+            PF_CRQ8_CASE(9)
             PF_CRQ8_CASE(15)
+            PF_CRQ8_CASE(21)
             PF_CRQ8_CASE(25)
             PF_CRQ8_CASE(35)
             PF_CRQ8_CASE(45)

@@ -65,6 +65,8 @@ CASES = [
     ("q7_112_224", (112, 224), 2, 1, 1, 0, "ortho"),
     ("q7_336_280_448", (336, 280, 16), 1, 1, 1, 0, None),
     ("q7_448", (448, 20), 2, 1, 0, 0, None),
+    ("q8_small", (40, 56, 72), 2, 1, 1, 0, "ortho"),
+    ("q8_168", (168, 168), 2, 1, 1, 0, None),
 ]
 
 

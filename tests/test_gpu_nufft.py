@@ -73,7 +73,7 @@ def test_fwd_adj_3d_multicoil_parity(cuda_device):
                                            ((384, 12, 10), 1.25), ((512, 12, 10), 1.25), ((320, 12, 10), 1.25),
                                            ((144, 12, 10), 2.0), ((64, 12, 10), 1.25), ((48, 12, 10), 2.0),
                                            ((96, 12, 10), 1.25), ((160, 12, 10), 1.25), ((180, 12, 10), 2.0),
-                                           ((112, 12, 10), 2.0), ((224, 12, 10), 1.25)])
+                                           ((112, 12, 10), 2.0), ((224, 12, 10), 1.25), ((32, 32, 32), 1.25)])
 def test_fwd_adj_3d_register_z_pass(cuda_device, shape, oversamp):
     """3-D plans whose z axis (160, 192, 240, 320, 384, 128, 480, 640, 400, 288, 80, 96, 120,
     200, 360, 224, 280 points) runs the

@@ -89,6 +89,13 @@ CASES = [
     ((10, 112), 2.0),      # 20 x 224: rows
     ((6, 224), 1.25),      # 8 x 280
     ((6, 224), 2.0),       # 12 x 448
+    ((32, 20), 1.25),      # 40 x 25: 8 x 5
+    ((28, 10, 8), 2.0),    # 56 x 20 x 16: 8 x 7
+    ((36, 20), 2.0),       # 72 x 40: 8 x 9
+    ((84, 12), 2.0),       # 168 x 24: 8 x 21
+    ((10, 32), 1.25),      # 13 x 40: rows
+    ((12, 36), 2.0),       # 24 x 72
+    ((6, 84), 2.0),        # 12 x 168
 ]
 # (shape, oversamp, coils): 3 coils, plus 512-point row passes with 1 or 2
 # values per grid point (second-stage lanes m1-fastest) and with 4
