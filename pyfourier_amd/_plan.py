@@ -280,7 +280,9 @@ def plan_nufft(
             T = dt * torch.arange(coord.shape[-2], dtype=torch.float32)
         zt, zs = mri_exp_approx(zmap, T, L, nbins)
         if L_batch_size is None:
-            L_batch_size = L
+            # an x grid with a factor 7 has register FFT passes only, which
+            # take at most 16 segments per call (batching segments is exact)
+            L_batch_size = min(L, 16) if int(os_shape[-1]) % 7 == 0 else L
     else:
         zt, zs = None, None
     return NUFFTPlan(

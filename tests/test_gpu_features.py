@@ -155,8 +155,9 @@ def test_b0_segments(cuda_device):
 
 def test_b0_segments_on_a_7_smooth_grid(cuda_device):
     """224-point rows (2^5 x 7) have register FFT passes only: up to 16
-    segments per call run fused; 20 segments raise a clear error unless the
-    segments are batched (zmap_batch_size), which matches the oracle."""
+    segments per call run fused, so the plan batches 20 segments by 16 (the
+    operator matches the oracle); an explicit batch of 20 raises a clear
+    error."""
     rng = np.random.default_rng(9)
     N, C, L = 112, 2, 20
     coord = demo.variable_density_spiral_2d(4, 512, N)
@@ -168,9 +169,11 @@ def test_b0_segments_on_a_7_smooth_grid(cuda_device):
     b0 = (plan.zmap_t_kernel.cpu().numpy().astype(np.complex128), plan.zmap_s_kernel.cpu().numpy().astype(np.complex128))
     P = R.Plan(coord, N, width=4, oversamp=2.0)
     x = demo.complex_gaussian(rng, (C, N, N))
+    assert plan.zmap_batch_size == 16
+    plan.zmap_batch_size = 20
     with pytest.raises(RuntimeError, match="segment batch size"):
         pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, device=cuda_device)
-    plan.zmap_batch_size = 10
+    plan.zmap_batch_size = 16
     y = pf.nufft(torch.from_numpy(x).to(cuda_device), nufft_plan=plan, device=cuda_device)
     assert rel(y.cpu().numpy().reshape(C, 1, -1), R.nufft(x[:, None], P, b0=b0)) < PARITY
     ya = demo.complex_gaussian(rng, tuple(y.shape))
