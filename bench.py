@@ -853,10 +853,52 @@ def bench_default_params(pf, dev, steps):
     return out
 
 
+def bench_fft_sizes(pf, dev, steps):
+    """Fused fwd+adj (drop-in calls, wall clock) and the pad+FFT / IFFT+crop
+    stages (HIP events) on grids whose FFT lines run the register four-step
+    passes of 16 x Q or 8 x Q points (160, 200, 224, 320, 400, 480, 640) and
+    the 256-point ones: random coordinates (2^18 per 2-D/3-D plan), complex
+    Gaussian images, width 4."""
+    import torch
+
+    from pyfourier_amd._nufft import stage_timer
+
+    cases = [((256, 256), 32, 1.25), ((320, 320), 32, 1.25), ((384, 384), 32, 1.25), ((512, 512), 32, 1.25),
+             ((160, 160, 160), 8, 1.25), ((112, 112, 112), 16, 2.0), ((160, 160, 160), 8, 2.0),
+             ((128, 128, 128), 8, 2.0)]
+    out = {}
+    for shape, C, os_ in cases:
+        d = len(shape)
+        rng = np.random.default_rng(1)
+        coord = torch.from_numpy(rng.uniform(-np.pi, np.pi, (1 << 18, d)).astype(np.float32))
+        plan = pf.plan_nufft(coord, shape, width=4, oversamp=os_).to(dev)
+        g = torch.Generator(device=dev)
+        g.manual_seed(1)
+        x = torch.randn((C,) + shape, dtype=torch.complex64, device=dev, generator=g)
+
+        def step():
+            return pf.nufft_adj(pf.nufft(x, nufft_plan=plan, device=dev), nufft_plan=plan, device=dev)
+
+        ms = _time(step, dev, max(5, steps), warmup=2)
+        with stage_timer() as t:
+            for _ in range(3):
+                step()
+            torch.cuda.synchronize(dev)
+        st = t.summary()
+        key = "x".join(str(n) for n in shape) + f"_{C}coil_os{os_}"
+        out[key] = {"grid": list(plan.interpolator.os_shape), "fwd_adj_ms": round(ms, 4),
+                    "pad_fft_ms": round(float(np.mean(st["pad_fft"])), 4),
+                    "ifft_crop_ms": round(float(np.mean(st["ifft_crop"])), 4)}
+        del plan, x
+        torch.cuda.empty_cache()
+    return out
+
+
 EXTRAS = (("cfg1_latency", bench_cfg1_latency), ("batched_uneven", bench_uneven_batch), ("batched_subspace", bench_subspace),
           ("autograd_b0", bench_autograd), ("autograd_3d", bench_autograd_3d), ("toeplitz", bench_toeplitz),
           ("cartesian", bench_cartesian),
-          ("grog_sparse", bench_grog_sparse), ("voronoi", bench_voronoi), ("default_params", bench_default_params))
+          ("grog_sparse", bench_grog_sparse), ("voronoi", bench_voronoi), ("default_params", bench_default_params),
+          ("fft_sizes", bench_fft_sizes))
 
 
 def main():
